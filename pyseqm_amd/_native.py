@@ -1,0 +1,102 @@
+"""ctypes binding of libpyseqm_hip.so (the C ABI declared in include/pyseqm_hip.h).
+
+The library is loaded from this package directory only.  There is no fallback: if it is
+missing or the device is not a HIP GPU, every compute entry point raises.
+"""
+import ctypes
+import os
+
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpyseqm_hip.so")
+
+c_int, c_double, c_void_p, c_longlong = ctypes.c_int, ctypes.c_double, ctypes.c_void_p, ctypes.c_longlong
+
+NPAR = 32
+METHODS = {"MNDO": 0, "AM1": 1, "PM3": 2}
+
+
+class Layout(ctypes.Structure):
+    _fields_ = [
+        ("nmol", c_int), ("natoms", c_int), ("ntile", c_int * 3), ("npairs", c_int),
+        ("mol_atom_off", c_void_p), ("mol_norb", c_void_p), ("mol_nocc", c_void_p),
+        ("mol_mat_off", c_void_p), ("mol_tile_off", c_void_p), ("atom_mol", c_void_p),
+        ("atom_ao", c_void_p), ("atom_Z", c_void_p), ("tiles", c_void_p), ("csr_off", c_void_p),
+        ("csr_idx", c_void_p),
+    ]
+
+
+_P = ctypes.POINTER(Layout)
+# name -> argtypes (after the layout pointer); every function returns int status
+_SIGS = {
+    "pyseqm_atom_setup": [c_void_p, c_void_p],
+    "pyseqm_pair_integrals": [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                              c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
+    "pyseqm_fock": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                    c_void_p, c_void_p, c_void_p, c_void_p],
+    "pyseqm_pad_for_eigh": [c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p],
+    "pyseqm_density_from_evecs": [c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p],
+    "pyseqm_mix": [c_void_p, c_int, c_int, c_double, c_void_p, c_void_p, c_void_p, c_void_p],
+    "pyseqm_copy_active": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p],
+    "pyseqm_scf_check": [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_void_p,
+                         c_void_p, c_void_p],
+    "pyseqm_gemm": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p],
+    "pyseqm_diis_residual": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
+    "pyseqm_diis_dots": [c_void_p, c_int, c_void_p, c_longlong, c_int, c_int, c_int, c_void_p, c_void_p],
+    "pyseqm_diis_combine": [c_void_p, c_int, c_int, c_void_p, c_longlong, c_int, c_void_p, c_int, c_void_p,
+                            c_void_p],
+}
+_OPTIONAL = {}
+_lib = None
+
+
+class NativeError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load the native library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise NativeError(f"{LIB_PATH} not found: build it with `python -m pyseqm_amd.csrc.build` "
+                              "(or __graft_entry__.build()); there is no CPU fallback")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, args in _SIGS.items():
+            f = getattr(L, name)
+            f.argtypes = [_P] + args
+            f.restype = c_int
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    return list(_SIGS)
+
+
+def stream_ptr(device=None):
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def ptr(t):
+    """Device pointer of a tensor (None -> NULL)."""
+    if t is None:
+        return None
+    return ctypes.c_void_p(t.data_ptr())
+
+
+_CODES = {-1: "invalid argument", -2: "kernel launch failed", -3: "unsupported element/method"}
+
+
+def call(name, layout, *args):
+    rc = getattr(lib(), name)(ctypes.byref(layout), *args)
+    if rc != 0:
+        raise NativeError(f"{name} failed: {_CODES.get(rc, rc)}")
+
+
+def require_device(device):
+    dev = torch.device(device)
+    if dev.type != "cuda" or not torch.cuda.is_available():
+        raise NativeError("pyseqm_amd runs on a HIP device (MI355X); got device " + str(dev))
+    return dev

@@ -1,0 +1,81 @@
+// Internal helpers shared by the gfx950 kernels (not part of the C ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "../../include/pyseqm_hip.h"
+
+namespace pyseqm {
+
+constexpr double kEV = 27.21;         // eV per hartree used by MOPAC7 / PYSEQM (constants.py:4)
+constexpr double kA0 = 0.529167;      // bohr radius in Angstrom (constants.py:9)
+constexpr double kOverlapCut = 40.0;  // bohr (constants.py:23)
+constexpr int kTile = 8;              // pair tile edge: one 64-lane wave per 8 x 8 tile
+constexpr int kPartStride = 10;       // doubles per tile partial slot
+
+// tril pair (mu >= nu) of the 4 sp orbitals, index = mu*(mu+1)/2 + nu
+__device__ constexpr int kTriI[10] = {0, 1, 1, 2, 2, 2, 3, 3, 3, 3};
+__device__ constexpr int kTriJ[10] = {0, 0, 1, 0, 1, 2, 0, 1, 2, 3};
+__host__ __device__ constexpr int tri_index(int a, int b) {
+  return a >= b ? a * (a + 1) / 2 + b : b * (b + 1) / 2 + a;
+}
+
+struct Tile {
+  int mol, row0, col0, nrow, ncol, diag;
+};
+
+__device__ inline Tile load_tile(const int* tiles, int t) {
+  const int4 v = reinterpret_cast<const int4*>(tiles)[t];
+  Tile d;
+  d.mol = v.x;
+  d.row0 = v.y;
+  d.col0 = v.z;
+  d.nrow = v.w & 0xff;
+  d.ncol = (v.w >> 8) & 0xff;
+  d.diag = (v.w >> 16) & 1;
+  return d;
+}
+
+// Lane mapping inside a tile: lane = 8*r + c  <->  pair (row0 + r, col0 + c).
+__device__ inline bool lane_pair(const Tile& t, int lane, int& i, int& j) {
+  const int r = lane >> 3, c = lane & 7;
+  i = t.row0 + r;
+  j = t.col0 + c;
+  return r < t.nrow && c < t.ncol && (!t.diag || c > r);
+}
+
+// Sum over the 8 lanes sharing a row r (lanes 8r..8r+7) / a column c (lanes c, c+8, ...).
+__device__ inline double row_sum(double v) {
+  v += __shfl_xor(v, 1);
+  v += __shfl_xor(v, 2);
+  v += __shfl_xor(v, 4);
+  return v;
+}
+__device__ inline double col_sum(double v) {
+  v += __shfl_xor(v, 8);
+  v += __shfl_xor(v, 16);
+  v += __shfl_xor(v, 32);
+  return v;
+}
+__device__ inline double wave_sum(double v) { return col_sum(row_sum(v)); }
+
+// Write the row and column partial sums of a 10-vector held per lane.
+// Slot layout: part[(tile*16 + s)*10 + k], s < 8 rows, s >= 8 columns.
+template <int NR, int NC>
+__device__ inline void write_partials(double* part, int tile, int lane, const double (&rowv)[NR],
+                                      const double (&colv)[NC]) {
+  const int r = lane >> 3, c = lane & 7;
+  double* prow = part + ((size_t)tile * 16 + r) * kPartStride;
+  double* pcol = part + ((size_t)tile * 16 + 8 + c) * kPartStride;
+#pragma unroll
+  for (int k = 0; k < NR; ++k) {
+    const double rs = row_sum(rowv[k]);
+    if (c == 0) prow[k] = rs;
+  }
+#pragma unroll
+  for (int k = 0; k < NC; ++k) {
+    const double cs = col_sum(colv[k]);
+    if (r == 0) pcol[k] = cs;
+  }
+}
+
+}  // namespace pyseqm

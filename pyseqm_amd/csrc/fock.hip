@@ -1,0 +1,221 @@
+// Fock build F = H + G(P) for the closed-shell NDDO Hamiltonian on gfx950, FP64.
+//
+// Pass 1 (one 64-lane wave per 8x8 pair tile, one lane per atom pair i<j):
+//   * exchange block  K_ij[mu][la] = -1/2 sum_{nu,si} P_ij[nu][si] (mu nu|la si)
+//     -> F_ij = H_ij + K_ij written to both triangles,
+//   * Coulomb terms  J_i += sum_{la si} P_jj (mu nu|la si),  J_j += sum_{mu nu} P_ii (..),
+//     reduced over the tile's rows / columns with wave shuffles into per-tile partials,
+//   * this pair's share of 0.5 sum P (H + F).
+// Pass 2 (one thread per atom): F_ii = H_ii + one-center G(P_ii) + the atom's partials in a fixed
+// order, and the atom's share of the energy.  Pass 3 sums the energy per molecule.
+// Follows fock.py:132-347 (one center 187-231, two center 259-347) and energy.py:26-49;
+// w is the compact, tile-major integral storage written by pairs.hip.
+#include "common.h"
+
+using namespace pyseqm;
+
+namespace pyseqm {
+void launch_mol_sum(const pyseqm_layout_t* L, const double* tile_v, const double* atom_v,
+                    const unsigned char* active, double* out, hipStream_t s);
+}
+
+namespace {
+
+enum { XX = 0, XH = 1, HH = 2 };
+
+template <int KIND>
+__global__ __launch_bounds__(64) void k_fock_tiles(pyseqm_layout_t L, int tile_base, const double* __restrict__ w,
+                                                    const double* __restrict__ P, const double* __restrict__ H,
+                                                    double* __restrict__ F, double* __restrict__ part,
+                                                    double* __restrict__ etile,
+                                                    const unsigned char* __restrict__ active) {
+  const int tl = blockIdx.x;
+  const int tile = tile_base + tl;
+  const int lane = threadIdx.x;
+  const Tile t = load_tile(L.tiles, tile);
+  if (active && !active[t.mol]) return;  // uniform over the wave
+  int i, j;
+  const bool valid = lane_pair(t, lane, i, j);
+  constexpr int NR = KIND == HH ? 1 : 10;
+  constexpr int NC = KIND == XX ? 10 : 1;
+  double rowv[NR], colv[NC];
+#pragma unroll
+  for (int k = 0; k < NR; ++k) rowv[k] = 0.0;
+#pragma unroll
+  for (int k = 0; k < NC; ++k) colv[k] = 0.0;
+  double e = 0.0;
+  if (valid) {
+    const long long mo = L.mol_mat_off[t.mol];
+    const int n = L.mol_norb[t.mol];
+    const int ai = L.atom_ao[i], aj = L.atom_ao[j];
+    const double* Pm = P + mo;
+    const double* Hm = H + mo;
+    double* Fm = F + mo;
+    if constexpr (KIND == XX) {
+      const double* wt = w + (size_t)tl * 6400 + lane;
+      double pa[10], pb[10], pab[4][4];
+#pragma unroll
+      for (int k = 0; k < 10; ++k) {
+        const int mu = kTriI[k], nu = kTriJ[k];
+        const double f = mu == nu ? 1.0 : 2.0;
+        pa[k] = f * Pm[(size_t)(ai + mu) * n + ai + nu];
+        pb[k] = f * Pm[(size_t)(aj + mu) * n + aj + nu];
+      }
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) pab[a][b] = Pm[(size_t)(ai + a) * n + aj + b];
+      double K[4][4];
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) K[a][b] = 0.0;
+#pragma unroll
+      for (int k = 0; k < 10; ++k) {
+        const int mu = kTriI[k], nu = kTriJ[k];
+#pragma unroll
+        for (int l = 0; l < 10; ++l) {
+          const int la = kTriI[l], si = kTriJ[l];
+          const double v = wt[(size_t)(k * 10 + l) * 64];
+          colv[l] += v * pa[k];
+          rowv[k] += v * pb[l];
+          K[mu][la] += v * pab[nu][si];
+          if (mu != nu) K[nu][la] += v * pab[mu][si];
+          if (la != si) K[mu][si] += v * pab[nu][la];
+          if (mu != nu && la != si) K[nu][si] += v * pab[mu][la];
+        }
+      }
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          const double h = Hm[(size_t)(ai + a) * n + aj + b];
+          const double f = h - 0.5 * K[a][b];
+          Fm[(size_t)(ai + a) * n + aj + b] = f;
+          Fm[(size_t)(aj + b) * n + ai + a] = f;
+          e += pab[a][b] * (h + f);
+        }
+    } else if constexpr (KIND == XH) {
+      const double* wt = w + (size_t)tl * 640 + lane;
+      double pa[10], pab[4];
+#pragma unroll
+      for (int k = 0; k < 10; ++k) {
+        const int mu = kTriI[k], nu = kTriJ[k];
+        pa[k] = (mu == nu ? 1.0 : 2.0) * Pm[(size_t)(ai + mu) * n + ai + nu];
+      }
+      const double pb = Pm[(size_t)aj * n + aj];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) pab[a] = Pm[(size_t)(ai + a) * n + aj];
+      double K[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int k = 0; k < 10; ++k) {
+        const int mu = kTriI[k], nu = kTriJ[k];
+        const double v = wt[(size_t)k * 64];
+        colv[0] += v * pa[k];
+        rowv[k] = v * pb;
+        K[mu] += v * pab[nu];
+        if (mu != nu) K[nu] += v * pab[mu];
+      }
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        const double h = Hm[(size_t)(ai + a) * n + aj];
+        const double f = h - 0.5 * K[a];
+        Fm[(size_t)(ai + a) * n + aj] = f;
+        Fm[(size_t)aj * n + ai + a] = f;
+        e += pab[a] * (h + f);
+      }
+    } else {
+      const double v = w[(size_t)tl * 64 + lane];
+      const double pij = Pm[(size_t)ai * n + aj];
+      rowv[0] = v * Pm[(size_t)aj * n + aj];
+      colv[0] = v * Pm[(size_t)ai * n + ai];
+      const double h = Hm[(size_t)ai * n + aj];
+      const double f = h - 0.5 * v * pij;
+      Fm[(size_t)ai * n + aj] = f;
+      Fm[(size_t)aj * n + ai] = f;
+      e = pij * (h + f);
+    }
+  }
+  write_partials(part, tile, lane, rowv, colv);
+  const double es = wave_sum(e);
+  if (lane == 0) etile[tile] = es;
+}
+
+// Diagonal blocks: H_ii + one-center G (fock.py:196-224) + Coulomb partials; energy share.
+__global__ void k_fock_diag(pyseqm_layout_t L, const double* __restrict__ par, const double* __restrict__ P,
+                            const double* __restrict__ H, const double* __restrict__ part, double* __restrict__ F,
+                            double* __restrict__ eatom, const unsigned char* __restrict__ active) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= L.natoms) return;
+  const int m = L.atom_mol[a];
+  if (active && !active[m]) return;
+  const int n = L.mol_norb[m];
+  const long long mo = L.mol_mat_off[m];
+  const double* Pm = P + mo;
+  const double* Hm = H + mo;
+  double* Fm = F + mo;
+  const int ao = L.atom_ao[a];
+  const double* p = par + (size_t)a * PYSEQM_NPAR;
+  if (L.atom_Z[a] > 1) {
+    double acc[10], pd[10], hd[10];
+#pragma unroll
+    for (int k = 0; k < 10; ++k) {
+      pd[k] = Pm[(size_t)(ao + kTriI[k]) * n + ao + kTriJ[k]];
+      hd[k] = Hm[(size_t)(ao + kTriI[k]) * n + ao + kTriJ[k]];
+      acc[k] = hd[k];
+    }
+    for (int e = L.csr_off[a]; e < L.csr_off[a + 1]; ++e) {
+      const double* src = part + (size_t)L.csr_idx[e] * kPartStride;
+#pragma unroll
+      for (int k = 0; k < 10; ++k) acc[k] += src[k];
+    }
+    const double gss = p[PAR_GSS], gsp = p[PAR_GSP], gpp = p[PAR_GPP], gp2 = p[PAR_GP2], hsp = p[PAR_HSP];
+    const double pss = pd[0], ppt = pd[2] + pd[5] + pd[9];
+    const double f1 = gsp - 0.5 * hsp, f2 = 1.5 * hsp - 0.5 * gsp;
+    const double fd = 1.25 * gp2 - 0.25 * gpp, fo = 0.75 * gpp - 1.25 * gp2;
+    acc[0] += 0.5 * pss * gss + ppt * f1;
+    acc[2] += pss * f1 + 0.5 * pd[2] * gpp + (ppt - pd[2]) * fd;
+    acc[5] += pss * f1 + 0.5 * pd[5] * gpp + (ppt - pd[5]) * fd;
+    acc[9] += pss * f1 + 0.5 * pd[9] * gpp + (ppt - pd[9]) * fd;
+    acc[1] += pd[1] * f2;
+    acc[3] += pd[3] * f2;
+    acc[6] += pd[6] * f2;
+    acc[4] += pd[4] * fo;
+    acc[7] += pd[7] * fo;
+    acc[8] += pd[8] * fo;
+    double e = 0.0;
+#pragma unroll
+    for (int k = 0; k < 10; ++k) e += (kTriI[k] == kTriJ[k] ? 0.5 : 1.0) * pd[k] * (hd[k] + acc[k]);
+    eatom[a] = e;
+#pragma unroll
+    for (int mu = 0; mu < 4; ++mu)
+#pragma unroll
+      for (int nu = 0; nu < 4; ++nu) Fm[(size_t)(ao + mu) * n + ao + nu] = acc[tri_index(mu, nu)];
+  } else {
+    const double pd = Pm[(size_t)ao * n + ao], hd = Hm[(size_t)ao * n + ao];
+    double acc = hd + 0.5 * pd * p[PAR_GSS];
+    for (int e = L.csr_off[a]; e < L.csr_off[a + 1]; ++e) acc += part[(size_t)L.csr_idx[e] * kPartStride];
+    Fm[(size_t)ao * n + ao] = acc;
+    eatom[a] = 0.5 * pd * (hd + acc);
+  }
+}
+
+}  // namespace
+
+extern "C" int pyseqm_fock(const pyseqm_layout_t* L, const double* atom_par, const double* w_xx,
+                           const double* w_xh, const double* w_hh, const double* P, const double* H, double* F,
+                           double* part, double* etile, double* eatom, double* eelec,
+                           const unsigned char* active, void* stream) {
+  if (!L) return PYSEQM_EINVAL;
+  hipStream_t s = (hipStream_t)stream;
+  int base = 0;
+  if (L->ntile[0]) k_fock_tiles<XX><<<L->ntile[0], 64, 0, s>>>(*L, base, w_xx, P, H, F, part, etile, active);
+  base += L->ntile[0];
+  if (L->ntile[1]) k_fock_tiles<XH><<<L->ntile[1], 64, 0, s>>>(*L, base, w_xh, P, H, F, part, etile, active);
+  base += L->ntile[1];
+  if (L->ntile[2]) k_fock_tiles<HH><<<L->ntile[2], 64, 0, s>>>(*L, base, w_hh, P, H, F, part, etile, active);
+  if (L->natoms)
+    k_fock_diag<<<(L->natoms + 127) / 128, 128, 0, s>>>(*L, atom_par, P, H, part, F, eatom, active);
+  pyseqm::launch_mol_sum(L, etile, eatom, active, eelec, s);
+  return hipGetLastError() == hipSuccess ? PYSEQM_OK : PYSEQM_ELAUNCH;
+}

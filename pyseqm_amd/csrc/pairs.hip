@@ -1,0 +1,631 @@
+// Pair integrals for MNDO/AM1/PM3 sp atoms on gfx950.
+//
+// One 64-lane wave per 8x8 pair tile; each lane owns one atom pair and computes, in FP64:
+//   * the local-frame two-electron integrals (MOPAC repp point-multipole formulas,
+//     two_elec_two_center_int_local_frame.py:18-293),
+//   * their rotation to the molecular frame, w = T L T^T with T the 10x10 map from local to
+//     molecular orbital-product pairs (same rotation as two_elec_two_center_int.py:1384-1631),
+//   * the electron-core attractions e1b/e2a (two_elec_two_center_int.py:1533-1569),
+//   * the STO overlap block and the off-diagonal core Hamiltonian H_ij = S_ij (b_i + b_j)/2
+//     (diat_overlap_PM6_SP.py:6-444, hcore.py:155-173),
+//   * the core-core repulsion (energy.py:91-134).
+// Diagonal core-Hamiltonian blocks are summed per atom from per-tile row/column partials in a
+// fixed order (deterministic, no atomics).
+#include "common.h"
+
+using namespace pyseqm;
+
+namespace {
+
+__device__ constexpr double kTore[19] = {0, 1, 0, 1, 2, 3, 4, 5, 6, 7, 0, 1, 2, 3, 4, 5, 6, 7, 0};
+__device__ constexpr int kQn[19] = {0, 1, 1, 2, 2, 2, 2, 2, 2, 2, 2, 3, 3, 3, 3, 3, 3, 3, 3};
+
+// ---------------------------------------------------------------- per-atom setup
+// rho1 = 1/(2 ad): hsp = ad/2 - 1/(2 sqrt(4 D1^2 + 1/ad^2)) solved by 5 secant steps from the
+// closed-form start (cal_par.py:106-160).
+__device__ double secant(int which, double target, double D, double x1) {
+  double x2 = x1 + 0.04;
+  for (int it = 0; it < 5; ++it) {
+    double f1, f2;
+    if (which == 1) {
+      f1 = 0.5 * x1 - 0.5 / sqrt(4.0 * D * D + 1.0 / (x1 * x1));
+      f2 = 0.5 * x2 - 0.5 / sqrt(4.0 * D * D + 1.0 / (x2 * x2));
+    } else {
+      f1 = 0.25 * x1 - 0.5 / sqrt(4.0 * D * D + 1.0 / (x1 * x1)) + 0.25 / sqrt(8.0 * D * D + 1.0 / (x1 * x1));
+      f2 = 0.25 * x2 - 0.5 / sqrt(4.0 * D * D + 1.0 / (x2 * x2)) + 0.25 / sqrt(8.0 * D * D + 1.0 / (x2 * x2));
+    }
+    const double x3 = fabs(f2 - f1) > 1.0e-16 ? x1 + (x2 - x1) * (target - f1) / (f2 - f1) : x2;
+    x1 = x2;
+    x2 = x3;
+  }
+  return x2;
+}
+
+__global__ void k_atom_setup(pyseqm_layout_t L, double* par) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= L.natoms) return;
+  double* p = par + (size_t)a * PYSEQM_NPAR;
+  const int Z = L.atom_Z[a];
+  const double qn = kQn[Z];
+  double dd = 0.0, qq = 0.0, r1 = 0.0, r2 = 0.0;
+  if (Z > 2) {
+    const double zs = p[PAR_ZS], zp = p[PAR_ZP];
+    dd = (2.0 * qn + 1.0) * pow(4.0 * zs * zp, qn + 0.5) / pow(zs + zp, 2.0 * qn + 2.0) / sqrt(3.0);
+    qq = sqrt((4.0 * qn * qn + 6.0 * qn + 2.0) / 20.0) / zp;
+    const double hsp = p[PAR_HSP] / kEV;
+    double d1 = cbrt(fabs(hsp) / (dd * dd));
+    if (hsp < 0.0) d1 = -d1;
+    r1 = 0.5 / secant(1, hsp, dd, d1);
+    const double hpp = fmax(0.5 * (p[PAR_GPP] - p[PAR_GP2]), 0.1) / kEV;
+    double q1 = pow(fabs(hpp) / 3.0 / (qq * qq * qq * qq), 0.2);
+    if (hpp < 0.0) q1 = -q1;
+    r2 = 0.5 / secant(2, hpp, qq, q1);
+  }
+  p[PAR_DD] = dd;
+  p[PAR_QQ] = qq;
+  p[PAR_RHO0] = 0.5 * kEV / p[PAR_GSS];
+  p[PAR_RHO1] = r1;
+  p[PAR_RHO2] = r2;
+  p[PAR_TORE] = kTore[Z];
+  p[PAR_QN] = qn;
+  p[PAR_Z] = (double)Z;
+}
+
+// ------------------------------------------------------------ local-frame integrals
+// 22 heavy-heavy integrals in MOPAC storage order (SS/SS)=0 ... (P*P/P*P)=21, local axis
+// sigma pointing from atom j to atom i (two_elec_two_center_int_local_frame.py:126-262).
+__device__ void local_xx(double r, double da, double db, double qa1, double qb1, double r0a,
+                         double r0b, double r1a, double r1b, double r2a, double r2b, double ri[22]) {
+  const double e1 = kEV * 0.5, e2 = kEV * 0.25, e3 = kEV * 0.125, e4 = kEV * 0.0625;
+  const double qa = 2.0 * qa1, qb = 2.0 * qb1;
+  const double aee = (r0a + r0b) * (r0a + r0b), ade = (r1a + r0b) * (r1a + r0b);
+  const double aqe = (r2a + r0b) * (r2a + r0b), aed = (r0a + r1b) * (r0a + r1b);
+  const double aeq = (r0a + r2b) * (r0a + r2b), axx = (r1a + r1b) * (r1a + r1b);
+  const double adq = (r1a + r2b) * (r1a + r2b), aqd = (r2a + r1b) * (r2a + r1b);
+  const double aqq = (r2a + r2b) * (r2a + r2b);
+  const double r2 = r * r;
+  auto inv = [](double x) { return 1.0 / sqrt(x); };
+  const double ee = kEV * inv(r2 + aee);
+  const double dze = -e1 * inv((r + da) * (r + da) + ade) + e1 * inv((r - da) * (r - da) + ade);
+  const double c6 = e1 * inv(r2 + aqe);
+  const double qzze = e2 * inv((r - qa) * (r - qa) + aqe) + e2 * inv((r + qa) * (r + qa) + aqe) - c6;
+  const double qxxe = e1 * inv(r2 + qa * qa + aqe) - c6;
+  const double edz = -e1 * inv((r - db) * (r - db) + aed) + e1 * inv((r + db) * (r + db) + aed);
+  const double c12 = e1 * inv(r2 + aeq);
+  const double eqzz = e2 * inv((r - qb) * (r - qb) + aeq) + e2 * inv((r + qb) * (r + qb) + aeq) - c12;
+  const double eqxx = e1 * inv(r2 + qb * qb + aeq) - c12;
+  const double t20 = e2 * inv((r + da) * (r + da) + adq), t22 = e2 * inv((r - da) * (r - da) + adq);
+  const double t24 = e2 * inv((r - db) * (r - db) + aqd), t26 = e2 * inv((r + db) * (r + db) + aqd);
+  const double t36 = e2 * inv(r2 + aqq), t39 = e2 * inv(r2 + qa * qa + aqq), t40 = e2 * inv(r2 + qb * qb + aqq);
+  const double t42 = e3 * inv((r - qb) * (r - qb) + aqq), t44 = e3 * inv((r + qb) * (r + qb) + aqq);
+  const double t46 = e3 * inv((r + qa) * (r + qa) + aqq), t48 = e3 * inv((r - qa) * (r - qa) + aqq);
+  auto sq = [](double x) { return x * x; };
+  ri[0] = ee;
+  ri[1] = -dze;
+  ri[2] = ee + qzze;
+  ri[3] = ee + qxxe;
+  ri[4] = -edz;
+  ri[5] = e2 * inv(sq(r + da - db) + axx) + e2 * inv(sq(r - da + db) + axx) -
+          e2 * inv(sq(r - da - db) + axx) - e2 * inv(sq(r + da + db) + axx);
+  ri[6] = e1 * inv(r2 + sq(da - db) + axx) - e1 * inv(r2 + sq(da + db) + axx);
+  ri[7] = -edz + e3 * inv(sq(r + qa - db) + aqd) - e3 * inv(sq(r + qa + db) + aqd) +
+          e3 * inv(sq(r - qa - db) + aqd) - e3 * inv(sq(r - qa + db) + aqd) - t24 + t26;
+  ri[8] = -edz - t24 + e2 * inv(sq(r - db) + qa * qa + aqd) + t26 - e2 * inv(sq(r + db) + qa * qa + aqd);
+  ri[9] = e2 * inv(sq(qa1 - db) + sq(r + qa1) + aqd) - e2 * inv(sq(qa1 - db) + sq(r - qa1) + aqd) -
+          e2 * inv(sq(qa1 + db) + sq(r + qa1) + aqd) + e2 * inv(sq(qa1 + db) + sq(r - qa1) + aqd);
+  ri[10] = ee + eqzz;
+  ri[11] = ee + eqxx;
+  ri[12] = -dze + e3 * inv(sq(r + da - qb) + adq) - e3 * inv(sq(r - da - qb) + adq) +
+           e3 * inv(sq(r + da + qb) + adq) - e3 * inv(sq(r - da + qb) + adq) + t22 - t20;
+  ri[13] = -dze - t20 + e2 * inv(sq(r + da) + qb * qb + adq) + t22 - e2 * inv(sq(r - da) + qb * qb + adq);
+  ri[14] = e2 * inv(sq(da - qb1) + sq(r - qb1) + adq) - e2 * inv(sq(da - qb1) + sq(r + qb1) + adq) -
+           e2 * inv(sq(da + qb1) + sq(r - qb1) + adq) + e2 * inv(sq(da + qb1) + sq(r + qb1) + adq);
+  ri[15] = ee + eqzz + qzze + e4 * inv(sq(r + qa - qb) + aqq) + e4 * inv(sq(r + qa + qb) + aqq) +
+           e4 * inv(sq(r - qa - qb) + aqq) + e4 * inv(sq(r - qa + qb) + aqq) - t48 - t46 - t42 - t44 + t36;
+  ri[16] = ee + eqzz + qxxe + e3 * inv(sq(r - qb) + qa * qa + aqq) + e3 * inv(sq(r + qb) + qa * qa + aqq) -
+           t42 - t44 - t39 + t36;
+  ri[17] = ee + eqxx + qzze + e3 * inv(sq(r + qa) + qb * qb + aqq) + e3 * inv(sq(r - qa) + qb * qb + aqq) -
+           t46 - t48 - t40 + t36;
+  const double qxxqxx = e3 * inv(r2 + sq(qa - qb) + aqq) + e3 * inv(r2 + sq(qa + qb) + aqq) - t39 - t40 + t36;
+  ri[18] = ee + eqxx + qxxe + qxxqxx;
+  const double dm = sq(qa1 - qb1), dp = sq(qa1 + qb1);
+  ri[19] = e3 * inv(sq(r + qa1 - qb1) + dm + aqq) - e3 * inv(sq(r + qa1 + qb1) + dm + aqq) -
+           e3 * inv(sq(r - qa1 - qb1) + dm + aqq) + e3 * inv(sq(r - qa1 + qb1) + dm + aqq) -
+           e3 * inv(sq(r + qa1 - qb1) + dp + aqq) + e3 * inv(sq(r + qa1 + qb1) + dp + aqq) +
+           e3 * inv(sq(r - qa1 - qb1) + dp + aqq) - e3 * inv(sq(r - qa1 + qb1) + dp + aqq);
+  const double qxxqyy = e2 * inv(r2 + qa * qa + qb * qb + aqq) - t39 - t40 + t36;
+  ri[20] = ee + eqxx + qxxe + qxxqyy;
+  ri[21] = 0.5 * (qxxqxx - qxxqyy);
+}
+
+// 4 heavy-H integrals (two_elec_two_center_int_local_frame.py:90-107)
+__device__ void local_xh(double r, double da, double qa1, double r0a, double r1a, double r2a,
+                         double r0b, double ri[4]) {
+  const double e1 = kEV * 0.5, e2 = kEV * 0.25;
+  const double qa = 2.0 * qa1;
+  const double aee = (r0a + r0b) * (r0a + r0b), ade = (r1a + r0b) * (r1a + r0b);
+  const double aqe = (r2a + r0b) * (r2a + r0b);
+  const double ee = kEV / sqrt(r * r + aee);
+  const double c = e1 / sqrt(r * r + aqe);
+  ri[0] = ee;
+  ri[1] = e1 / sqrt((r + da) * (r + da) + ade) - e1 / sqrt((r - da) * (r - da) + ade);
+  ri[2] = ee + e2 / sqrt((r + qa) * (r + qa) + aqe) + e2 / sqrt((r - qa) * (r - qa) + aqe) - c;
+  ri[3] = ee + e1 / sqrt(r * r + qa * qa + aqe) - c;
+}
+
+// Rotation with R v = e_x built from the quaternion of v (two_elec_two_center_int.py:1576-1631),
+// including its antipodal fallback (|1 + v_x| < 1e-7 -> 180 deg turn about z).
+__device__ inline void quat_frame(double vx, double vy, double vz, double R[3][3]) {
+  double qy = vz, qz = -vy, qw = 1.0 + vx;
+  if (fabs(qw) < 1.0e-7) {
+    qy = 0.0;
+    qz = 1.0;
+    qw = 0.0;
+  }
+  const double n = 1.0 / sqrt(qy * qy + qz * qz + qw * qw);
+  qy *= n;
+  qz *= n;
+  qw *= n;
+  R[0][0] = 1.0 - 2.0 * (qy * qy + qz * qz);
+  R[0][1] = -2.0 * (qz * qw);
+  R[0][2] = 2.0 * (qy * qw);
+  R[1][0] = 2.0 * (qz * qw);
+  R[1][1] = 1.0 - 2.0 * (qz * qz);
+  R[1][2] = 2.0 * (qy * qz);
+  R[2][0] = -2.0 * (qy * qw);
+  R[2][1] = 2.0 * (qy * qz);
+  R[2][2] = 1.0 - 2.0 * (qy * qy);
+}
+
+// T[k][p]: coefficient of local pair p = (a>=b) in molecular pair k = (mu>=nu), where
+// molecular p_m = sum_a R[a][m] * local_(1+a).
+__device__ inline void pair_map(const double R[3][3], double T[10][10]) {
+  double U[4][4];
+  U[0][0] = 1.0;
+#pragma unroll
+  for (int a = 1; a < 4; ++a) U[0][a] = U[a][0] = 0.0;
+#pragma unroll
+  for (int m = 0; m < 3; ++m)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) U[1 + m][1 + a] = R[a][m];
+#pragma unroll
+  for (int k = 0; k < 10; ++k)
+#pragma unroll
+    for (int p = 0; p < 10; ++p) {
+      const int mu = kTriI[k], nu = kTriJ[k], a = kTriI[p], b = kTriJ[p];
+      T[k][p] = U[mu][a] * U[nu][b] + (a != b ? U[mu][b] * U[nu][a] : 0.0);
+    }
+}
+
+// Local tensor over local orbital-product pairs (s,s)=0 (sig,s)=1 (sig,sig)=2 (pi,s)=3
+// (pi,sig)=4 (pi,pi)=5 (pi',s)=6 (pi',sig)=7 (pi',pi)=8 (pi',pi')=9.
+__device__ inline void local_tensor(const double ri[22], double Lm[10][10]) {
+#pragma unroll
+  for (int p = 0; p < 10; ++p)
+#pragma unroll
+    for (int q = 0; q < 10; ++q) Lm[p][q] = 0.0;
+  Lm[0][0] = ri[0];  Lm[0][1] = ri[4];  Lm[0][2] = ri[10]; Lm[0][5] = ri[11]; Lm[0][9] = ri[11];
+  Lm[1][0] = ri[1];  Lm[1][1] = ri[5];  Lm[1][2] = ri[12]; Lm[1][5] = ri[13]; Lm[1][9] = ri[13];
+  Lm[2][0] = ri[2];  Lm[2][1] = ri[7];  Lm[2][2] = ri[15]; Lm[2][5] = ri[17]; Lm[2][9] = ri[17];
+  Lm[3][3] = ri[6];  Lm[3][4] = ri[14];
+  Lm[4][3] = ri[9];  Lm[4][4] = ri[19];
+  Lm[5][0] = ri[3];  Lm[5][1] = ri[8];  Lm[5][2] = ri[16]; Lm[5][5] = ri[18]; Lm[5][9] = ri[20];
+  Lm[6][6] = ri[6];  Lm[6][7] = ri[14];
+  Lm[7][6] = ri[9];  Lm[7][7] = ri[19];
+  Lm[8][8] = ri[21];
+  Lm[9][0] = ri[3];  Lm[9][1] = ri[8];  Lm[9][2] = ri[16]; Lm[9][5] = ri[20]; Lm[9][9] = ri[18];
+}
+
+// ------------------------------------------------------------------- STO overlaps
+// A_k(x) = int_1^inf t^k e^{-xt} dt and B_k(x) = int_-1^1 t^k e^{-xt} dt, k = 0..6, with the
+// reference's recurrences / truncated series (diat_overlap_PM6_SP.py:464-670).
+__device__ inline void aint7(double x, double a[7]) {
+  if (x == 0.0) {
+#pragma unroll
+    for (int k = 0; k < 7; ++k) a[k] = 0.0;
+    return;
+  }
+  a[0] = exp(-x) / x;
+#pragma unroll
+  for (int k = 1; k < 7; ++k) a[k] = a[0] + k * a[k - 1] / x;
+}
+
+__device__ inline void bint7(double x, double b[7]) {
+  const double ax = fabs(x);
+  if (ax > 0.5) {
+    const double tx = exp(x) / x, tmx = -exp(-x) / x;
+    b[0] = tx + tmx;
+#pragma unroll
+    for (int k = 1; k < 7; ++k) b[k] = ((k & 1) ? -tx : tx) + tmx + k * b[k - 1] / x;
+  } else if (ax > 1.0e-6) {
+    const double x2 = x * x, x3 = x2 * x, x4 = x2 * x2, x5 = x4 * x, x6 = x4 * x2;
+    b[0] = 2.0 + x2 / 3.0 + x4 / 60.0 + x6 / 2520.0;
+    b[2] = 2.0 / 3.0 + x2 / 5.0 + x4 / 84.0 + x6 / 3240.0;
+    b[4] = 2.0 / 5.0 + x2 / 7.0 + x4 / 108.0 + x6 / 3960.0;
+    b[6] = 2.0 / 7.0 + x2 / 9.0 + x4 / 132.0 + x6 / 4680.0;
+    b[1] = -2.0 / 3.0 * x - x3 / 15.0 - x5 / 420.0;
+    b[3] = -2.0 / 5.0 * x - x3 / 21.0 - x5 / 540.0;
+    b[5] = -2.0 / 7.0 * x - x3 / 27.0 - x5 / 660.0;
+  } else {
+    b[0] = 2.0; b[1] = 0.0; b[2] = 2.0 / 3.0; b[3] = 0.0; b[4] = 2.0 / 5.0; b[5] = 0.0; b[6] = 2.0 / 7.0;
+  }
+}
+
+__device__ inline int jcall_of(int qi, int qj) {
+  if (qi == 1 && qj == 1) return 2;
+  if (qi == 2 && qj == 1) return 3;
+  if (qi == 2 && qj == 2) return 4;
+  if (qi == 3 && qj == 1) return 431;
+  if (qi == 3 && qj == 2) return 5;
+  if (qi == 3 && qj == 3) return 6;
+  return 0;
+}
+
+// S[0..4] = S111 (s|s), S211 (p_sig(i)|s(j)), S121, S221, S222 in the diatomic frame
+// (diat_overlap_PM6_SP.py:84-345).  za/zb = (zeta_s, zeta_p) of atoms i / j.
+__device__ void overlap_local(int jc, double r, double zas, double zap, double zbs, double zbp,
+                              bool need_p_i, bool need_p_j, double S[5]) {
+  S[0] = S[1] = S[2] = S[3] = S[4] = 0.0;
+  double A[7], B[7];
+  const double s3 = 1.7320508075688772;
+  const double s10 = 3.1622776601683795, s30 = 5.477225575051661;
+  auto AB = [&](double z1, double z2) {
+    aint7(0.5 * r * (z1 + z2), A);
+    bint7(0.5 * r * (z1 - z2), B);
+  };
+  if (jc == 2) {
+    AB(zas, zbs);
+    S[0] = pow(zas * zbs * r * r, 1.5) * (A[2] * B[0] - B[2] * A[0]) / 4.0;
+  } else if (jc == 3) {
+    const double r4 = r * r * r * r;
+    AB(zas, zbs);
+    S[0] = pow(zbs, 1.5) * pow(zas, 2.5) * r4 * (A[3] * B[0] - B[3] * A[0] + A[2] * B[1] - B[2] * A[1]) / (s3 * 8.0);
+    if (need_p_i) {
+      AB(zap, zbs);
+      S[1] = pow(zbs, 1.5) * pow(zap, 2.5) * r4 * (A[2] * B[0] - B[2] * A[0] + A[3] * B[1] - B[3] * A[1]) / 8.0;
+    }
+  } else if (jc == 4) {
+    const double r5 = r * r * r * r * r;
+    AB(zas, zbs);
+    S[0] = pow(zbs * zas, 2.5) * r5 * (A[4] * B[0] + B[4] * A[0] - 2.0 * A[2] * B[2]) / 48.0;
+    AB(zap, zbs);
+    S[1] = pow(zbs * zap, 2.5) * r5 *
+           (A[3] * (B[0] - B[2]) - A[1] * (B[2] - B[4]) + B[3] * (A[0] - A[2]) - B[1] * (A[2] - A[4])) /
+           (16.0 * s3);
+    AB(zas, zbp);
+    S[2] = pow(zbp * zas, 2.5) * r5 *
+           (A[3] * (B[0] - B[2]) - A[1] * (B[2] - B[4]) - B[3] * (A[0] - A[2]) + B[1] * (A[2] - A[4])) /
+           (16.0 * s3);
+    AB(zap, zbp);
+    const double w = pow(zbp * zap, 2.5) * r5 / 16.0;
+    S[3] = -w * (B[2] * (A[4] + A[0]) - A[2] * (B[4] + B[0]));
+    S[4] = 0.5 * w * (A[4] * (B[0] - B[2]) - B[4] * (A[0] - A[2]) - A[2] * B[0] + B[2] * A[0]);
+  } else if (jc == 431) {
+    const double r5 = r * r * r * r * r;
+    AB(zas, zbs);
+    S[0] = pow(zbs, 1.5) * pow(zas, 3.5) * r5 *
+           (A[4] * B[0] + 2.0 * B[1] * A[3] - 2.0 * A[1] * B[3] - B[4] * A[0]) / (s10 * 24.0);
+    if (need_p_i) {
+      AB(zap, zbs);
+      S[1] = pow(zbs, 1.5) * pow(zap, 3.5) * r5 *
+             (A[3] * (B[0] + B[2]) - A[1] * (B[4] + B[2]) + B[1] * (A[2] + A[4]) - B[3] * (A[2] + A[0])) /
+             (8.0 * s30);
+    }
+  } else if (jc == 5) {
+    const double r6 = r * r * r * r * r * r;
+    AB(zas, zbs);
+    S[0] = pow(zbs, 2.5) * pow(zas, 3.5) * r6 *
+           (A[5] * B[0] + B[1] * A[4] - 2.0 * B[2] * A[3] - 2.0 * A[2] * B[3] + B[4] * A[1] + B[5] * A[0]) /
+           (s30 * 48.0);
+    AB(zap, zbs);
+    S[1] = pow(zbs, 2.5) * pow(zap, 3.5) * r6 *
+           (A[4] * B[0] + B[1] * A[5] - 2.0 * B[3] * A[3] - 2.0 * A[2] * B[2] + A[1] * B[5] + A[0] * B[4]) /
+           (48.0 * s10);
+    AB(zas, zbp);
+    S[2] = pow(zbp, 2.5) * pow(zas, 3.5) * r6 *
+           ((A[4] * B[0] - A[5] * B[1]) + 2.0 * (A[3] * B[1] - A[4] * B[2]) - 2.0 * (A[1] * B[3] - A[2] * B[4]) -
+            (A[0] * B[4] - A[1] * B[5])) /
+           (48.0 * s10);
+    AB(zap, zbp);
+    const double pre = pow(zbp, 2.5) * pow(zap, 3.5) * r6;
+    S[3] = pre *
+           ((A[3] * B[0] - A[5] * B[2]) + (A[2] * B[1] - A[4] * B[3]) - (A[1] * B[2] - A[3] * B[4]) -
+            (A[0] * B[3] - A[2] * B[5])) /
+           (16.0 * s30);
+    S[4] = pre *
+           ((A[5] - A[3]) * (B[0] - B[2]) + (A[4] - A[2]) * (B[1] - B[3]) - (A[3] - A[1]) * (B[2] - B[4]) -
+            (A[2] - A[0]) * (B[3] - B[5])) /
+           (32.0 * s30);
+  } else if (jc == 6) {
+    const double r7 = r * r * r * r * r * r * r;
+    AB(zas, zbs);
+    S[0] = pow(zbs * zas, 3.5) * r7 * (A[6] * B[0] - 3.0 * B[2] * A[4] + 3.0 * A[2] * B[4] - A[0] * B[6]) / 1440.0;
+    AB(zap, zbs);
+    S[1] = pow(zbs * zap, 3.5) * r7 *
+           ((A[5] * B[0] + A[6] * B[1]) + (-A[4] * B[1] - A[5] * B[2]) - 2.0 * (A[3] * B[2] + A[4] * B[3]) -
+            2.0 * (-A[2] * B[3] - A[3] * B[4]) + (A[1] * B[4] + A[2] * B[5]) + (-A[0] * B[5] - A[1] * B[6])) /
+           (480.0 * s3);
+    AB(zas, zbp);
+    S[2] = pow(zbp * zas, 3.5) * r7 *
+           ((A[5] * B[0] - A[6] * B[1]) + (A[4] * B[1] - A[5] * B[2]) - 2.0 * (A[3] * B[2] - A[4] * B[3]) -
+            2.0 * (A[2] * B[3] - A[3] * B[4]) + (A[1] * B[4] - A[2] * B[5]) + (A[0] * B[5] - A[1] * B[6])) /
+           (480.0 * s3);
+    AB(zap, zbp);
+    const double pre = pow(zbp, 3.5) * pow(zap, 3.5) * r7;
+    S[3] = pre * ((A[4] * B[0] - A[6] * B[2]) - 2.0 * (A[2] * B[2] - A[4] * B[4]) + (A[0] * B[4] - A[2] * B[6])) / 480.0;
+    S[4] = pre *
+           ((A[6] - A[4]) * (B[0] - B[2]) - 2.0 * (A[4] - A[2]) * (B[2] - B[4]) + (A[2] - A[0]) * (B[4] - B[6])) /
+           960.0;
+  }
+  (void)need_p_j;
+}
+
+// --------------------------------------------------------------- core-core repulsion
+__device__ double core_repulsion(int method, int Zi, int Zj, double rbohr, double gam, const double* pi,
+                            const double* pj) {
+  const double ra = rbohr * kA0;
+  const double ti = pi[PAR_TORE], tj = pj[PAR_TORE];
+  const double t1 = ti * tj * gam;
+  double t2 = exp(-pi[PAR_ALPHA] * ra);
+  if ((Zi == 7 || Zi == 8) && Zj == 1) t2 *= ra;
+  const double t3 = exp(-pj[PAR_ALPHA] * ra);
+  double e = t1 * (1.0 + t2 + t3);
+  const int ng = method == PYSEQM_AM1 ? 4 : (method == PYSEQM_PM3 ? 2 : 0);
+  if (ng) {
+    double g = 0.0;
+    for (int k = 0; k < ng; ++k) {
+      const double* ki = pi + PAR_GAUSS + 3 * k;
+      const double* kj = pj + PAR_GAUSS + 3 * k;
+      g += ki[0] * exp(-ki[1] * (ra - ki[2]) * (ra - ki[2]));
+      g += kj[0] * exp(-kj[1] * (ra - kj[2]) * (ra - kj[2]));
+    }
+    e += ti * tj / ra * g;
+  }
+  return e;
+}
+
+struct PairGeom {
+  double x[3];  // unit vector i -> j
+  double r;     // bohr
+};
+
+__device__ inline PairGeom pair_geom(const double* coords, int i, int j) {
+  PairGeom g;
+  const double dx = coords[3 * j] - coords[3 * i], dy = coords[3 * j + 1] - coords[3 * i + 1],
+               dz = coords[3 * j + 2] - coords[3 * i + 2];
+  const double d = sqrt(dx * dx + dy * dy + dz * dz);
+  g.x[0] = dx / d;
+  g.x[1] = dy / d;
+  g.x[2] = dz / d;
+  g.r = d * (1.0 / kA0);
+  return g;
+}
+
+// ------------------------------------------------------------------- tile kernels
+enum { XX = 0, XH = 1, HH = 2 };
+
+template <int KIND>
+__global__ __launch_bounds__(64) void k_pair_tiles(pyseqm_layout_t L, int method, int tile_base,
+                                                    const double* __restrict__ coords,
+                                                    const double* __restrict__ par, double* __restrict__ wout,
+                                                    double* __restrict__ H, double* __restrict__ part,
+                                                    double* __restrict__ enuc_tile,
+                                                    double* __restrict__ enuc_pair,
+                                                    const int* __restrict__ pair_ref) {
+  const int tl = blockIdx.x;
+  const int tile = tile_base + tl;
+  const int lane = threadIdx.x;
+  const Tile t = load_tile(L.tiles, tile);
+  int i, j;
+  const bool valid = lane_pair(t, lane, i, j);
+  constexpr int NR = KIND == HH ? 1 : 10;
+  constexpr int NC = KIND == XX ? 10 : 1;
+  double rowv[NR], colv[NC];
+#pragma unroll
+  for (int k = 0; k < NR; ++k) rowv[k] = 0.0;
+#pragma unroll
+  for (int k = 0; k < NC; ++k) colv[k] = 0.0;
+  double en = 0.0;
+  constexpr int NW = KIND == XX ? 100 : (KIND == XH ? 10 : 1);
+  double* wt = wout + (size_t)tl * NW * 64 + lane;
+
+  if (valid) {
+    const double* pi = par + (size_t)i * PYSEQM_NPAR;
+    const double* pj = par + (size_t)j * PYSEQM_NPAR;
+    const PairGeom g = pair_geom(coords, i, j);
+    const int Zi = (int)pi[PAR_Z], Zj = (int)pj[PAR_Z];
+    const long long mo = L.mol_mat_off[t.mol];
+    const int norb = L.mol_norb[t.mol];
+    const int ai = L.atom_ao[i], aj = L.atom_ao[j];
+    double R[3][3];
+    quat_frame(-g.x[0], -g.x[1], -g.x[2], R);
+    double gam;
+    if constexpr (KIND == XX) {
+      double ri[22];
+      local_xx(g.r, pi[PAR_DD], pj[PAR_DD], pi[PAR_QQ], pj[PAR_QQ], pi[PAR_RHO0], pj[PAR_RHO0], pi[PAR_RHO1],
+               pj[PAR_RHO1], pi[PAR_RHO2], pj[PAR_RHO2], ri);
+      double T[10][10], Lm[10][10];
+      pair_map(R, T);
+      local_tensor(ri, Lm);
+      double w00 = 0.0;
+#pragma unroll
+      for (int l = 0; l < 10; ++l) {
+        double v[10];
+#pragma unroll
+        for (int p = 0; p < 10; ++p) {
+          double s = 0.0;
+#pragma unroll
+          for (int q = 0; q < 10; ++q) s += Lm[p][q] * T[l][q];
+          v[p] = s;
+        }
+#pragma unroll
+        for (int k = 0; k < 10; ++k) {
+          double s = 0.0;
+#pragma unroll
+          for (int p = 0; p < 10; ++p) s += T[k][p] * v[p];
+          wt[(size_t)(k * 10 + l) * 64] = s;
+          if (l == 0 && k == 0) w00 = s;
+          if (l == 0) rowv[k] = -pj[PAR_TORE] * s;   // e1b: electrons of i, core of j
+          if (k == 0) colv[l] = -pi[PAR_TORE] * s;   // e2a: electrons of j, core of i
+        }
+      }
+      gam = w00;
+    } else if constexpr (KIND == XH) {
+      double ri[4];
+      local_xh(g.r, pi[PAR_DD], pi[PAR_QQ], pi[PAR_RHO0], pi[PAR_RHO1], pi[PAR_RHO2], pj[PAR_RHO0], ri);
+      double T[10][10];
+      pair_map(R, T);
+#pragma unroll
+      for (int k = 0; k < 10; ++k) {
+        const double s = T[k][0] * ri[0] + T[k][1] * ri[1] + T[k][2] * ri[2] + (T[k][5] + T[k][9]) * ri[3];
+        wt[(size_t)k * 64] = s;
+        rowv[k] = -pj[PAR_TORE] * s;
+      }
+      gam = ri[0];
+      colv[0] = -pi[PAR_TORE] * gam;
+    } else {
+      gam = kEV / sqrt(g.r * g.r + (pi[PAR_RHO0] + pj[PAR_RHO0]) * (pi[PAR_RHO0] + pj[PAR_RHO0]));
+      wt[0] = gam;
+      rowv[0] = -pj[PAR_TORE] * gam;
+      colv[0] = -pi[PAR_TORE] * gam;
+    }
+
+    // overlap block and off-diagonal core Hamiltonian
+    constexpr int NI = KIND == HH ? 1 : 4;
+    constexpr int NJ = KIND == XX ? 4 : 1;
+    double di[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) di[a][b] = 0.0;
+    if (g.r <= kOverlapCut) {
+      const int jc = jcall_of((int)pi[PAR_QN], (int)pj[PAR_QN]);  // validated on the host
+      double S[5];
+      overlap_local(jc, g.r, pi[PAR_ZS], pi[PAR_ZP], pj[PAR_ZS], pj[PAR_ZP], NI == 4, NJ == 4, S);
+      double Rp[3][3];
+      quat_frame(g.x[0], g.x[1], g.x[2], Rp);
+      di[0][0] = S[0];
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        di[1 + m][0] = S[1] * Rp[0][m];
+        di[0][1 + m] = -S[2] * Rp[0][m];
+      }
+      if (NJ == 4) {
+        const double D[3] = {-S[3], S[4], S[4]};
+#pragma unroll
+        for (int m = 0; m < 3; ++m)
+#pragma unroll
+          for (int n = 0; n < 3; ++n) di[1 + m][1 + n] = Rp[0][m] * D[0] * Rp[0][n] + Rp[1][m] * D[1] * Rp[1][n] +
+                                                         Rp[2][m] * D[2] * Rp[2][n];
+      }
+    }
+    const double bi[4] = {pi[PAR_BS], pi[PAR_BP], pi[PAR_BP], pi[PAR_BP]};
+    const double bj[4] = {pj[PAR_BS], pj[PAR_BP], pj[PAR_BP], pj[PAR_BP]};
+    double* Hm = H + mo;
+#pragma unroll
+    for (int a = 0; a < NI; ++a)
+#pragma unroll
+      for (int b = 0; b < NJ; ++b) {
+        const double h = di[a][b] * 0.5 * (bi[a] + bj[b]);
+        Hm[(size_t)(ai + a) * norb + aj + b] = h;
+        Hm[(size_t)(aj + b) * norb + ai + a] = h;
+      }
+    en = core_repulsion(method, Zi, Zj, g.r, gam, pi, pj);
+    if (enuc_pair) enuc_pair[pair_ref[(size_t)tile * 64 + lane]] = en;
+  } else {
+#pragma unroll
+    for (int k = 0; k < NW; ++k) wt[(size_t)k * 64] = 0.0;
+  }
+  write_partials(part, tile, lane, rowv, colv);
+  const double es = wave_sum(en);
+  if (lane == 0) enuc_tile[tile] = es;
+}
+
+// Diagonal core-Hamiltonian blocks: U on the diagonal plus the electron-core attractions
+// collected from every tile the atom appears in (hcore.py:127-150), fixed summation order.
+__global__ void k_hcore_diag(pyseqm_layout_t L, const double* __restrict__ par, const double* __restrict__ part,
+                             double* __restrict__ H) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= L.natoms) return;
+  const double* p = par + (size_t)a * PYSEQM_NPAR;
+  const bool heavy = L.atom_Z[a] > 1;
+  double acc[10];
+#pragma unroll
+  for (int k = 0; k < 10; ++k) acc[k] = 0.0;
+  acc[0] = p[PAR_USS];
+  if (heavy) acc[2] = acc[5] = acc[9] = p[PAR_UPP];
+  const int nk = heavy ? 10 : 1;
+  for (int e = L.csr_off[a]; e < L.csr_off[a + 1]; ++e) {
+    const double* src = part + (size_t)L.csr_idx[e] * kPartStride;
+    for (int k = 0; k < nk; ++k) acc[k] += src[k];
+  }
+  const int m = L.atom_mol[a];
+  const int norb = L.mol_norb[m];
+  double* Hm = H + L.mol_mat_off[m];
+  const int ao = L.atom_ao[a];
+  if (heavy) {
+#pragma unroll
+    for (int mu = 0; mu < 4; ++mu)
+#pragma unroll
+      for (int nu = 0; nu < 4; ++nu) Hm[(size_t)(ao + mu) * norb + ao + nu] = acc[tri_index(mu, nu)];
+  } else {
+    Hm[(size_t)ao * norb + ao] = acc[0];
+  }
+}
+
+// Per-molecule sum of per-tile values (and optionally per-atom values), fixed order.
+__global__ void k_mol_sum(pyseqm_layout_t L, const double* __restrict__ tile_v, const double* __restrict__ atom_v,
+                          const unsigned char* __restrict__ active, double* __restrict__ out) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= L.nmol) return;
+  if (active && !active[m]) return;
+  double s = 0.0;
+  for (int kind = 0; kind < 3; ++kind) {
+    const int* off = L.mol_tile_off + kind * (L.nmol + 1);
+    for (int t = off[m]; t < off[m + 1]; ++t) s += tile_v[t];
+  }
+  if (atom_v)
+    for (int a = L.mol_atom_off[m]; a < L.mol_atom_off[m + 1]; ++a) s += atom_v[a];
+  out[m] = s;
+}
+
+}  // namespace
+
+namespace pyseqm {
+void launch_mol_sum(const pyseqm_layout_t* L, const double* tile_v, const double* atom_v,
+                    const unsigned char* active, double* out, hipStream_t s) {
+  if (L->nmol > 0) k_mol_sum<<<(L->nmol + 127) / 128, 128, 0, s>>>(*L, tile_v, atom_v, active, out);
+}
+}  // namespace pyseqm
+
+extern "C" int pyseqm_atom_setup(const pyseqm_layout_t* L, double* atom_par, void* stream) {
+  if (!L || L->natoms < 0) return PYSEQM_EINVAL;
+  if (L->natoms == 0) return PYSEQM_OK;
+  hipStream_t s = (hipStream_t)stream;
+  k_atom_setup<<<(L->natoms + 127) / 128, 128, 0, s>>>(*L, atom_par);
+  return hipGetLastError() == hipSuccess ? PYSEQM_OK : PYSEQM_ELAUNCH;
+}
+
+extern "C" int pyseqm_pair_integrals(const pyseqm_layout_t* L, int method, const double* coords,
+                                     const double* atom_par, double* w_xx, double* w_xh, double* w_hh,
+                                     double* H, double* part, double* enuc_tile, double* enuc_mol,
+                                     double* enuc_pair, const int* pair_ref, void* stream) {
+  if (!L || method < PYSEQM_MNDO || method > PYSEQM_PM3) return PYSEQM_EINVAL;
+  if (enuc_pair && !pair_ref) return PYSEQM_EINVAL;
+  hipStream_t s = (hipStream_t)stream;
+  int base = 0;
+  if (L->ntile[0])
+    k_pair_tiles<XX><<<L->ntile[0], 64, 0, s>>>(*L, method, base, coords, atom_par, w_xx, H, part, enuc_tile,
+                                                enuc_pair, pair_ref);
+  base += L->ntile[0];
+  if (L->ntile[1])
+    k_pair_tiles<XH><<<L->ntile[1], 64, 0, s>>>(*L, method, base, coords, atom_par, w_xh, H, part, enuc_tile,
+                                                enuc_pair, pair_ref);
+  base += L->ntile[1];
+  if (L->ntile[2])
+    k_pair_tiles<HH><<<L->ntile[2], 64, 0, s>>>(*L, method, base, coords, atom_par, w_hh, H, part, enuc_tile,
+                                                enuc_pair, pair_ref);
+  if (L->natoms) k_hcore_diag<<<(L->natoms + 127) / 128, 128, 0, s>>>(*L, atom_par, part, H);
+  pyseqm::launch_mol_sum(L, enuc_tile, nullptr, nullptr, enuc_mol, s);
+  return hipGetLastError() == hipSuccess ? PYSEQM_OK : PYSEQM_ELAUNCH;
+}

@@ -1,0 +1,367 @@
+// SCF-loop kernels on the packed (ragged) batch, FP64, gfx950.
+//
+// All kernels work on a list of active molecules idx[0..nact) (device array) so that converged
+// molecules cost nothing; grid.y indexes that list.  They restate, per molecule:
+//   density-step plumbing of sym_eig_trunc (diag.py:110-241): Gershgorin-shifted padding of the
+//     packed Fock matrix for a batched eigensolver and P = 2 C_occ C_occ^T,
+//   mixing (scf_loop.py:266-267, 1052-1056) and the 4-way convergence test get_error
+//     (scf_loop.py:106-147),
+//   the Pulay/DIIS residual FP - PF, its max-norm, the B-matrix row and the extrapolated Fock
+//     matrix (scf_loop.py:997-1035),
+//   the XL-BOMD density propagation (MolecularDynamics.py:1418-1427).
+#include "common.h"
+
+using namespace pyseqm;
+
+namespace {
+
+constexpr int kEl = 256;  // threads of the element-wise / reduction kernels
+
+__device__ inline double block_sum(double v, double* sh) {
+  v = wave_sum(v);
+  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) s += sh[k];
+  __syncthreads();
+  return s;  // valid in thread 0
+}
+
+__device__ inline double wave_max(double v) {
+  for (int o = 1; o < 64; o <<= 1) v = fmax(v, __shfl_xor(v, o));
+  return v;
+}
+
+__device__ inline double block_max(double v, double* sh) {
+  v = wave_max(v);
+  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  double s = -1.0e308;
+  if (threadIdx.x == 0)
+    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) s = fmax(s, sh[k]);
+  __syncthreads();
+  return s;
+}
+
+// Packed F of active molecule -> zero-padded (nmax x nmax) matrix whose padded diagonal is
+// hN + (1 + 0.005 t) dE, t = 1, 2, ... above the Gershgorin range [h1, hN] (diag.py:172-192),
+// so that padded eigenpairs sort above every real one.  One block per molecule.
+__global__ void k_pad_for_eigh(pyseqm_layout_t L, const double* __restrict__ F, const int* __restrict__ idx,
+                               int nmax, double* __restrict__ out) {
+  __shared__ double sh[kEl / 64];
+  const int k = blockIdx.x;
+  const int m = idx[k];
+  const int n = L.mol_norb[m];
+  const double* Fm = F + L.mol_mat_off[m];
+  double* O = out + (size_t)k * nmax * nmax;
+  double hi = -1.0e300, lo = 1.0e300;
+  for (int r = threadIdx.x; r < n; r += blockDim.x) {
+    double s = 0.0;
+    for (int c = 0; c < n; ++c) s += fabs(Fm[(size_t)r * n + c]);
+    const double d = Fm[(size_t)r * n + r];
+    const double rad = s - fabs(d);
+    hi = fmax(hi, d + rad);
+    lo = fmin(lo, d - rad);
+  }
+  const double hN = block_max(hi, sh);
+  const double mlo = block_max(-lo, sh);
+  __shared__ double hs[2];
+  if (threadIdx.x == 0) {
+    hs[0] = hN;
+    hs[1] = hN + mlo;  // dE = hN - h1
+  }
+  __syncthreads();
+  const double top = hs[0], dE = hs[1];
+  for (size_t e = threadIdx.x; e < (size_t)nmax * nmax; e += blockDim.x) {
+    const int r = (int)(e / nmax), c = (int)(e % nmax);
+    double v = 0.0;
+    if (r < n && c < n)
+      v = Fm[(size_t)r * n + c];
+    else if (r == c)
+      v = top + (1.0 + 0.005 * (r - n + 1)) * dE;
+    O[e] = v;
+  }
+}
+
+// P = 2 sum_{k < nocc} V[:,k] V[:,k]^T (diag.py:230-232) from the padded eigenvectors (columns),
+// 16x16 output tile per block, LDS-staged rows of V.
+__global__ void k_density_from_evecs(pyseqm_layout_t L, const double* __restrict__ V, int nmax,
+                                     const int* __restrict__ idx, double* __restrict__ P) {
+  __shared__ double Va[16][33], Vb[16][33];
+  const int k = blockIdx.y;
+  const int m = idx[k];
+  const int n = L.mol_norb[m];
+  const int nt = (nmax + 15) / 16;
+  const int ti = blockIdx.x / nt, tj = blockIdx.x % nt;
+  if (ti * 16 >= n || tj * 16 >= n) return;
+  const int nocc = L.mol_nocc[m];
+  const double* Vm = V + (size_t)k * nmax * nmax;
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  const int r = ti * 16 + ty, c = tj * 16 + tx;
+  double s = 0.0;
+  for (int k0 = 0; k0 < nocc; k0 += 32) {
+    for (int e = threadIdx.x; e < 16 * 32; e += 256) {
+      const int rr = e / 32, kk = e % 32;
+      const int gi = ti * 16 + rr, gj = tj * 16 + rr, gk = k0 + kk;
+      Va[rr][kk] = (gi < n && gk < nocc) ? Vm[(size_t)gi * nmax + gk] : 0.0;
+      Vb[rr][kk] = (gj < n && gk < nocc) ? Vm[(size_t)gj * nmax + gk] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll 8
+    for (int kk = 0; kk < 32; ++kk) s += Va[ty][kk] * Vb[tx][kk];
+    __syncthreads();
+  }
+  if (r < n && c < n) P[L.mol_mat_off[m] + (size_t)r * n + c] = 2.0 * s;
+}
+
+// Pold = P; P = a*P + (1-a)*Pnew on the active molecules.
+__global__ void k_mix(pyseqm_layout_t L, const int* __restrict__ idx, double a, double* __restrict__ P,
+                      const double* __restrict__ Pnew, double* __restrict__ Pold) {
+  const int m = idx[blockIdx.y];
+  const size_t n2 = (size_t)L.mol_norb[m] * L.mol_norb[m];
+  const size_t off = L.mol_mat_off[m];
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n2; e += (size_t)gridDim.x * blockDim.x) {
+    const double p = P[off + e];
+    if (Pold) Pold[off + e] = p;
+    P[off + e] = a * p + (1.0 - a) * Pnew[off + e];
+  }
+}
+
+// Convergence test of get_error (scf_loop.py:106-147): |dE| <= eps, max|FP-PF| <= 50 eps,
+// ||dP||_F / (4 natoms) <= 2 eps, max|dP| <= 15 eps.  Writes notconv[m] and the metrics.
+__global__ void k_scf_check(pyseqm_layout_t L, const int* __restrict__ idx, const double* __restrict__ P,
+                            const double* __restrict__ Pold, const double* __restrict__ e_new,
+                            double* __restrict__ e_old, const double* __restrict__ diis_err, double eps,
+                            unsigned char* __restrict__ notconv, double* __restrict__ metrics) {
+  __shared__ double sh[kEl / 64];
+  const int m = idx[blockIdx.x];
+  const int n = L.mol_norb[m];
+  const size_t off = L.mol_mat_off[m];
+  double ss = 0.0, mx = 0.0;
+  for (size_t e = threadIdx.x; e < (size_t)n * n; e += blockDim.x) {
+    const double d = P[off + e] - Pold[off + e];
+    ss += d * d;
+    mx = fmax(mx, fabs(d));
+  }
+  const double s2 = block_sum(ss, sh);
+  const double m2 = block_max(mx, sh);
+  if (threadIdx.x == 0) {
+    const int nat = L.mol_atom_off[m + 1] - L.mol_atom_off[m];
+    const double dm = sqrt(s2) / (4.0 * nat);
+    const double de = e_new[m] - e_old[m];
+    const double dd = diis_err ? diis_err[m] : 0.0;
+    const bool bad = fabs(de) > eps || dd > 50.0 * eps || dm > 2.0 * eps || m2 > 15.0 * eps;
+    notconv[m] = bad ? 1 : 0;
+    if (bad) e_old[m] = e_new[m];
+    if (metrics) {
+      metrics[4 * m + 0] = de;
+      metrics[4 * m + 1] = dd;
+      metrics[4 * m + 2] = dm;
+      metrics[4 * m + 3] = m2;
+    }
+  }
+}
+
+// Ragged FP64 GEMM C = A B per active molecule (n x n), 64x64 output tile per 256-thread block,
+// 4x4 outputs per thread, K staged through LDS in slabs of 16.
+__global__ __launch_bounds__(256) void k_gemm64(pyseqm_layout_t L, const int* __restrict__ idx, int nmax,
+                                                const double* __restrict__ A, const double* __restrict__ B,
+                                                double* __restrict__ C) {
+  __shared__ double As[16][64 + 1], Bs[16][64 + 1];
+  const int m = idx[blockIdx.y];
+  const int n = L.mol_norb[m];
+  const int nt = (nmax + 63) / 64;
+  const int ti = blockIdx.x / nt, tj = blockIdx.x % nt;
+  if (ti * 64 >= n || tj * 64 >= n) return;
+  const size_t off = L.mol_mat_off[m];
+  const double* Am = A + off;
+  const double* Bm = B + off;
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  double acc[4][4] = {};
+  for (int k0 = 0; k0 < n; k0 += 16) {
+    for (int e = threadIdx.x; e < 16 * 64; e += 256) {
+      const int kk = e & 15, rr = e >> 4;  // A tile: rows rr, cols k0+kk
+      const int gi = ti * 64 + rr, gk = k0 + kk;
+      As[kk][rr] = (gi < n && gk < n) ? Am[(size_t)gi * n + gk] : 0.0;
+      const int cc = e & 63, kb = e >> 6;  // B tile: rows k0+kb, cols cc
+      const int gkb = k0 + kb, gj = tj * 64 + cc;
+      Bs[kb][cc] = (gkb < n && gj < n) ? Bm[(size_t)gkb * n + gj] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < 16; ++kk) {
+      double a[4], b[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        a[u] = As[kk][ty + 16 * u];
+        b[u] = Bs[kk][tx + 16 * u];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[u][v] += a[u] * b[v];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int gi = ti * 64 + ty + 16 * u, gj = tj * 64 + tx + 16 * v;
+      if (gi < n && gj < n) C[off + (size_t)gi * n + gj] = acc[u][v];
+    }
+}
+
+// C = X - X^T (X = F P, so C = FP - PF) into ring slot; diis_err[m] = max|C|.
+__global__ void k_antisym(pyseqm_layout_t L, const int* __restrict__ idx, const double* __restrict__ X,
+                          double* __restrict__ Cout, double* __restrict__ diis_err) {
+  __shared__ double sh[kEl / 64];
+  const int m = idx[blockIdx.x];
+  const int n = L.mol_norb[m];
+  const size_t off = L.mol_mat_off[m];
+  double mx = 0.0;
+  for (size_t e = threadIdx.x; e < (size_t)n * n; e += blockDim.x) {
+    const int r = (int)(e / n), c = (int)(e % n);
+    const double v = X[off + e] - X[off + (size_t)c * n + r];
+    Cout[off + e] = v;
+    mx = fmax(mx, fabs(v));
+  }
+  const double r = block_max(mx, sh);
+  if (threadIdx.x == 0) diis_err[m] = r;
+}
+
+// B[m][slot][j] = <C_slot, C_j> over the packed upper triangle = 0.5 * full Frobenius product
+// (C is antisymmetric), j = 0..count-1 (scf_loop.py:1003-1013).  Block per (j, molecule).
+__global__ void k_diis_dots(pyseqm_layout_t L, const int* __restrict__ idx, const double* __restrict__ ring,
+                            size_t ring_stride, int slot, int nfock, double* __restrict__ Bmat) {
+  __shared__ double sh[kEl / 64];
+  const int j = blockIdx.x;
+  const int m = idx[blockIdx.y];
+  const int n = L.mol_norb[m];
+  const size_t off = L.mol_mat_off[m];
+  const double* a = ring + (size_t)slot * ring_stride + off;
+  const double* b = ring + (size_t)j * ring_stride + off;
+  double s = 0.0;
+  for (size_t e = threadIdx.x; e < (size_t)n * n; e += blockDim.x) s += a[e] * b[e];
+  const double r = block_sum(s, sh);
+  if (threadIdx.x == 0) Bmat[((size_t)m * (nfock + 1) + slot) * (nfock + 1) + j] = 0.5 * r;
+}
+
+// F = sum_j c[m][j] F_j over the stored Fock matrices (scf_loop.py:1033-1035).
+__global__ void k_diis_combine(pyseqm_layout_t L, const int* __restrict__ idx, const double* __restrict__ ring,
+                               size_t ring_stride, int count, const double* __restrict__ coef, int cstride,
+                               double* __restrict__ F) {
+  const int m = idx[blockIdx.y];
+  const size_t n2 = (size_t)L.mol_norb[m] * L.mol_norb[m];
+  const size_t off = L.mol_mat_off[m];
+  const double* c = coef + (size_t)m * cstride;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n2; e += (size_t)gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int j = 0; j < count; ++j) s += c[j] * ring[(size_t)j * ring_stride + off + e];
+    F[off + e] = s;
+  }
+}
+
+// dst = src on the active molecules' blocks.
+__global__ void k_copy_active(pyseqm_layout_t L, const int* __restrict__ idx, const double* __restrict__ src,
+                              double* __restrict__ dst) {
+  const int m = idx[blockIdx.y];
+  const size_t n2 = (size_t)L.mol_norb[m] * L.mol_norb[m];
+  const size_t off = L.mol_mat_off[m];
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n2; e += (size_t)gridDim.x * blockDim.x)
+    dst[off + e] = src[off + e];
+}
+
+inline int ceil_div(long long a, long long b) { return (int)((a + b - 1) / b); }
+
+inline int elem_blocks(int nmax) {
+  const long long n2 = (long long)nmax * nmax;
+  int b = ceil_div(n2, kEl);
+  return b < 1 ? 1 : (b > 1024 ? 1024 : b);
+}
+
+inline int status() { return hipGetLastError() == hipSuccess ? PYSEQM_OK : PYSEQM_ELAUNCH; }
+
+}  // namespace
+
+extern "C" int pyseqm_pad_for_eigh(const pyseqm_layout_t* L, const double* F, const int* idx, int nact, int nmax,
+                                   double* out, void* stream) {
+  if (!L || nact < 0 || nmax <= 0) return PYSEQM_EINVAL;
+  if (nact == 0) return PYSEQM_OK;
+  k_pad_for_eigh<<<nact, kEl, 0, (hipStream_t)stream>>>(*L, F, idx, nmax, out);
+  return status();
+}
+
+extern "C" int pyseqm_density_from_evecs(const pyseqm_layout_t* L, const double* V, int nmax, const int* idx,
+                                         int nact, double* P, void* stream) {
+  if (!L || nact < 0 || nmax <= 0) return PYSEQM_EINVAL;
+  if (nact == 0) return PYSEQM_OK;
+  const int nt = ceil_div(nmax, 16);
+  k_density_from_evecs<<<dim3(nt * nt, nact), 256, 0, (hipStream_t)stream>>>(*L, V, nmax, idx, P);
+  return status();
+}
+
+extern "C" int pyseqm_mix(const pyseqm_layout_t* L, const int* idx, int nact, int nmax, double alpha, double* P,
+                          const double* Pnew, double* Pold, void* stream) {
+  if (!L || nact < 0) return PYSEQM_EINVAL;
+  if (nact == 0) return PYSEQM_OK;
+  k_mix<<<dim3(elem_blocks(nmax), nact), kEl, 0, (hipStream_t)stream>>>(*L, idx, alpha, P, Pnew, Pold);
+  return status();
+}
+
+extern "C" int pyseqm_copy_active(const pyseqm_layout_t* L, const int* idx, int nact, int nmax, const double* src,
+                                  double* dst, void* stream) {
+  if (!L || nact < 0) return PYSEQM_EINVAL;
+  if (nact == 0) return PYSEQM_OK;
+  k_copy_active<<<dim3(elem_blocks(nmax), nact), kEl, 0, (hipStream_t)stream>>>(*L, idx, src, dst);
+  return status();
+}
+
+extern "C" int pyseqm_scf_check(const pyseqm_layout_t* L, const int* idx, int nact, const double* P,
+                                const double* Pold, const double* e_new, double* e_old, const double* diis_err,
+                                double eps, unsigned char* notconv, double* metrics, void* stream) {
+  if (!L || nact < 0) return PYSEQM_EINVAL;
+  if (nact == 0) return PYSEQM_OK;
+  k_scf_check<<<nact, kEl, 0, (hipStream_t)stream>>>(*L, idx, P, Pold, e_new, e_old, diis_err, eps, notconv,
+                                                     metrics);
+  return status();
+}
+
+extern "C" int pyseqm_gemm(const pyseqm_layout_t* L, const int* idx, int nact, int nmax, const double* A,
+                           const double* B, double* C, void* stream) {
+  if (!L || nact < 0 || nmax <= 0) return PYSEQM_EINVAL;
+  if (nact == 0) return PYSEQM_OK;
+  const int nt = ceil_div(nmax, 64);
+  k_gemm64<<<dim3(nt * nt, nact), 256, 0, (hipStream_t)stream>>>(*L, idx, nmax, A, B, C);
+  return status();
+}
+
+extern "C" int pyseqm_diis_residual(const pyseqm_layout_t* L, const int* idx, int nact, int nmax, const double* F,
+                                    const double* P, double* X, double* Cout, double* diis_err, void* stream) {
+  int rc = pyseqm_gemm(L, idx, nact, nmax, F, P, X, stream);
+  if (rc != PYSEQM_OK || nact == 0) return rc;
+  k_antisym<<<nact, kEl, 0, (hipStream_t)stream>>>(*L, idx, X, Cout, diis_err);
+  return status();
+}
+
+extern "C" int pyseqm_diis_dots(const pyseqm_layout_t* L, const int* idx, int nact, const double* ring,
+                                long long ring_stride, int slot, int count, int nfock, double* Bmat, void* stream) {
+  if (!L || nact < 0 || count <= 0 || count > nfock || slot < 0 || slot >= nfock) return PYSEQM_EINVAL;
+  if (nact == 0) return PYSEQM_OK;
+  k_diis_dots<<<dim3(count, nact), kEl, 0, (hipStream_t)stream>>>(*L, idx, ring, (size_t)ring_stride, slot, nfock,
+                                                                   Bmat);
+  return status();
+}
+
+extern "C" int pyseqm_diis_combine(const pyseqm_layout_t* L, const int* idx, int nact, int nmax, const double* ring,
+                                   long long ring_stride, int count, const double* coef, int cstride, double* F,
+                                   void* stream) {
+  if (!L || nact < 0 || count <= 0) return PYSEQM_EINVAL;
+  if (nact == 0) return PYSEQM_OK;
+  k_diis_combine<<<dim3(elem_blocks(nmax), nact), kEl, 0, (hipStream_t)stream>>>(*L, idx, ring, (size_t)ring_stride,
+                                                                                  count, coef, cstride, F);
+  return status();
+}

@@ -1,0 +1,303 @@
+"""Device SCF engine: drives the gfx950 kernels of libpyseqm_hip.so on one packed batch.
+
+Host code only schedules kernels and reads one byte per molecule per SCF iteration (the
+convergence flags), like the reference's ``int(notconverged.sum())`` host sync
+(seqm/seqm_functions/scf_loop.py:1101).  Every floating-point array lives on the device.
+"""
+import math
+
+import numpy as np
+import torch
+
+from . import _native as nat
+from .layout import PackedBatch
+from .seqm_functions.constants import EHEAT, GP2C, GPPC, GSPC, GSSC, HSPC, TORE, UPPC, USSC, ev_kcalpmol
+from .seqm_functions.parameters import COLUMNS, param_table
+
+MAX_ITER = 1000
+NFOCK = 10
+
+
+class SCFResult:
+    pass
+
+
+class SCFEngine:
+    """Pair integrals, Fock builds, density steps and the SCF loop for one packed batch."""
+
+    def __init__(self, species, coordinates, method, device="cuda", charges=None, batch=None):
+        self.device = nat.require_device(device)
+        nat.lib()
+        self.method = method
+        self.b = batch if batch is not None else PackedBatch(species, method, self.device, charges)
+        b = self.b
+        dev = self.device
+        f64 = dict(dtype=torch.float64, device=dev)
+        tab = param_table(method)
+        par = np.zeros((b.natoms, nat.NPAR))
+        par[:, :len(COLUMNS)] = tab[b.Z]
+        self.atom_par = torch.as_tensor(par, **f64)
+        self.stream = nat.stream_ptr(dev)
+        nat.call("pyseqm_atom_setup", b.L, nat.ptr(self.atom_par), self.stream)
+        nt = b.ntiles
+        self.w_xx = torch.empty(max(b.ntile[0], 1) * 6400, **f64)
+        self.w_xh = torch.empty(max(b.ntile[1], 1) * 640, **f64)
+        self.w_hh = torch.empty(max(b.ntile[2], 1) * 64, **f64)
+        self.H = torch.empty(max(b.total, 1), **f64)
+        self.part = torch.empty(max(nt, 1) * 160, **f64)
+        self.tile_v = torch.empty(max(nt, 1), **f64)
+        self.atom_v = torch.empty(max(b.natoms, 1), **f64)
+        self.enuc = torch.zeros(b.nmol, **f64)
+        self.eelec = torch.zeros(b.nmol, **f64)
+        # isolated-atom energies and atomic heats of formation (geometry independent)
+        Z = b.Z
+        eiso = (tab[Z, 0] * np.take(USSC, Z) + tab[Z, 1] * np.take(UPPC, Z) + tab[Z, 6] * np.take(GSSC, Z)
+                + tab[Z, 8] * np.take(GPPC, Z) + tab[Z, 7] * np.take(GSPC, Z) + tab[Z, 9] * np.take(GP2C, Z)
+                + tab[Z, 10] * np.take(HSPC, Z))
+        self.eiso_atom = eiso
+        self.eiso = torch.as_tensor(np.bincount(b.atom_mol, eiso, minlength=b.nmol), **f64)
+        self.eheat = torch.as_tensor(np.bincount(b.atom_mol, np.take(EHEAT, Z) / ev_kcalpmol, minlength=b.nmol),
+                                     **f64)
+        self.coords = None
+        if coordinates is not None:
+            self.set_coordinates(coordinates)
+
+    # ------------------------------------------------------------- integrals
+    def set_coordinates(self, coordinates, enuc_pair=False):
+        """coordinates: (nmol, molsize, 3) Angstrom in the reference's padded layout, or (natoms, 3)."""
+        b = self.b
+        c = torch.as_tensor(coordinates, dtype=torch.float64, device=self.device)
+        if c.dim() == 3:
+            c = c.reshape(-1, 3)[torch.as_tensor(b.real_flat, device=self.device)]
+        self.coords = c.contiguous()
+        self.enuc_pair = torch.zeros(max(b.npairs, 1), dtype=torch.float64, device=self.device) if enuc_pair \
+            else None
+        nat.call("pyseqm_pair_integrals", b.L, nat.METHODS[self.method], nat.ptr(self.coords),
+                 nat.ptr(self.atom_par), nat.ptr(self.w_xx), nat.ptr(self.w_xh), nat.ptr(self.w_hh),
+                 nat.ptr(self.H), nat.ptr(self.part), nat.ptr(self.tile_v), nat.ptr(self.enuc),
+                 nat.ptr(self.enuc_pair), nat.ptr(b.d["pair_ref"]) if enuc_pair else None, self.stream)
+
+    # ------------------------------------------------------------------ fock
+    def fock(self, P, F, active=None, eelec=None):
+        b = self.b
+        out = self.eelec if eelec is None else eelec
+        nat.call("pyseqm_fock", b.L, nat.ptr(self.atom_par), nat.ptr(self.w_xx), nat.ptr(self.w_xh),
+                 nat.ptr(self.w_hh), nat.ptr(P), nat.ptr(self.H), nat.ptr(F), nat.ptr(self.part),
+                 nat.ptr(self.tile_v), nat.ptr(self.atom_v), nat.ptr(out), nat.ptr(active), self.stream)
+        return out
+
+    def initial_density(self):
+        """Diagonal guess (scf_loop.py:2066-2081): heavy atoms tore/4 on s and p, H 1.0."""
+        b = self.b
+        P = torch.zeros(max(b.total, 1), dtype=torch.float64, device=self.device)
+        didx, dmol = b.diag_index()
+        vals = []
+        for a in range(b.natoms):
+            vals.extend([TORE[b.Z[a]] / 4.0] * 4 if b.Z[a] > 1 else [1.0])
+        # diag_index is molecule-major in orbital order, the same order as the atoms
+        P[torch.as_tensor(didx, device=self.device)] = torch.as_tensor(vals, dtype=torch.float64,
+                                                                        device=self.device)
+        return P
+
+    # ---------------------------------------------------------- density step
+    def density_eigh(self, F, idx, nact, Pout, want_evecs=False):
+        """Batched FP64 eigensolve of the active Fock matrices -> Pout (diag.py:110-241)."""
+        b = self.b
+        nmax = int(b.norb[self._idx_host].max())
+        Fp = torch.empty(nact, nmax, nmax, dtype=torch.float64, device=self.device)
+        nat.call("pyseqm_pad_for_eigh", b.L, nat.ptr(F), nat.ptr(idx), nact, nmax, nat.ptr(Fp), self.stream)
+        e, V = torch.linalg.eigh(Fp)
+        V = V.contiguous()
+        nat.call("pyseqm_density_from_evecs", b.L, nat.ptr(V), nmax, nat.ptr(idx), nact, nat.ptr(Pout),
+                 self.stream)
+        if want_evecs:
+            return e, V, nmax
+        return None
+
+    def density(self, F, idx, nact, Pout):
+        if self.density_kind == "eigh":
+            self.density_eigh(F, idx, nact, Pout)
+        else:
+            from . import sp2
+            sp2.density_sp2(self, F, idx, nact, Pout)
+
+    # ------------------------------------------------------------------- SCF
+    def _active(self, notconv_host):
+        idx = np.nonzero(notconv_host)[0]
+        self._idx_host = idx
+        return torch.as_tensor(idx.astype(np.int32), device=self.device), len(idx)
+
+    def scf(self, eps=1e-8, converger=(2,), density="eigh", sp2_eps=1e-4, P=None, max_iter=MAX_ITER):
+        """scf_loop (scf_loop.py:2034-2394) with mixers [0] linear, [1] adaptive, [2] Pulay DIIS."""
+        b = self.b
+        dev = self.device
+        self.density_kind = density
+        self.sp2_eps = sp2_eps
+        f64 = dict(dtype=torch.float64, device=dev)
+        P = self.initial_density() if P is None else P.clone()
+        F = torch.empty_like(P)
+        Pnew = torch.zeros_like(P)
+        Pold = torch.zeros_like(P)
+        e_old = torch.zeros(b.nmol, **f64)
+        e_new = torch.zeros(b.nmol, **f64)
+        self.fock(P, F, eelec=e_old)
+        notconv = torch.ones(b.nmol, dtype=torch.uint8, device=dev)
+        nc_host = np.ones(b.nmol, dtype=bool)
+        kind = converger[0]
+        it = 0
+        if kind == 2:
+            res = self._pulay(P, F, Pnew, Pold, e_old, e_new, notconv, eps, max_iter)
+        elif kind in (0, 1):
+            res = self._simple(kind, converger, P, F, Pnew, Pold, e_old, e_new, notconv, eps, max_iter)
+        else:
+            raise ValueError("scf_converger must be [0, alpha], [1] or [2]")
+        P, it, nc_host = res
+        out = SCFResult()
+        out.P = P
+        out.iterations = it
+        out.notconverged = nc_host
+        return out
+
+    def _simple(self, kind, converger, P, F, Pnew, Pold, e_old, e_new, notconv, eps, max_iter):
+        b = self.b
+        nc_host = np.ones(b.nmol, dtype=bool)
+        it = 0
+        if kind == 1:
+            from .mixing import AdaptiveMixer
+            mixer = AdaptiveMixer(self)
+        krange = range(0, max_iter + 1) if kind == 0 else range(1, max_iter + 1)
+        for k in krange:
+            idx, nact = self._active(nc_host)
+            if nact == 0:
+                break
+            it += 1
+            nmax = int(b.norb[self._idx_host].max())
+            self.density(F, idx, nact, Pnew)
+            if kind == 0:
+                nat.call("pyseqm_mix", b.L, nat.ptr(idx), nact, nmax, float(converger[1]), nat.ptr(P),
+                         nat.ptr(Pnew), nat.ptr(Pold), self.stream)
+            else:
+                mixer.step(k, P, Pnew, Pold, idx, self._idx_host)
+            act = self._act_mask(nc_host)
+            self.fock(P, F, active=act, eelec=e_new)
+            nat.call("pyseqm_scf_check", b.L, nat.ptr(idx), nact, nat.ptr(P), nat.ptr(Pold), nat.ptr(e_new),
+                     nat.ptr(e_old), None, float(eps), nat.ptr(notconv), None, self.stream)
+            nc_host = notconv.cpu().numpy().astype(bool)
+        return P, it, nc_host
+
+    def _act_mask(self, nc_host):
+        return torch.as_tensor(nc_host.astype(np.uint8), device=self.device)
+
+    def _pulay(self, P, F, Pnew, Pold, e_old, e_new, notconv, eps, max_iter):
+        """scf_forward2 (scf_loop.py:639-1132): Pulay DIIS on FP - PF with a 10-deep history,
+        batch-global history counter and reset on ill-conditioning (cond > 1e7)."""
+        b = self.b
+        dev = self.device
+        total = max(b.total, 1)
+        f64 = dict(dtype=torch.float64, device=dev)
+        Fring = torch.zeros(NFOCK, total, **f64)
+        Cring = torch.zeros(NFOCK, total, **f64)
+        X = torch.empty(total, **f64)
+        diis_err = torch.full((b.nmol,), torch.finfo(torch.float64).max, **f64)
+        emat0 = torch.tril(torch.eye(NFOCK + 1, **f64) - 1.0)
+        EMAT = emat0.expand(b.nmol, -1, -1).contiguous()
+        coef = torch.zeros(b.nmol, NFOCK, **f64)
+        nc_host = np.ones(b.nmol, dtype=bool)
+        counter, cF = -1, 0
+        it = 0
+        for k in range(max_iter + 1):
+            idx, nact = self._active(nc_host)
+            if nact == 0:
+                break
+            it += 1
+            nmax = int(b.norb[self._idx_host].max())
+            cF = min(cF + 1, NFOCK)
+            counter = (counter + 1) % NFOCK
+            Fslot = Fring[counter]
+            nat.call("pyseqm_copy_active", b.L, nat.ptr(idx), nact, nmax, nat.ptr(F), nat.ptr(Fslot), self.stream)
+            nat.call("pyseqm_diis_residual", b.L, nat.ptr(idx), nact, nmax, nat.ptr(F), nat.ptr(P), nat.ptr(X),
+                     nat.ptr(Cring[counter]), nat.ptr(diis_err), self.stream)
+            nat.call("pyseqm_diis_dots", b.L, nat.ptr(idx), nact, nat.ptr(Cring), total, counter, cF, NFOCK,
+                     nat.ptr(EMAT), self.stream)
+            reset = False
+            if cF >= 2:
+                il = idx.long()
+                EV = EMAT[il, :cF + 1, :cF + 1].clone()
+                den = EV[:, counter, counter].clamp(min=1e-15)
+                EV[:, :cF, :cF] /= den.view(-1, 1, 1)
+                # the reference's eigh reads only the lower triangle (stale upper entries included);
+                # mirror it explicitly so the result does not depend on the solver's UPLO handling
+                EV = torch.tril(EV) + torch.tril(EV, -1).transpose(1, 2)
+                Lw, Q = torch.linalg.eigh(EV)
+                absL = Lw.abs()
+                cond = absL.amax(-1) / absL.amin(-1)
+                inv = torch.where(absL > 1e-13, 1.0 / torch.where(absL > 1e-13, Lw, torch.ones_like(Lw)),
+                                  torch.zeros_like(Lw))
+                c = -torch.einsum("bki,bi,bi->bk", Q[:, :cF, :], inv, Q[:, -1, :])
+                coef[il, :cF] = c
+                reset = bool((cond > 1e7).any())
+                nat.call("pyseqm_diis_combine", b.L, nat.ptr(idx), nact, nmax, nat.ptr(Fring), total, cF,
+                         nat.ptr(coef), NFOCK, nat.ptr(F), self.stream)
+            self.density(F, idx, nact, Pnew)
+            alpha = 0.5 if cF < 2 else 0.0
+            nat.call("pyseqm_mix", b.L, nat.ptr(idx), nact, nmax, alpha, nat.ptr(P), nat.ptr(Pnew),
+                     nat.ptr(Pold), self.stream)
+            act = self._act_mask(nc_host)
+            self.fock(P, F, active=act, eelec=e_new)
+            nat.call("pyseqm_scf_check", b.L, nat.ptr(idx), nact, nat.ptr(P), nat.ptr(Pold), nat.ptr(e_new),
+                     nat.ptr(e_old), nat.ptr(diis_err), float(eps), nat.ptr(notconv), None, self.stream)
+            nc_host = notconv.cpu().numpy().astype(bool)
+            if reset:
+                counter, cF = -1, 0
+                EMAT.copy_(emat0.expand(b.nmol, -1, -1))
+        return P, it, nc_host
+
+    # -------------------------------------------------------------- results
+    def energies(self, P):
+        """Final F = fock(P), Eelec, Enuc, Etot, Eiso, Hf per molecule (basics.py:862-892, 1220-1239)."""
+        F = torch.empty_like(P)
+        eelec = torch.zeros(self.b.nmol, dtype=torch.float64, device=self.device)
+        self.fock(P, F, eelec=eelec)
+        etot = eelec + self.enuc
+        hf = etot - self.eiso + self.eheat
+        return dict(F=F, Eelec=eelec, Enuc=self.enuc.clone(), Etot=etot, Eiso=self.eiso.clone(), Hf=hf)
+
+    def eig(self, F):
+        """Orbital energies in the reference layout e (nmol, 4*molsize), zero padded, and the packed
+        eigenvectors (nmol, nmax, nmax), sym_eig_trunc(eig_only=True) (diag.py:110-241)."""
+        b = self.b
+        self._idx_host = np.arange(b.nmol)
+        idx = torch.arange(b.nmol, dtype=torch.int32, device=self.device)
+        nmax = b.nmax
+        Fp = torch.empty(b.nmol, nmax, nmax, dtype=torch.float64, device=self.device)
+        nat.call("pyseqm_pad_for_eigh", b.L, nat.ptr(F), nat.ptr(idx), b.nmol, nmax, nat.ptr(Fp), self.stream)
+        e0, V = torch.linalg.eigh(Fp)
+        e = torch.zeros(b.nmol, 4 * b.molsize, dtype=torch.float64, device=self.device)
+        mask = torch.arange(nmax, device=self.device)[None, :] < torch.as_tensor(b.norb, device=self.device)[:, None]
+        e[:, :nmax] = torch.where(mask, e0, torch.zeros_like(e0))
+        return e, V
+
+    # ------------------------------------------------- reference-order views
+    def w_reference(self):
+        """Two-center integrals as the reference's dense (npairs, 10, 10) array (hcore.py:9-179)."""
+        b = self.b
+        dev = self.device
+        w = torch.zeros(max(b.npairs, 1), 10, 10, dtype=torch.float64, device=dev)
+        ref = torch.as_tensor(b.pair_ref_host, device=dev).long()
+        valid = torch.as_tensor(b.lane_valid, device=dev)
+        nt = b.ntile
+        lo = 0
+        for kind, buf, nw in ((0, self.w_xx, 100), (1, self.w_xh, 10), (2, self.w_hh, 1)):
+            if nt[kind] == 0:
+                continue
+            rows = buf[:nt[kind] * nw * 64].view(nt[kind], nw, 64).permute(0, 2, 1).reshape(-1, nw)
+            sel = slice(lo * 64, (lo + nt[kind]) * 64)
+            v, r = valid[sel], ref[sel]
+            if kind == 0:
+                w[r[v]] = rows[v].view(-1, 10, 10)
+            elif kind == 1:
+                w[r[v], :, 0] = rows[v]
+            else:
+                w[r[v], 0, 0] = rows[v][:, 0]
+            lo += nt[kind]
+        return w[:b.npairs]

@@ -1,0 +1,235 @@
+"""Packed, padding-free batch layout on the device.
+
+Builds, from the reference's padded ``species`` (nmol, molsize) input, everything the gfx950
+kernels index with: per-molecule atom / orbital / matrix offsets, the 8x8 pair tiles of the three
+pair kinds (heavy-heavy, heavy-H, H-H) and the per-atom lists of tile partial slots.  Parsing
+follows the reference Parser (seqm/basics.py:219-403): atoms are the non-zero entries of each
+row in order, pairs are i < j within a molecule in row-major order (the reference pair index
+is kept in ``pair_ref`` for outputs in reference order), nocc = sum(tore)/2 - charge/2.
+"""
+import numpy as np
+import torch
+
+from . import _native
+from .seqm_functions.constants import QN, TORE, MAX_Z
+from .seqm_functions.parameters import supported_elements
+
+TILE = 8
+
+
+def check_species(species):
+    """Rows must be non-increasing (seqm/Molecule.py:188-206); same message as the reference."""
+    ok = np.all(species[:, :-1] >= species[:, 1:], axis=1)
+    if not ok.all():
+        bad = np.nonzero(~ok)[0].tolist()
+        rows = ", ".join(map(str, bad))
+        word, verb = ("row", "is") if len(bad) == 1 else ("rows", "are")
+        raise ValueError(f"species must be non-increasing along each row, but {word} {rows} {verb} not sorted.")
+
+
+def _region(mol, a0, r_start, nr, c_start, nc, tri):
+    """Tiles covering rows [r_start, r_start+nr) x cols [c_start, c_start+nc) of molecule `mol`
+    (local atom indices); tri: upper triangle i < j of a square region."""
+    nbr, nbc = -(-nr // TILE), -(-nc // TILE)
+    if nbr == 0 or nbc == 0:
+        return np.zeros((0, 6), dtype=np.int64)
+    bi, bj = np.meshgrid(np.arange(nbr), np.arange(nbc), indexing="ij")
+    bi, bj = bi.ravel(), bj.ravel()
+    if tri:
+        keep = bj >= bi
+        bi, bj = bi[keep], bj[keep]
+    nrow = np.minimum(TILE, nr - TILE * bi)
+    ncol = np.minimum(TILE, nc - TILE * bj)
+    diag = (bi == bj) & tri
+    keep = ~(diag & (nrow < 2))  # a diagonal tile with one atom has no pair
+    t = np.stack([np.full(len(bi), mol), a0 + r_start + TILE * bi, a0 + c_start + TILE * bj, nrow, ncol,
+                  diag.astype(np.int64)], 1)
+    return t[keep]
+
+
+class PackedBatch:
+    """Host description + device tensors of one packed batch."""
+
+    def __init__(self, species, method, device, charges=None):
+        sp = species.detach().cpu().numpy() if torch.is_tensor(species) else np.asarray(species)
+        sp = sp.astype(np.int64)
+        if sp.ndim == 1:
+            sp = sp[None]
+        check_species(sp)
+        self.method = method
+        self.device = torch.device(device)
+        self.species = sp
+        nmol, molsize = sp.shape
+        self.nmol, self.molsize = nmol, molsize
+        real = sp > 0
+        flat = np.nonzero(real.reshape(-1))[0]
+        self.real_flat = flat                      # padded flat index of each atom
+        self.Z = sp.reshape(-1)[flat]
+        self.natoms = len(flat)
+        if self.natoms and (self.Z.max() > MAX_Z or np.any(~np.isin(self.Z, supported_elements(method)))):
+            raise ValueError("\nError from diat.py, overlap matrix\nSome elements are not supported yet")
+        self.atom_mol = flat // molsize
+        self.atom_loc = flat % molsize
+        self.nat = real.sum(1)
+        self.nheavy = (sp > 1).sum(1)
+        self.nhydro = (sp == 1).sum(1)
+        tore = np.asarray(TORE, dtype=np.float64)
+        nel = tore[sp].sum(1).astype(np.int64)
+        if charges is not None:
+            ch = charges.detach().cpu().numpy() if torch.is_tensor(charges) else np.asarray(charges)
+            nel = nel - np.broadcast_to(ch, (nmol,)).astype(np.int64)
+        if np.any(nel % 2 == 1):
+            raise ValueError("RHF setting requires closed shell systems (even number of electrons)")
+        self.nocc = nel // 2
+        self.norb = 4 * self.nheavy + self.nhydro
+        self.mol_atom_off = np.concatenate([[0], np.cumsum(self.nat)]).astype(np.int64)
+        n2 = self.norb.astype(np.int64) ** 2
+        self.mol_mat_off = np.concatenate([[0], np.cumsum(n2)])[:-1].astype(np.int64)
+        self.total = int(n2.sum())
+        self.nmax = int(self.norb.max()) if nmol else 0
+        loc = self.atom_loc
+        nh_of = self.nheavy[self.atom_mol]
+        self.atom_ao = np.where(self.Z > 1, 4 * loc, 4 * nh_of + (loc - nh_of)).astype(np.int64)
+        self.npairs_mol = self.nat * (self.nat - 1) // 2
+        self.pair_base = np.concatenate([[0], np.cumsum(self.npairs_mol)]).astype(np.int64)
+        self.npairs = int(self.pair_base[-1])
+        self._build_tiles()
+        self._to_device()
+
+    # ------------------------------------------------------------------ tiles
+    def _build_tiles(self):
+        kinds = []
+        for kind in range(3):
+            per_mol = []
+            for m in range(self.nmol):
+                a0, nh, nH = int(self.mol_atom_off[m]), int(self.nheavy[m]), int(self.nhydro[m])
+                if kind == 0:
+                    per_mol.append(_region(m, a0, 0, nh, 0, nh, True))
+                elif kind == 1:
+                    per_mol.append(_region(m, a0, 0, nh, nh, nH, False))
+                else:
+                    per_mol.append(_region(m, a0, nh, nH, nh, nH, True))
+            counts = np.array([len(t) for t in per_mol], dtype=np.int64)
+            tiles = np.concatenate(per_mol) if len(per_mol) else np.zeros((0, 6), np.int64)
+            kinds.append((tiles, np.concatenate([[0], np.cumsum(counts)])))
+        self.ntile = [len(k[0]) for k in kinds]
+        base = np.cumsum([0] + self.ntile)
+        self.tile_base = base[:3]
+        T = np.concatenate([k[0] for k in kinds]) if sum(self.ntile) else np.zeros((0, 6), np.int64)
+        self.tiles_host = T
+        self.mol_tile_off = np.concatenate([k[1] + b for k, b in zip(kinds, base[:3])]).astype(np.int64)
+        nt = len(T)
+        r = np.arange(64) // TILE
+        c = np.arange(64) % TILE
+        i = T[:, 1:2] + r[None]
+        j = T[:, 2:3] + c[None]
+        valid = (r[None] < T[:, 3:4]) & (c[None] < T[:, 4:5]) & ((T[:, 5:6] == 0) | (c[None] > r[None]))
+        mol = T[:, 0:1]
+        a0 = self.mol_atom_off[mol]
+        li, lj = i - a0, j - a0
+        nat = self.nat[mol]
+        ref = self.pair_base[mol] + li * nat - li * (li + 1) // 2 + (lj - li - 1)
+        self.pair_ref_host = np.where(valid, ref, -1).reshape(-1)
+        self.lane_valid = valid.reshape(-1)
+        # partial slots: row r of tile t <-> atom row0 + r, column c <-> atom col0 + c
+        tid = np.repeat(np.arange(nt), 16)
+        s = np.tile(np.arange(16), nt)
+        atom = np.where(s < 8, T[tid, 1] + s, T[tid, 2] + (s - 8))
+        inrange = np.where(s < 8, s < T[tid, 3], (s - 8) < T[tid, 4])
+        atom, slot = atom[inrange], (tid * 16 + s)[inrange]
+        order = np.lexsort((slot, atom))
+        self.csr_idx_host = slot[order].astype(np.int64)
+        self.csr_off_host = np.searchsorted(atom[order], np.arange(self.natoms + 1)).astype(np.int64)
+        self.tile_kind = np.concatenate([np.full(n, k) for k, n in enumerate(self.ntile)]).astype(np.int64)
+
+    # ----------------------------------------------------------------- device
+    def _to_device(self):
+        dev = self.device
+        i32 = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=torch.int32, device=dev)
+        self.d = {
+            "mol_atom_off": i32(self.mol_atom_off), "mol_norb": i32(self.norb), "mol_nocc": i32(self.nocc),
+            "mol_mat_off": torch.as_tensor(self.mol_mat_off, dtype=torch.int64, device=dev),
+            "mol_tile_off": i32(self.mol_tile_off), "atom_mol": i32(self.atom_mol), "atom_ao": i32(self.atom_ao),
+            "atom_Z": i32(self.Z), "csr_off": i32(self.csr_off_host), "csr_idx": i32(self.csr_idx_host),
+            "pair_ref": i32(self.pair_ref_host),
+        }
+        T = self.tiles_host
+        packed = np.zeros((len(T), 4), dtype=np.int64)
+        if len(T):
+            packed[:, 0] = T[:, 0]
+            packed[:, 1] = T[:, 1]
+            packed[:, 2] = T[:, 2]
+            packed[:, 3] = T[:, 3] | (T[:, 4] << 8) | (T[:, 5] << 16)
+        self.d["tiles"] = i32(packed.reshape(-1)) if len(T) else torch.zeros(4, dtype=torch.int32, device=dev)
+        L = _native.Layout()
+        L.nmol, L.natoms, L.npairs = self.nmol, self.natoms, self.npairs
+        for k in range(3):
+            L.ntile[k] = self.ntile[k]
+        for name in ("mol_atom_off", "mol_norb", "mol_nocc", "mol_mat_off", "mol_tile_off", "atom_mol",
+                     "atom_ao", "atom_Z", "tiles", "csr_off", "csr_idx"):
+            setattr(L, name, self.d[name].data_ptr())
+        self.L = L
+
+    # ------------------------------------------------------------- helpers
+    @property
+    def ntiles(self):
+        return sum(self.ntile)
+
+    def padded_orbital_index(self):
+        """For each packed orbital (molecule-major): (mol, padded row in the reference 4*molsize layout)."""
+        mols, rows = [], []
+        for a in range(self.natoms):
+            m, l = self.atom_mol[a], self.atom_loc[a]
+            k = 4 if self.Z[a] > 1 else 1
+            mols.extend([m] * k)
+            rows.extend(range(4 * l, 4 * l + k))
+        return np.asarray(mols), np.asarray(rows)
+
+    def matrix_index(self):
+        """Flat packed index + (mol, padded row, padded col) of every matrix element."""
+        if not hasattr(self, "_midx"):
+            om, orow = self.padded_orbital_index()
+            orb_off = np.concatenate([[0], np.cumsum(self.norb)])
+            flat, mm, rr, cc = [], [], [], []
+            for m in range(self.nmol):
+                n = int(self.norb[m])
+                rows = orow[orb_off[m]:orb_off[m] + n]
+                ii, jj = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+                flat.append(self.mol_matoff(m) + (ii * n + jj).ravel())
+                mm.append(np.full(n * n, m))
+                rr.append(rows[ii.ravel()])
+                cc.append(rows[jj.ravel()])
+            self._midx = tuple(np.concatenate(x) for x in (flat, mm, rr, cc))
+        return self._midx
+
+    def mol_matoff(self, m):
+        return int(self.mol_mat_off[m])
+
+    def diag_index(self):
+        """Flat packed index of every diagonal element and its molecule."""
+        idx, mol = [], []
+        for m in range(self.nmol):
+            n = int(self.norb[m])
+            idx.append(self.mol_mat_off[m] + np.arange(n) * (n + 1))
+            mol.append(np.full(n, m))
+        return np.concatenate(idx), np.concatenate(mol)
+
+    def to_padded(self, X):
+        """Packed matrices (flat) -> reference layout (nmol, 4*molsize, 4*molsize) on the device."""
+        flat, mm, rr, cc = (torch.as_tensor(a, device=X.device) for a in self.matrix_index())
+        out = torch.zeros(self.nmol, 4 * self.molsize, 4 * self.molsize, dtype=X.dtype, device=X.device)
+        out[mm, rr, cc] = X[flat]
+        return out
+
+    def from_padded(self, Xp):
+        flat, mm, rr, cc = (torch.as_tensor(a, device=Xp.device) for a in self.matrix_index())
+        out = torch.zeros(self.total, dtype=Xp.dtype, device=Xp.device)
+        out[flat] = Xp[mm, rr, cc]
+        return out
+
+    def atom_to_padded(self, v, fill=0.0):
+        """Per-atom (natoms, ...) -> (nmol, molsize, ...) in the reference's padded atom layout."""
+        shape = (self.nmol * self.molsize,) + tuple(v.shape[1:])
+        out = torch.full(shape, fill, dtype=v.dtype, device=v.device)
+        out[torch.as_tensor(self.real_flat, device=v.device)] = v
+        return out.reshape((self.nmol, self.molsize) + tuple(v.shape[1:]))
