@@ -70,6 +70,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--fock-reps", type=int, default=20)
+    ap.add_argument("--compare", action="store_true",
+                    help="also run the FP64 diagonalisation path once: speedup and |dHf|/atom")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (this path is "
@@ -82,6 +84,8 @@ def main():
 
     method, sp, xyz = make_workload(args.config, args.seed)
     conv = (2,)
+    if args.density == "sp2" and args.eps == 1e-8:
+        args.eps = 1e-5  # energy tolerance of the mixed-precision path (its precision floor is ~1e-8 eV/atom)
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     eng = SCFEngine(sp, None, method, dev)
@@ -108,6 +112,22 @@ def main():
     ms = dt / args.steps * 1e3
     value = nconv / dt
 
+    # SP2 GEMM roofline (one more timed step with events around the SP2 GEMM loops)
+    sp2_roof = None
+    if args.density == "sp2":
+        from pyseqm_amd import sp2 as sp2mod
+        eng._sp2ws.timing = []
+        step()
+        torch.cuda.synchronize()
+        t_ms = sum(e0.elapsed_time(e1) for e0, e1, *_ in eng._sp2ws.timing)
+        flops = sum(sp2mod.gemm_flops(eng, ix, it) for _, _, ix, _, it in eng._sp2ws.timing)
+        eng._sp2ws.timing = None
+        ach = flops / (t_ms * 1e-3) / 1e12
+        peak = 2500.0 / 3.0
+        sp2_roof = {"kernel": "k_sp2_bf16x3 (SP2 X^2, bf16x3 MFMA, FP32 accumulate, fused branch/trace)",
+                    "bound": "mfma", "achieved": ach, "peak": peak, "unit": "TFLOP/s", "frac": ach / peak,
+                    "traffic": None, "algorithmic_flops": flops, "gemm_loop_ms": t_ms,
+                    "note": "peak = dense bf16 2.5 PFLOP/s / 3 products; flops = 2 n^3 per SP2 iteration"}
     # Fock-build roofline: HIP events on the stream the kernels are launched on
     P = r.P
     F = torch.empty_like(P)
@@ -126,7 +146,8 @@ def main():
     line = {
         "metric": "converged SCF/s", "value": value, "unit": "molecules/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if args.density == "eigh" else "bf16x3 (fp32 accumulate) + f64 McWeeny", "data": "synthetic",
         "config": {"workload": f"{args.config}: {nmol} {method} molecules "
                                f"({'alkanes n~U[2,30]' if args.config == 'C2' else args.config}), "
                                f"density step {args.density}, Pulay DIIS, eps {args.eps:g} eV",
@@ -138,6 +159,22 @@ def main():
                      "traffic": None, "algorithmic_bytes": nbytes, "launch_ms": fock_ms,
                      "pairs": {"XX": nxx, "XH": nxh, "HH": nhh}},
     }
+    if sp2_roof is not None:
+        line["roofline_fock"] = line["roofline"]
+        line["roofline"] = sp2_roof
+    if args.compare:
+        e2 = SCFEngine(sp, None, method, dev)
+        e2.set_coordinates(coords)
+        torch.cuda.synchronize()
+        tc = time.perf_counter()
+        r2 = e2.scf(eps=1e-8, converger=conv, density="eigh")
+        en2 = e2.energies(r2.P)
+        torch.cuda.synchronize()
+        tc = time.perf_counter() - tc
+        dhf = (en["Hf"] - en2["Hf"]).abs().cpu().numpy() / eng.b.nat
+        line["compare_fp64_eigh"] = {"ms_per_step": tc * 1e3, "speedup": tc * 1e3 / ms,
+                                     "max_abs_dHf_per_atom_eV": float(dhf.max()),
+                                     "scf_iterations": r2.iterations}
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(method, sp, xyz, args.cpu_sample, args.eps, conv)
     print(json.dumps(line), flush=True)

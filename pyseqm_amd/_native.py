@@ -39,13 +39,21 @@ _SIGS = {
     "pyseqm_density_from_evecs": [c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p],
     "pyseqm_mix": [c_void_p, c_int, c_int, c_double, c_void_p, c_void_p, c_void_p, c_void_p],
     "pyseqm_copy_active": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p],
-    "pyseqm_scf_check": [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_void_p,
-                         c_void_p, c_void_p],
+    "pyseqm_scf_check": [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_double,
+                         c_void_p, c_void_p, c_void_p],
     "pyseqm_gemm": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p],
     "pyseqm_diis_residual": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
     "pyseqm_diis_dots": [c_void_p, c_int, c_void_p, c_longlong, c_int, c_int, c_int, c_void_p, c_void_p],
     "pyseqm_diis_combine": [c_void_p, c_int, c_int, c_void_p, c_longlong, c_int, c_void_p, c_int, c_void_p,
                             c_void_p],
+    "pyseqm_sp2_init": [c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
+    "pyseqm_sp2_iterate": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                           c_int, c_double, c_int, c_void_p, c_void_p],
+    "pyseqm_dgemm_sym": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
+                         c_void_p],
+    "pyseqm_sp2_x_to_f64": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                            c_void_p],
+    "pyseqm_sp2_unpad": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
 }
 _OPTIONAL = {}
 _lib = None

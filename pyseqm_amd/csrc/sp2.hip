@@ -1,0 +1,470 @@
+// SP2 density purification on MFMA (gfx950) and the FP64 MFMA GEMM it is corrected with.
+//
+// Trace-correcting SP2 (seqm/seqm_functions/SP2.py:9-85): X0 = (hN I - F)/(hN - h1) from the
+// Gershgorin bounds, then X <- X^2 if |tr X^2 - N| < |2 tr X - tr X^2 - N| else 2X - X^2, until the
+// trace error stops improving.  Here
+//   * X is kept in FP32 in a zero-padded per-molecule block (leading dimension ld = n rounded up
+//     to 32, so K never needs a tail and rows are 16-byte aligned);
+//   * X^2 runs on v_mfma_f32_32x32x16_bf16 as the 3-term split hi*hi + hi*lo + lo*hi
+//     (X = hi + lo, both bf16) with FP32 accumulation;
+//   * X is symmetric: only tiles ti <= tj are computed, the epilogue applies the branch, writes the
+//     tile and its mirror, and returns FP64 partial sums of tr(X') and ||X'||_F^2 = tr(X'^2), which
+//     is exactly the next iteration's branch test, so no second pass over X is needed;
+//   * the branch and stopping decisions are taken on the device (one thread per molecule);
+//   * a final McWeeny step X <- 3X^2 - 2X^3 in FP64 (v_mfma_f64_16x16x4_f64) restores FP64
+//     idempotency: P = 2 (3 X^2 - 2 X^3).
+#include "common.h"
+
+using namespace pyseqm;
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+
+namespace {
+
+// per-molecule SP2 state, doubles
+enum { S_TR = 0, S_FRO, S_E0, S_E1, S_E2, S_BRANCH, S_DONE, S_IT, S_NST };
+
+__device__ inline double wsum(double v) {
+  for (int o = 1; o < 64; o <<= 1) v += __shfl_xor(v, o);
+  return v;
+}
+__device__ inline double wmax(double v) {
+  for (int o = 1; o < 64; o <<= 1) v = fmax(v, __shfl_xor(v, o));
+  return v;
+}
+
+// ------------------------------------------------------------------ init
+// X0 (FP32, padded) and the FP64 trace / Frobenius norm of the rounded X0.  One block per molecule.
+__global__ void k_sp2_init(pyseqm_layout_t L, const double* __restrict__ F, const int* __restrict__ idx,
+                           const long long* __restrict__ xoff, const int* __restrict__ xld, float* __restrict__ X,
+                           double* __restrict__ st) {
+  __shared__ double sh[4][4];
+  const int m = idx[blockIdx.x];
+  const int n = L.mol_norb[m];
+  const int ld = xld[m];
+  const double* Fm = F + L.mol_mat_off[m];
+  float* Xm = X + xoff[m];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  double hi = -1.0e300, lo = 1.0e300;
+  for (int r = threadIdx.x; r < n; r += blockDim.x) {
+    double s = 0.0;
+    for (int c = 0; c < n; ++c) s += fabs(Fm[(size_t)r * n + c]);
+    const double d = Fm[(size_t)r * n + r];
+    hi = fmax(hi, d + (s - fabs(d)));
+    lo = fmin(lo, d - (s - fabs(d)));
+  }
+  hi = wmax(hi);
+  lo = -wmax(-lo);
+  if (lane == 0) {
+    sh[0][w] = hi;
+    sh[1][w] = lo;
+  }
+  __syncthreads();
+  double hN = sh[0][0], h1 = sh[1][0];
+  for (int k = 1; k < (int)(blockDim.x >> 6); ++k) {
+    hN = fmax(hN, sh[0][k]);
+    h1 = fmin(h1, sh[1][k]);
+  }
+  const double inv = 1.0 / (hN - h1);
+  double tr = 0.0, fro = 0.0;
+  for (size_t e = threadIdx.x; e < (size_t)ld * ld; e += blockDim.x) {
+    const int r = (int)(e / ld), c = (int)(e % ld);
+    float v = 0.0f;
+    if (r < n && c < n) v = (float)(((r == c ? hN : 0.0) - Fm[(size_t)r * n + c]) * inv);
+    Xm[e] = v;
+    fro += (double)v * v;
+    if (r == c) tr += v;
+  }
+  tr = wsum(tr);
+  fro = wsum(fro);
+  __syncthreads();
+  if (lane == 0) {
+    sh[2][w] = tr;
+    sh[3][w] = fro;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0, f = 0.0;
+    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) {
+      t += sh[2][k];
+      f += sh[3][k];
+    }
+    double* s = st + (size_t)m * S_NST;
+    const double N = L.mol_nocc[m];
+    s[S_TR] = t;
+    s[S_FRO] = f;
+    s[S_E0] = s[S_E1] = s[S_E2] = fabs(t - N);
+    s[S_BRANCH] = fabs(f - N) < fabs(2.0 * t - f - N) ? 1.0 : 0.0;
+    s[S_DONE] = 0.0;
+    s[S_IT] = 0.0;
+  }
+}
+
+// ------------------------------------------------------------- BF16x3 GEMM
+constexpr int BT = 128;       // output tile edge
+constexpr int BK = 32;        // K slab
+constexpr int LDSK = BK + 8;  // padded LDS row (bf16): 80 bytes, 16-byte aligned
+
+// upper-triangle tile enumeration t = tj*(tj+1)/2 + ti, ti <= tj
+__device__ inline void tri_decode(int t, int& ti, int& tj) {
+  tj = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+  while ((tj + 1) * (tj + 2) / 2 <= t) ++tj;
+  while (tj * (tj + 1) / 2 > t) --tj;
+  ti = t - tj * (tj + 1) / 2;
+}
+
+__global__ __launch_bounds__(256) void k_sp2_bf16x3(const int* __restrict__ idx, const int* __restrict__ norb,
+                                                    const long long* __restrict__ xoff,
+                                                    const int* __restrict__ xld, const float* __restrict__ Xin,
+                                                    float* __restrict__ Xout, double* __restrict__ st,
+                                                    double* __restrict__ part, int part_stride) {
+  __shared__ __attribute__((aligned(16))) __bf16 As[2][BT][LDSK];  // [hi/lo][row][k]
+  __shared__ __attribute__((aligned(16))) __bf16 Bs[2][BT][LDSK];
+  __shared__ double red[2][4];
+  const int m = idx[blockIdx.y];
+  const double* s = st + (size_t)m * S_NST;
+  if (s[S_DONE] != 0.0) return;
+  const int n = norb[m];
+  const int ld = xld[m];
+  int ti, tj;
+  tri_decode(blockIdx.x, ti, tj);
+  if (tj * BT >= n) return;
+  const bool sq = s[S_BRANCH] != 0.0;
+  const float* Xm = Xin + xoff[m];
+  float* Ym = Xout + xoff[m];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = wave >> 1, wc = wave & 1;  // 2x2 waves, 64x64 each
+  const int i0 = ti * BT, j0 = tj * BT;
+  floatx16 acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) acc[a][b][v] = 0.0f;
+
+  // global -> registers (next slab, in flight while the MFMAs of this slab run) -> LDS as bf16 hi/lo
+  float4 ra[4], rb[4];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int f = tid + 256 * q;  // float4 index in the 128 x 8 slab
+      const int row = f >> 3, kq = f & 7;
+      ra[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+      rb[q] = ra[q];
+      if (i0 + row < n) ra[q] = *reinterpret_cast<const float4*>(Xm + (size_t)(i0 + row) * ld + k0 + 4 * kq);
+      if (j0 + row < n) rb[q] = *reinterpret_cast<const float4*>(Xm + (size_t)(j0 + row) * ld + k0 + 4 * kq);
+    }
+  };
+  auto split_store = [&](const float4& v, __bf16 (*hi)[LDSK], __bf16 (*lo)[LDSK], int row, int col) {
+    union {
+      __bf16 h[4];
+      uint2 u;
+    } ph, pl;
+    const float x[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      ph.h[u] = (__bf16)x[u];
+      pl.h[u] = (__bf16)(x[u] - (float)ph.h[u]);
+    }
+    *reinterpret_cast<uint2*>(&hi[row][col]) = ph.u;
+    *reinterpret_cast<uint2*>(&lo[row][col]) = pl.u;
+  };
+  gload(0);
+  for (int k0 = 0; k0 < ld; k0 += BK) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int f = tid + 256 * q;
+      const int row = f >> 3, kq = f & 7;
+      split_store(ra[q], As[0], As[1], row, 4 * kq);
+      split_store(rb[q], Bs[0], Bs[1], row, 4 * kq);
+    }
+    __syncthreads();
+    if (k0 + BK < ld) gload(k0 + BK);
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 16) {
+      bf16x8 ah[2], al[2], bh[2], bl[2];
+      const int kofs = kk + 8 * (lane >> 5);
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+        const int r = wr * 64 + a * 32 + (lane & 31);
+        ah[a] = *reinterpret_cast<const bf16x8*>(&As[0][r][kofs]);
+        al[a] = *reinterpret_cast<const bf16x8*>(&As[1][r][kofs]);
+        const int c = wc * 64 + a * 32 + (lane & 31);
+        bh[a] = *reinterpret_cast<const bf16x8*>(&Bs[0][c][kofs]);
+        bl[a] = *reinterpret_cast<const bf16x8*>(&Bs[1][c][kofs]);
+      }
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[a], bh[b], acc[a][b], 0, 0, 0);
+          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a], bl[b], acc[a][b], 0, 0, 0);
+          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a], bh[b], acc[a][b], 0, 0, 0);
+        }
+    }
+    __syncthreads();
+  }
+  // epilogue: Y = X^2 or 2X - X^2, mirror write, FP64 partial trace / Frobenius norm
+  double tr = 0.0, fro = 0.0;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const int r = i0 + wr * 64 + a * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+        const int c = j0 + wc * 64 + b * 32 + (lane & 31);
+        if (r < n && c < n && (ti != tj || r <= c)) {
+          const float x2 = acc[a][b][v];
+          const float y = sq ? x2 : 2.0f * Xm[(size_t)r * ld + c] - x2;
+          Ym[(size_t)r * ld + c] = y;
+          Ym[(size_t)c * ld + r] = y;
+          const double w = (r == c) ? 1.0 : 2.0;
+          fro += w * (double)y * (double)y;
+          if (r == c) tr += y;
+        }
+      }
+  tr = wsum(tr);
+  fro = wsum(fro);
+  if (lane == 0) {
+    red[0][wave] = tr;
+    red[1][wave] = fro;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double* p = part + (size_t)m * part_stride + 2 * blockIdx.x;
+    p[0] = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+    p[1] = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+  }
+}
+
+// Sum the tile partials (fixed order), update the trace errors, decide the next branch and
+// whether the molecule stops: err < eps and no improvement over two iterations ago (the
+// reference's reduced-precision criterion, SP2.py:75-79), or iteration cap.
+__global__ void k_sp2_decide(const int* __restrict__ idx, int nact, const int* __restrict__ norb,
+                             const int* __restrict__ nocc, double* __restrict__ st, const double* __restrict__ part,
+                             int part_stride, double eps, int max_iter, int* __restrict__ ndone) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nact) return;
+  const int m = idx[k];
+  double* s = st + (size_t)m * S_NST;
+  if (s[S_DONE] != 0.0) return;
+  const int n = norb[m];
+  const int nt = (n + BT - 1) / BT;
+  const int ntri = nt * (nt + 1) / 2;
+  double tr = 0.0, fro = 0.0;
+  const double* p = part + (size_t)m * part_stride;
+  for (int t = 0; t < ntri; ++t) {
+    int ti, tj;
+    tri_decode(t, ti, tj);
+    if (tj * BT >= n) continue;
+    tr += p[2 * t];
+    fro += p[2 * t + 1];
+  }
+  const double N = nocc[m];
+  s[S_TR] = tr;
+  s[S_FRO] = fro;
+  s[S_E2] = s[S_E1];
+  s[S_E1] = s[S_E0];
+  s[S_E0] = fabs(tr - N);
+  s[S_IT] += 1.0;
+  s[S_BRANCH] = fabs(fro - N) < fabs(2.0 * tr - fro - N) ? 1.0 : 0.0;
+  const bool stop = (s[S_E0] < eps && s[S_E0] >= s[S_E2] && s[S_IT] >= 3.0) || s[S_IT] >= max_iter;
+  if (stop) {
+    s[S_DONE] = 1.0;
+    atomicAdd(ndone, 1);
+  }
+}
+
+// ---------------------------------------------------------------- FP64 GEMM
+// C = A * B for symmetric B (B is read by rows, B[k][j] = B[j][k]); A read by rows.  Per active
+// molecule, square n x n, row-major with leading dimension n (packed layout) or ld (SP2 layout).
+// Tile DT x DT per 256-thread block (4 waves as 2x2), v_mfma_f64_16x16x4_f64, K slab 16 staged
+// through registers (prefetched during the MFMAs of the previous slab) and LDS.
+// mode bit 0: McWeeny epilogue C <- 2 (3 A - 2 A B) (A = X^2, B = X); bit 1: the product is known
+// to be symmetric (X^2, X^2 X): only tiles ti <= tj are computed and mirrored.
+template <int DT>
+__global__ __launch_bounds__(256) void k_dgemm_sym(const int* __restrict__ idx, const int* __restrict__ norb,
+                                                   const long long* __restrict__ off, const int* __restrict__ lds,
+                                                   const double* __restrict__ A, const double* __restrict__ B,
+                                                   double* __restrict__ C, int mode, int ntile_side) {
+  constexpr int WT = DT / 2;    // per-wave tile edge
+  constexpr int NB = WT / 16;   // 16x16 MFMA blocks per wave side
+  constexpr int NQ = DT * 16 / 256;
+  __shared__ double As[DT][17], Bs[DT][17];
+  const int m = idx[blockIdx.y];
+  const int n = norb[m];
+  int ti, tj;
+  const bool symout = (mode & 2) != 0;
+  if (symout) {
+    tri_decode(blockIdx.x, ti, tj);
+  } else {
+    ti = blockIdx.x / ntile_side;
+    tj = blockIdx.x % ntile_side;
+  }
+  if (ti * DT >= n || tj * DT >= n) return;
+  const int ld = lds ? lds[m] : n;
+  const size_t o = off[m];
+  const double* Am = A + o;
+  const double* Bm = B + o;
+  double* Cm = C + o;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = wave >> 1, wc = wave & 1;
+  const int i0 = ti * DT, j0 = tj * DT;
+  doublex4 acc[NB][NB];
+#pragma unroll
+  for (int a = 0; a < NB; ++a)
+#pragma unroll
+    for (int b = 0; b < NB; ++b) acc[a][b] = (doublex4){0.0, 0.0, 0.0, 0.0};
+  double ra[NQ], rb[NQ];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int e = tid + 256 * q;
+      const int row = e >> 4, gk = k0 + (e & 15);
+      ra[q] = (i0 + row < n && gk < n) ? Am[(size_t)(i0 + row) * ld + gk] : 0.0;
+      rb[q] = (j0 + row < n && gk < n) ? Bm[(size_t)(j0 + row) * ld + gk] : 0.0;
+    }
+  };
+  gload(0);
+  for (int k0 = 0; k0 < n; k0 += 16) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int e = tid + 256 * q;
+      As[e >> 4][e & 15] = ra[q];
+      Bs[e >> 4][e & 15] = rb[q];
+    }
+    __syncthreads();
+    if (k0 + 16 < n) gload(k0 + 16);
+#pragma unroll
+    for (int kk = 0; kk < 16; kk += 4) {
+      double av[NB], bv[NB];
+#pragma unroll
+      for (int a = 0; a < NB; ++a) {
+        av[a] = As[wr * WT + a * 16 + (lane & 15)][kk + (lane >> 4)];
+        bv[a] = Bs[wc * WT + a * 16 + (lane & 15)][kk + (lane >> 4)];
+      }
+#pragma unroll
+      for (int a = 0; a < NB; ++a)
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+          acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int a = 0; a < NB; ++a)
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = i0 + wr * WT + a * 16 + 4 * v + (lane >> 4);
+        const int c = j0 + wc * WT + b * 16 + (lane & 15);
+        if (r < n && c < n) {
+          const double x = acc[a][b][v];
+          const double y = (mode & 1) ? 2.0 * (3.0 * Am[(size_t)r * ld + c] - 2.0 * x) : x;
+          Cm[(size_t)r * ld + c] = y;
+          if (symout && ti != tj) Cm[(size_t)c * ld + r] = y;
+        }
+      }
+}
+
+// final FP32 X of each molecule (in buffer Xa or Xb by the parity of its iteration count) ->
+// FP64 padded copy
+__global__ void k_x_to_f64(const int* __restrict__ idx, const long long* __restrict__ xoff,
+                           const int* __restrict__ xld, const double* __restrict__ st, const float* __restrict__ Xa,
+                           const float* __restrict__ Xb, double* __restrict__ Y) {
+  const int m = idx[blockIdx.y];
+  const size_t n2 = (size_t)xld[m] * xld[m];
+  const size_t o = xoff[m];
+  const float* X = (((long long)st[(size_t)m * S_NST + S_IT]) & 1) ? Xb : Xa;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n2; e += (size_t)gridDim.x * blockDim.x)
+    Y[o + e] = X[o + e];
+}
+
+// padded (ld) FP64 block -> packed (n) FP64 block
+__global__ void k_unpad(const int* __restrict__ idx, const int* __restrict__ norb, const long long* __restrict__ xoff,
+                        const int* __restrict__ xld, const long long* __restrict__ poff, const double* __restrict__ Y,
+                        double* __restrict__ P) {
+  const int m = idx[blockIdx.y];
+  const int n = norb[m], ld = xld[m];
+  const size_t n2 = (size_t)n * n;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n2; e += (size_t)gridDim.x * blockDim.x) {
+    const size_t r = e / n, c = e % n;
+    P[poff[m] + e] = Y[xoff[m] + r * ld + c];
+  }
+}
+
+inline int st_ok() { return hipGetLastError() == hipSuccess ? PYSEQM_OK : PYSEQM_ELAUNCH; }
+inline int eblocks(long long n2) {
+  long long b = (n2 + 255) / 256;
+  return (int)(b < 1 ? 1 : (b > 2048 ? 2048 : b));
+}
+
+}  // namespace
+
+extern "C" int pyseqm_sp2_init(const pyseqm_layout_t* L, const double* F, const int* idx, int nact,
+                               const long long* xoff, const int* xld, float* X, double* state, void* stream) {
+  if (!L || nact < 0) return PYSEQM_EINVAL;
+  if (nact == 0) return PYSEQM_OK;
+  k_sp2_init<<<nact, 256, 0, (hipStream_t)stream>>>(*L, F, idx, xoff, xld, X, state);
+  return st_ok();
+}
+
+extern "C" int pyseqm_sp2_iterate(const pyseqm_layout_t* L, const int* idx, int nact, int nmax,
+                                  const long long* xoff, const int* xld, const float* Xin, float* Xout,
+                                  double* state, double* part, int part_stride, double eps, int max_iter,
+                                  int* ndone, void* stream) {
+  if (!L || nact < 0 || nmax <= 0) return PYSEQM_EINVAL;
+  if (nact == 0) return PYSEQM_OK;
+  const int nt = (nmax + BT - 1) / BT;
+  if (part_stride < nt * (nt + 1)) return PYSEQM_EINVAL;
+  hipStream_t s = (hipStream_t)stream;
+  k_sp2_bf16x3<<<dim3(nt * (nt + 1) / 2, nact), 256, 0, s>>>(idx, L->mol_norb, xoff, xld, Xin, Xout, state, part,
+                                                               part_stride);
+  k_sp2_decide<<<(nact + 63) / 64, 64, 0, s>>>(idx, nact, L->mol_norb, L->mol_nocc, state, part, part_stride, eps,
+                                               max_iter, ndone);
+  return st_ok();
+}
+
+extern "C" int pyseqm_dgemm_sym(const pyseqm_layout_t* L, const int* idx, int nact, int nmax,
+                                const long long* off, const int* lds, const double* A, const double* B, double* C,
+                                int mode, void* stream) {
+  if (!L || nact < 0 || nmax <= 0 || mode < 0 || mode > 3) return PYSEQM_EINVAL;
+  if (nact == 0) return PYSEQM_OK;
+  hipStream_t s = (hipStream_t)stream;
+  const bool symout = (mode & 2) != 0;
+  if (nmax <= 256) {
+    const int nt = (nmax + 63) / 64;
+    k_dgemm_sym<64><<<dim3(symout ? nt * (nt + 1) / 2 : nt * nt, nact), 256, 0, s>>>(idx, L->mol_norb, off, lds, A,
+                                                                                      B, C, mode, nt);
+  } else {
+    const int nt = (nmax + 127) / 128;
+    k_dgemm_sym<128><<<dim3(symout ? nt * (nt + 1) / 2 : nt * nt, nact), 256, 0, s>>>(idx, L->mol_norb, off, lds,
+                                                                                       A, B, C, mode, nt);
+  }
+  return st_ok();
+}
+
+extern "C" int pyseqm_sp2_x_to_f64(const pyseqm_layout_t* L, const int* idx, int nact, int nmax,
+                                   const long long* xoff, const int* xld, const double* state, const float* Xa,
+                                   const float* Xb, double* Y, void* stream) {
+  if (!L || nact < 0) return PYSEQM_EINVAL;
+  if (nact == 0) return PYSEQM_OK;
+  const int ld = (nmax + 31) / 32 * 32;
+  k_x_to_f64<<<dim3(eblocks((long long)ld * ld), nact), 256, 0, (hipStream_t)stream>>>(idx, xoff, xld, state, Xa,
+                                                                                       Xb, Y);
+  return st_ok();
+}
+
+extern "C" int pyseqm_sp2_unpad(const pyseqm_layout_t* L, const int* idx, int nact, int nmax,
+                                const long long* xoff, const int* xld, const double* Y, double* P, void* stream) {
+  if (!L || nact < 0) return PYSEQM_EINVAL;
+  if (nact == 0) return PYSEQM_OK;
+  k_unpad<<<dim3(eblocks((long long)nmax * nmax), nact), 256, 0, (hipStream_t)stream>>>(
+      idx, L->mol_norb, xoff, xld, L->mol_mat_off, Y, P);
+  return st_ok();
+}

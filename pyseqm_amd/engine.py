@@ -16,6 +16,7 @@ from .seqm_functions.parameters import COLUMNS, param_table
 
 MAX_ITER = 1000
 NFOCK = 10
+SP2_DIIS_TOL = 1.0e-2
 
 
 class SCFResult:
@@ -59,6 +60,7 @@ class SCFEngine:
         self.eheat = torch.as_tensor(np.bincount(b.atom_mol, np.take(EHEAT, Z) / ev_kcalpmol, minlength=b.nmol),
                                      **f64)
         self.coords = None
+        self.trace = None  # set to a list to collect per-iteration (dE, diis, dm, dm_max) metrics
         if coordinates is not None:
             self.set_coordinates(coordinates)
 
@@ -133,8 +135,12 @@ class SCFEngine:
         dev = self.device
         self.density_kind = density
         self.sp2_eps = sp2_eps
+        # max|FP - PF| test: 50 eps as the reference (scf_loop.py:52); the bf16x3 SP2 density has a
+        # commutator floor of ~1e-3 (its energy error stays ~1e-8 eV/atom), so it is tested at 1e-2
+        self.diis_tol = 50.0 * eps if density == "eigh" else max(50.0 * eps, SP2_DIIS_TOL)
         f64 = dict(dtype=torch.float64, device=dev)
         P = self.initial_density() if P is None else P.clone()
+        self._metrics = torch.zeros(b.nmol, 4, **f64)
         F = torch.empty_like(P)
         Pnew = torch.zeros_like(P)
         Pold = torch.zeros_like(P)
@@ -181,7 +187,7 @@ class SCFEngine:
             act = self._act_mask(nc_host)
             self.fock(P, F, active=act, eelec=e_new)
             nat.call("pyseqm_scf_check", b.L, nat.ptr(idx), nact, nat.ptr(P), nat.ptr(Pold), nat.ptr(e_new),
-                     nat.ptr(e_old), None, float(eps), nat.ptr(notconv), None, self.stream)
+                     nat.ptr(e_old), None, float(eps), 0.0, nat.ptr(notconv), None, self.stream)
             nc_host = notconv.cpu().numpy().astype(bool)
         return P, it, nc_host
 
@@ -245,8 +251,12 @@ class SCFEngine:
             act = self._act_mask(nc_host)
             self.fock(P, F, active=act, eelec=e_new)
             nat.call("pyseqm_scf_check", b.L, nat.ptr(idx), nact, nat.ptr(P), nat.ptr(Pold), nat.ptr(e_new),
-                     nat.ptr(e_old), nat.ptr(diis_err), float(eps), nat.ptr(notconv), None, self.stream)
+                     nat.ptr(e_old), nat.ptr(diis_err), float(eps), self.diis_tol, nat.ptr(notconv),
+                     nat.ptr(self._metrics),
+                     self.stream)
             nc_host = notconv.cpu().numpy().astype(bool)
+            if self.trace is not None:
+                self.trace.append(self._metrics.cpu().numpy().copy())
             if reset:
                 counter, cF = -1, 0
                 EMAT.copy_(emat0.expand(b.nmol, -1, -1))

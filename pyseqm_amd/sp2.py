@@ -1,0 +1,86 @@
+"""Mixed-precision SP2 density step on MFMA (host driver of the sp2.hip kernels).
+
+Replaces pack -> SP2 -> unpack of make_Pnew_factory (seqm/seqm_functions/scf_loop.py:66-78,
+SP2.py:9-85): trace-correcting SP2 with X in FP32 and X^2 on bf16x3 MFMA (FP32 accumulate),
+branch / stop decided on the device, then one FP64 McWeeny step P = 2 (3 X^2 - 2 X^3) on the
+FP64 MFMA GEMM.  The stop rule is the reference's reduced-precision one (trace error below eps and
+no longer improving).
+"""
+import numpy as np
+import torch
+
+from . import _native as nat
+
+NSTATE = 8
+SP2_EPS = 1.0e-2      # SP2.py:4 (reduced-precision threshold)
+SP2_MAX_ITER = 100
+POLL = 4              # iterations between host checks of the stop counter
+
+
+class SP2Workspace:
+    def __init__(self, eng):
+        b = eng.b
+        dev = eng.device
+        ld = ((b.norb + 31) // 32 * 32).astype(np.int64)
+        off = np.concatenate([[0], np.cumsum(ld * ld)]).astype(np.int64)
+        self.xtotal = int(off[-1])
+        self.xoff = torch.as_tensor(off[:-1], device=dev)
+        self.xld = torch.as_tensor(ld.astype(np.int32), device=dev)
+        self.Xa = torch.zeros(max(self.xtotal, 1), dtype=torch.float32, device=dev)
+        self.Xb = torch.zeros_like(self.Xa)
+        self.Y = torch.zeros(max(self.xtotal, 1), dtype=torch.float64, device=dev)
+        self.Y2 = torch.zeros_like(self.Y)
+        self.Y3 = torch.zeros_like(self.Y)
+        self.state = torch.zeros(b.nmol, NSTATE, dtype=torch.float64, device=dev)
+        nt = (b.nmax + 127) // 128
+        self.part_stride = max(nt * (nt + 1), 2)
+        self.part = torch.zeros(b.nmol, self.part_stride, dtype=torch.float64, device=dev)
+        self.ndone = torch.zeros(1, dtype=torch.int32, device=dev)
+        self.last_iters = None
+        self.timing = None  # list of (start event, end event, idx, nact) of the SP2 GEMM loops
+
+
+def density_sp2(eng, F, idx, nact, Pout, eps=SP2_EPS, max_iter=SP2_MAX_ITER):
+    b = eng.b
+    ws = getattr(eng, "_sp2ws", None)
+    if ws is None:
+        ws = eng._sp2ws = SP2Workspace(eng)
+    nmax = int(b.norb[eng._idx_host].max())
+    s = eng.stream
+    L = b.L
+    nat.call("pyseqm_sp2_init", L, nat.ptr(F), nat.ptr(idx), nact, nat.ptr(ws.xoff), nat.ptr(ws.xld),
+             nat.ptr(ws.Xa), nat.ptr(ws.state), s)
+    ws.ndone.zero_()
+    bufs = (ws.Xa, ws.Xb)
+    if ws.timing is not None:
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        ev0.record()
+    for it in range(max_iter):
+        nat.call("pyseqm_sp2_iterate", L, nat.ptr(idx), nact, nmax, nat.ptr(ws.xoff), nat.ptr(ws.xld),
+                 nat.ptr(bufs[it % 2]), nat.ptr(bufs[(it + 1) % 2]), nat.ptr(ws.state), nat.ptr(ws.part),
+                 ws.part_stride, float(eps), int(max_iter), nat.ptr(ws.ndone), s)
+        if (it + 1) % POLL == 0 and int(ws.ndone.item()) >= nact:
+            break
+    if ws.timing is not None:
+        ev1.record()
+        ws.timing.append((ev0, ev1, idx.clone(), nact))
+    # FP64 McWeeny correction: P = 2 (3 X^2 - 2 X^3)
+    nat.call("pyseqm_sp2_x_to_f64", L, nat.ptr(idx), nact, nmax, nat.ptr(ws.xoff), nat.ptr(ws.xld),
+             nat.ptr(ws.state), nat.ptr(ws.Xa), nat.ptr(ws.Xb), nat.ptr(ws.Y), s)
+    nat.call("pyseqm_dgemm_sym", L, nat.ptr(idx), nact, nmax, nat.ptr(ws.xoff), nat.ptr(ws.xld), nat.ptr(ws.Y),
+             nat.ptr(ws.Y), nat.ptr(ws.Y2), 2, s)
+    nat.call("pyseqm_dgemm_sym", L, nat.ptr(idx), nact, nmax, nat.ptr(ws.xoff), nat.ptr(ws.xld), nat.ptr(ws.Y2),
+             nat.ptr(ws.Y), nat.ptr(ws.Y3), 3, s)
+    nat.call("pyseqm_sp2_unpad", L, nat.ptr(idx), nact, nmax, nat.ptr(ws.xoff), nat.ptr(ws.xld), nat.ptr(ws.Y3),
+             nat.ptr(Pout), s)
+    ws.last_iters = ws.state[:, 7]
+    if ws.timing is not None:
+        ws.timing[-1] = ws.timing[-1] + (ws.state[:, 7].clone(),)
+
+
+def gemm_flops(eng, idx, iters):
+    """Algorithmic FLOPs of the SP2 X^2 products: 2 n^3 per iteration per molecule."""
+    n = torch.as_tensor(eng.b.norb, device=iters.device, dtype=torch.float64)
+    il = idx.long()
+    return float((2.0 * n[il] ** 3 * iters[il]).sum())
