@@ -29,6 +29,7 @@ extern "C" {
 #define PYSEQM_EUNSUPPORTED (-3) /* element pair or method outside MNDO/AM1/PM3 sp scope */
 
 #define PYSEQM_NPAR 32 /* doubles per atom in the atom-parameter array (see below) */
+#define PYSEQM_MAX_RED_BLOCKS 256 /* max blocks per molecule of the two-stage reductions */
 
 /* Atom-parameter record, PYSEQM_NPAR doubles per atom.  Slots 0..23 are inputs (MOPAC
  * parameters gathered by Z, seqm/basics.py:406-448); 24..31 are filled by
@@ -84,6 +85,20 @@ int pyseqm_pair_integrals(const pyseqm_layout_t* L, int method, const double* co
                           double* H, double* part, double* enuc_tile, double* enuc_mol,
                           double* enuc_pair, const int* pair_ref, void* stream);
 
+/* Forces -dE/dR (eV/Angstrom, [natoms*3]) at fixed density P: per atom pair, a 4th-order central
+ * difference (step h Angstrom) of the pair's Hellmann-Feynman energy 2 P_ij.H_ij + P_ii.e1b +
+ * P_jj.e2a + P_ii P_jj (ii|jj) - 1/2 P_ij P_ij (ij|ij) + E_nuc(ij) -- the energy the reference
+ * differentiates by autograd with scf_backward = 0 (basics.py:1315-1336, scf_loop.py:1996-2031).
+ * part: tile partial scratch as in pyseqm_pair_integrals. */
+int pyseqm_pair_forces(const pyseqm_layout_t* L, int method, const double* coords, const double* atom_par,
+                       const double* P, double h, double* part, double* forces, void* stream);
+/* XL-BOMD forces: -d/dR of E(D, P) = sum D F(P) - 1/2 sum (F(P) - H) P + E_nuc at fixed D and P
+ * (the energy of EnergyXL, energy.py:76-88, differentiated by ForceXL's backward,
+ * seqm/dynamics/xlbomd.py:536-551).  D: density from F(P); P: propagated density field. */
+int pyseqm_pair_forces_xl(const pyseqm_layout_t* L, int method, const double* coords, const double* atom_par,
+                          const double* D, const double* P, double h, double* part, double* forces,
+                          void* stream);
+
 /* Fock build F = H + G(P) with the electronic energy 0.5*sum P*(H+F) per molecule fused in.
  * Replaces fock() (seqm/seqm_functions/fock.py:132-347) and elec_energy (energy.py:26-49).
  *   active [nmol] (may be NULL = all): molecules with active[m] == 0 are skipped.
@@ -93,6 +108,95 @@ int pyseqm_fock(const pyseqm_layout_t* L, const double* atom_par, const double* 
                 const double* w_xh, const double* w_hh, const double* P, const double* H,
                 double* F, double* part, double* etile, double* eatom, double* eelec,
                 const unsigned char* active, void* stream);
+
+/* ------------------------------------------------------------------ SCF loop plumbing
+ * The functions below work on a list of active molecules idx[0..nact) (device int array): the
+ * reference's `notconverged` masks (scf_loop.py:974-1131).  `red` is reduction scratch of at
+ * least nmol * PYSEQM_MAX_RED_BLOCKS * 16 doubles; nb (1..PYSEQM_MAX_RED_BLOCKS) is the number of
+ * blocks per molecule of the two-stage reductions. */
+
+/* Density step, FP64 diagonalisation path (diag.py:110-241 sym_eig_trunc): copy each active
+ * packed Fock matrix into a zero-padded nmax x nmax block whose padded diagonal lies above the
+ * Gershgorin range (diag.py:172-192), for a batched symmetric eigensolver. */
+int pyseqm_pad_for_eigh(const pyseqm_layout_t* L, const double* F, const int* idx, int nact, int nmax,
+                        double* out, double* red, int nb, void* stream);
+/* P = 2 C_occ C_occ^T from the padded eigenvectors (columns) (diag.py:230-232). */
+int pyseqm_density_from_evecs(const pyseqm_layout_t* L, const double* V, int nmax, const int* idx,
+                              int nact, double* P, void* stream);
+
+/* Density mixing: Pold = P; P = alpha P + (1 - alpha) Pnew (scf_loop.py:266-267, 1052-1056). */
+int pyseqm_mix(const pyseqm_layout_t* L, const int* idx, int nact, int nmax, double alpha, double* P,
+               const double* Pnew, double* Pold, void* stream);
+int pyseqm_copy_active(const pyseqm_layout_t* L, const int* idx, int nact, int nmax, const double* src,
+                       double* dst, void* stream);
+
+/* Convergence test get_error (scf_loop.py:106-147): |dE| <= eps, max|FP-PF| <= diis_tol,
+ * ||dP||_F/(4 natoms) <= 2 eps, max|dP| <= 15 eps.  Writes notconv[m] (1 = not converged),
+ * moves e_new into e_old for unconverged molecules, optional metrics[4*m..] = (dE, diis, dm, dm_max). */
+int pyseqm_scf_check(const pyseqm_layout_t* L, const int* idx, int nact, const double* P,
+                     const double* Pold, const double* e_new, double* e_old, const double* diis_err,
+                     double eps, double diis_tol, unsigned char* notconv, double* metrics, double* red,
+                     int nb, void* stream);
+
+/* Ragged FP64 GEMM C = A B per active molecule (LDS-tiled FMA kernel, small matrices). */
+int pyseqm_gemm(const pyseqm_layout_t* L, const int* idx, int nact, int nmax, const double* A,
+                const double* B, double* C, void* stream);
+/* FP64 MFMA GEMM C = A B with B symmetric.  off/lds: per-molecule offset and leading dimension
+ * (lds NULL = packed, ld = n).  mode bit 0: McWeeny epilogue C = 2(3A - 2AB); bit 1: symmetric
+ * product, only upper tiles are computed and mirrored. */
+int pyseqm_dgemm_sym(const pyseqm_layout_t* L, const int* idx, int nact, int nmax, const long long* off,
+                     const int* lds, const double* A, const double* B, double* C, int mode, void* stream);
+
+/* Pulay DIIS (scf_loop.py:997-1035): residual C = FP - PF into a history slot and max|C|;
+ * B-matrix row B[m][slot][j] = <C_slot, C_j> (packed upper triangle); F = sum_j c[m][j] F_j. */
+int pyseqm_diis_residual(const pyseqm_layout_t* L, const int* idx, int nact, int nmax, const double* F,
+                         const double* P, double* X, double* Cout, double* diis_err, double* red, int nb,
+                         void* stream);
+int pyseqm_diis_dots(const pyseqm_layout_t* L, const int* idx, int nact, const double* ring,
+                     long long ring_stride, int slot, int count, int nfock, double* Bmat, double* red, int nb,
+                     void* stream);
+int pyseqm_diis_combine(const pyseqm_layout_t* L, const int* idx, int nact, int nmax, const double* ring,
+                        long long ring_stride, int count, const double* coef, int cstride, double* F,
+                        void* stream);
+
+/* ------------------------------------------------------------------ SP2 density step
+ * Trace-correcting SP2 (SP2.py:9-85) in mixed precision: X (FP32, per-molecule zero-padded
+ * blocks at xoff[m] with leading dimension xld[m], a multiple of 32), X^2 on bf16x3 MFMA with FP32
+ * accumulation, per-molecule state[8*m..] = (tr X, tr X^2, err0, err1, err2, branch, done, iters)
+ * on the device. */
+int pyseqm_sp2_init(const pyseqm_layout_t* L, const double* F, const int* idx, int nact,
+                    const long long* xoff, const int* xld, float* X, double* state, double* red, int nb,
+                    void* stream);
+/* One SP2 iteration for every active, not yet stopped molecule: Xout = Xin^2 or 2 Xin - Xin^2,
+ * then the device-side branch / stop decision; *ndone counts molecules that stopped. */
+int pyseqm_sp2_iterate(const pyseqm_layout_t* L, const int* idx, int nact, int nmax, const long long* xoff,
+                       const int* xld, const float* Xin, float* Xout, double* state, double* part,
+                       int part_stride, double eps, int max_iter, int* ndone, void* stream);
+/* Final X (buffer chosen by each molecule's iteration parity) -> FP64 padded copy. */
+int pyseqm_sp2_x_to_f64(const pyseqm_layout_t* L, const int* idx, int nact, int nmax, const long long* xoff,
+                        const int* xld, const double* state, const float* Xa, const float* Xb, double* Y,
+                        void* stream);
+/* FP64 padded block -> packed density. */
+int pyseqm_sp2_unpad(const pyseqm_layout_t* L, const int* idx, int nact, int nmax, const long long* xoff,
+                     const int* xld, const double* Y, double* P, void* stream);
+
+/* ------------------------------------------------------------------ XL-BOMD / MD
+ * Density propagation of XL_BOMD._propagate_P (seqm/MolecularDynamics.py:1418-1427) plus the
+ * ring update Pt[m-1-cindx] = P (1461) over `total` packed elements:
+ *   P = coef_D (c D + (1 - c) P) + sum_{j<m} coef[j] Pt[j*pt_stride + .];  Pt[slot] = P.
+ * coef: HOST array of m <= 11 doubles (the reference's coeff[cindx : cindx + m]). */
+int pyseqm_xl_propagate(long long total, const double* D, double* P, double* Pt, long long pt_stride, int m,
+                        const double* coef, double coef_D, double c, int slot, void* stream);
+/* XL electronic energy per molecule, elec_energy_xl (energy.py:76-88):
+ * eelec[m] = sum D F - 1/2 sum (F - H) P over the molecule's packed block (H full symmetric).
+ * red: [nmol*nb] scratch, 1 <= nb <= PYSEQM_MAX_RED_BLOCKS. */
+int pyseqm_xl_energy(const pyseqm_layout_t* L, const double* D, const double* P, const double* F,
+                     const double* H, double* eelec, double* red, int nb, void* stream);
+/* Velocity-Verlet half steps on packed (natoms, 3) arrays (MolecularDynamics.py:1449-1450,
+ * 1503-1505): phase 0: v += dt/2 a, x += dt v;  phase 1: a = f minv acc_scale, v += dt/2 a.
+ * minv [natoms]: inverse masses. */
+int pyseqm_md_verlet(int natoms, int phase, double dt, double acc_scale, double* x, double* v, double* a,
+                     const double* f, const double* minv, void* stream);
 
 #ifdef __cplusplus
 }

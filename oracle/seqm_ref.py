@@ -801,6 +801,12 @@ def elec_energy(P, F, H):
     return 0.5 * np.einsum("nij,nij->n", P, H + F)
 
 
+def elec_energy_xl(D, P, F, H):
+    """XL-BOMD electronic energy sum D*F(P) - 0.5 sum (F(P) - H)*P, seqm/seqm_functions/energy.py:76-88
+    (H symmetric here)."""
+    return np.einsum("nij,nij->n", D, F) - 0.5 * np.einsum("nij,nij->n", F - H, P)
+
+
 def pair_nuclear_energy(b, gam):
     """Core-core repulsion per pair, seqm/seqm_functions/energy.py:91-134 (MNDO/AM1/PM3)."""
     prm = b.prm
@@ -1066,8 +1072,21 @@ def energy_fixed_density(species, coords, method, P):
     return elec_energy(P, F, H) + Enuc
 
 
-def forces_fd(species, coords, method, P, h=1.0e-4):
-    """Force = -dE/dR (eV/Angstrom) by a 4th-order central difference of energy_fixed_density."""
+def energy_xl_fixed(species, coords, method, D, P):
+    """E(R) = sum D F(P) - 0.5 sum (F(P) - H) P + Enuc at fixed D and P: the XL-BOMD energy of
+    EnergyXL (seqm/dynamics/xlbomd.py:147-199, 430-447) whose gradient ForceXL's backward returns
+    (xlbomd.py:536-551; D and P carry no gradient)."""
+    b = parse(species, coords, method)
+    H, T, _, gam, _ = hcore(b)
+    F = fock(b, P, H, T)
+    Enuc = np.zeros(b.nmol)
+    np.add.at(Enuc, b.pair_mol, pair_nuclear_energy(b, gam))
+    return elec_energy_xl(D, P, F, H) + Enuc
+
+
+def forces_fd(species, coords, method, P, h=1.0e-4, D=None):
+    """Force = -dE/dR (eV/Angstrom) by a 4th-order central difference of energy_fixed_density
+    (D None) or of energy_xl_fixed (D given)."""
     coords = np.asarray(coords, dtype=np.float64)
     species = np.asarray(species)
     f = np.zeros_like(coords)
@@ -1080,8 +1099,46 @@ def forces_fd(species, coords, method, P, h=1.0e-4):
                 for s in (-2, -1, 1, 2):
                     c = coords.copy()
                     c[mo, a, x] += s * h
-                    vals.append(energy_fixed_density(species[mo:mo + 1], c[mo:mo + 1], method,
-                                                     P[mo:mo + 1])[0])
+                    if D is None:
+                        e = energy_fixed_density(species[mo:mo + 1], c[mo:mo + 1], method, P[mo:mo + 1])
+                    else:
+                        e = energy_xl_fixed(species[mo:mo + 1], c[mo:mo + 1], method, D[mo:mo + 1],
+                                            P[mo:mo + 1])
+                    vals.append(e[0])
                 g = (vals[0] - 8.0 * vals[1] + 8.0 * vals[2] - vals[3]) / (12.0 * h)
                 f[mo, a, x] = -g
     return f
+
+
+# --------------------------------------------------------------- XL-BOMD
+# Niklasson et al. JCP 130, 214109 (2009) coefficients kappa, alpha, c_0..c_k
+# (seqm/MolecularDynamics.py:1382-1390)
+XL_COEFFS = {
+    3: [1.69, 150e-3, -2.0, 3.0, 0.0, -1.0],
+    4: [1.75, 57e-3, -3.0, 6.0, -2.0, -2.0, 1.0],
+    5: [1.82, 18e-3, -6.0, 14.0, -8.0, -3.0, 4.0, -1.0],
+    6: [1.84, 5.5e-3, -14.0, 36.0, -27.0, -2.0, 12.0, -6.0, 1.0],
+    7: [1.86, 1.6e-3, -36.0, 99.0, -88.0, 11.0, 32.0, -25.0, 8.0, -1.0],
+    8: [1.88, 0.44e-3, -99.0, 286.0, -286.0, 78.0, 78.0, -90.0, 42.0, -10.0, 1.0],
+    9: [1.89, 0.12e-3, -286.0, 858.0, -936.0, 364.0, 168.0, -300.0, 184.0, -63.0, 12.0, -1.0],
+}
+
+
+def xl_propagate(D, P, Pt, k, step, c=0.95):
+    """One density propagation of XL_BOMD (seqm/MolecularDynamics.py:1392-1427, 1458-1461):
+    P(n+1) = 2 P(n) - P(n-1) + kappa (D' - P(n)) + alpha sum_i c_i P(n-i), D' = c D + (1-c) P(n),
+    written over the ring Pt (m = k+1 slots; slot (m-1-cindx') holds the density of step
+    n - ((cindx - cindx') mod m)).  Returns (P_new, Pt) with Pt updated in place."""
+    kappa, alpha = XL_COEFFS[k][0], XL_COEFFS[k][1]
+    cs = np.asarray(XL_COEFFS[k][2:]) * alpha
+    m = k + 1
+    cindx = step % m
+    # P(n-i) sits in slot (m - cindx + i) mod m (the previous step wrote P(n) to slot m - cindx);
+    # at the start every slot holds the seed density.
+    hist = [Pt[(m - 1 - ((cindx - 1 - i) % m))] for i in range(m)]
+    Dp = c * D + (1.0 - c) * P
+    Pn = 2.0 * hist[0] - hist[1] + kappa * (Dp - hist[0])
+    for i in range(m):
+        Pn = Pn + cs[i] * hist[i]
+    Pt[m - 1 - cindx] = Pn
+    return Pn, Pt

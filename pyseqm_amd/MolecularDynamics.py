@@ -1,0 +1,193 @@
+"""Molecular dynamics drivers of the drop-in API (seqm/MolecularDynamics.py).
+
+``Molecular_Dynamics_Basic`` (velocity Verlet on converged SCF forces, 670-1092) and ``XL_BOMD``
+(extended-Lagrangian Born-Oppenheimer MD, 1373-1605) keep the reference's constructor and
+``run(molecule, steps, ..., seed)`` signatures.  The XL-BOMD trajectory batch lives on the device
+(``XLTrajectories``): one step is a fixed sequence of kernel launches with no host round trip.
+Thermodynamic records (T, Ek, Ep per step) are kept in memory in ``self.history``; the HDF5/xyz
+writers and checkpoints of the reference are not part of this path.
+"""
+import numpy as np
+import torch
+
+from .ElectronicStructure import Electronic_Structure
+from .xlbomd import (KINETIC_ENERGY_SCALE, TEMPERATURE_SCALE, VEL_SCALE, ACC_SCALE, XLTrajectories)
+
+
+class Molecular_Dynamics_Basic(torch.nn.Module):
+    def __init__(self, seqm_parameters, timestep=1.0, Temp=0.0, step_offset=0, output=None, *args, **kwargs):
+        super().__init__()
+        self.seqm_parameters = seqm_parameters
+        self.timestep = float(timestep)
+        self.Temp = float(Temp)
+        self.step_offset = int(step_offset)
+        self.output = output
+        self.esdriver = Electronic_Structure(seqm_parameters)
+        self.n_dof = None
+        self.history = None
+
+    # -------------------------------------------------------- thermodynamics
+    def _kinetic_energy(self, molecule):
+        return torch.sum(0.5 * molecule.mass * molecule.velocities ** 2, dim=(1, 2)) * KINETIC_ENERGY_SCALE
+
+    def _calc_temperature(self, Ek):
+        return Ek * TEMPERATURE_SCALE / (0.5 * self.n_dof)
+
+    def set_dof(self, molecule, constraints=0.0):
+        self.n_dof = 3.0 * molecule.num_atoms - constraints
+
+    def _thermo_potential(self, molecule):
+        return molecule.Etot
+
+    def initialize_velocity(self, molecule, vel_com=True):
+        """Maxwell-Boltzmann velocities rescaled to exactly Temp (MolecularDynamics.py:717-744).
+        The normal deviates are drawn on the CPU generator (seeded by ``run(seed=)``) in the padded
+        layout, so a given seed gives the same velocities on every device."""
+        if torch.is_tensor(molecule.velocities):
+            if vel_com:
+                self._zero_com(molecule, translate_to_origin=False, restore_kinetic_energy=False)
+            return molecule.velocities
+        if self.Temp == 0.0:
+            molecule.velocities = torch.zeros_like(molecule.coordinates)
+            return molecule.velocities
+        scale = torch.sqrt(self.Temp * molecule.mass_inverse) * VEL_SCALE
+        g = torch.randn(tuple(molecule.coordinates.shape), dtype=molecule.coordinates.dtype)
+        molecule.velocities = g.to(molecule.coordinates.device) * scale
+        Ek = self._kinetic_energy(molecule)
+        alpha = torch.sqrt(self.Temp / self._calc_temperature(Ek))
+        molecule.velocities.mul_(alpha.reshape(-1, 1, 1))
+        if vel_com:
+            self._zero_com(molecule, translate_to_origin=True)
+        return molecule.velocities
+
+    def _zero_com(self, molecule, remove_angular=True, translate_to_origin=False, restore_kinetic_energy=True):
+        """Remove centre-of-mass translation and rotation (MolecularDynamics.py:746-782)."""
+        mass = molecule.mass
+        M = mass.sum(dim=1, keepdim=True)
+        ek0 = self._kinetic_energy(molecule)
+        with torch.no_grad():
+            x = molecule.coordinates
+            r = x - (mass * x).sum(dim=1, keepdim=True) / M
+            if translate_to_origin:
+                molecule.coordinates.copy_(r)
+            v = molecule.velocities
+            v.sub_((mass * v).sum(dim=1, keepdim=True) / M)
+            if remove_angular:
+                Lmom = (mass * torch.linalg.cross(r, v, dim=2)).sum(dim=1)
+                rr = (r * r).sum(dim=2, keepdim=True)
+                eye = torch.eye(3, dtype=x.dtype, device=x.device)
+                inertia = (mass * rr).sum(dim=1, keepdim=True) * eye - (mass.unsqueeze(3) * r.unsqueeze(3)
+                                                                          * r.unsqueeze(2)).sum(dim=1)
+                omega = (torch.linalg.pinv(inertia, hermitian=True, atol=1e-10) @ Lmom.unsqueeze(2)).squeeze(-1)
+                v.sub_(torch.linalg.cross(omega.unsqueeze(1).expand_as(r), r, dim=2))
+            ek1 = self._kinetic_energy(molecule)
+            if torch.any(ek1 < 1e-12):
+                raise RuntimeError("Zero kinetic energy after removing COM momentum")
+            if restore_kinetic_energy:
+                v.mul_(torch.sqrt(ek0 / ek1).reshape(-1, 1, 1))
+
+    # ------------------------------------------------------------- stepping
+    def one_step(self, molecule, learned_parameters=dict(), *args, **kwargs):
+        """Velocity Verlet on SCF forces (MolecularDynamics.py:815-842)."""
+        dt = self.timestep
+        with torch.no_grad():
+            molecule.velocities.add_(0.5 * molecule.acc * dt)
+            molecule.coordinates.add_(molecule.velocities * dt)
+        self.esdriver(molecule, learned_parameters=learned_parameters, P0=molecule.dm, dm_prop="SCF")
+        with torch.no_grad():
+            molecule.acc = molecule.force * molecule.mass_inverse * ACC_SCALE
+            molecule.velocities.add_(0.5 * molecule.acc * dt)
+
+    def _do_integrator_step(self, i, molecule, learned_parameters, **kwargs):
+        return self.one_step(molecule, learned_parameters=learned_parameters, **kwargs)
+
+    def initialize(self, molecule, remove_com=None, learned_parameters=dict(), steps=None, *args, **kwargs):
+        if remove_com is not None:
+            raise NotImplementedError("periodic COM removal during the run is not part of this path")
+        self.set_dof(molecule, 0.0)
+        if self.step_offset == 0 or not torch.is_tensor(molecule.velocities):
+            self.initialize_velocity(molecule)
+        if not torch.is_tensor(molecule.force):
+            self.esdriver(molecule, learned_parameters=learned_parameters, P0=molecule.dm)
+        with torch.no_grad():
+            molecule.acc = molecule.force * molecule.mass_inverse * ACC_SCALE
+        self.history = {"T": [], "Ek": [], "Ep": []}
+        if self.step_offset == 0:
+            self._record(molecule)
+
+    def _record(self, molecule):
+        with torch.no_grad():
+            Ek = self._kinetic_energy(molecule)
+            self.history["Ek"].append(Ek.detach().cpu().numpy())
+            self.history["T"].append(self._calc_temperature(Ek).detach().cpu().numpy())
+            self.history["Ep"].append(self._thermo_potential(molecule).detach().cpu().numpy())
+
+    def run(self, molecule, steps, learned_parameters=dict(), reuse_P=True, remove_com=None, seed=None, *args,
+            **kwargs):
+        if seed is not None:
+            torch.manual_seed(int(seed))
+        self.initialize(molecule, remove_com=remove_com, learned_parameters=learned_parameters, steps=steps)
+        if not reuse_P:
+            molecule.dm = None
+        for i in range(self.step_offset, steps):
+            self._do_integrator_step(i, molecule, learned_parameters, **kwargs)
+            if not reuse_P:
+                molecule.dm = None
+            self._record(molecule)
+        return molecule.coordinates, molecule.velocities, molecule.acc
+
+    def thermo(self):
+        """(T, Ek, Ep, Etot) arrays of shape (nsteps + 1, nmol)."""
+        T, Ek, Ep = (np.array(self.history[k]) for k in ("T", "Ek", "Ep"))
+        return T, Ek, Ep, Ek + Ep
+
+
+class XL_BOMD(Molecular_Dynamics_Basic):
+    """XL-BOMD (MolecularDynamics.py:1373-1605) with the trajectory batch resident on the device."""
+
+    def __init__(self, damp=None, xl_bomd_params=dict(), *args, **kwargs):
+        if damp is not None:
+            raise NotImplementedError("the Langevin-damped XL-BOMD variant is not part of this path")
+        if "max_rank" in xl_bomd_params:
+            raise NotImplementedError("KSA-XL-BOMD (max_rank) is not part of this path")
+        self.k = int(xl_bomd_params["k"])
+        self.xl_bomd_params = xl_bomd_params
+        self.damp = damp
+        super().__init__(*args, **kwargs)
+        self.traj = None
+
+    def _thermo_potential(self, molecule):
+        return molecule.Etot + molecule.Electronic_entropy
+
+    def initialize(self, molecule, remove_com=None, learned_parameters=dict(), steps=None, *args, **kwargs):
+        molecule.Electronic_entropy = torch.zeros(molecule.species.shape[0], dtype=torch.float64,
+                                                  device=molecule.coordinates.device)
+        super().initialize(molecule, remove_com=remove_com, learned_parameters=learned_parameters, steps=steps)
+        eng = molecule.engine()
+        b = eng.b
+        sp2 = self.seqm_parameters.get("sp2", [False])
+        self.traj = XLTrajectories(eng, self.k, self.timestep, b.atoms_from_padded(molecule.coordinates.detach()),
+                                   b.atoms_from_padded(molecule.velocities.detach()), b.from_padded(molecule.dm),
+                                   density="sp2" if sp2[0] else "eigh")
+        self.traj.set_forces(b.atoms_from_padded(molecule.force.detach()))
+
+    def _do_integrator_step(self, i, molecule, learned_parameters, **kwargs):
+        tr = self.traj
+        tr.step_count = i
+        tr.step()
+        self._sync_molecule(molecule)
+
+    def _sync_molecule(self, molecule):
+        """Write the device state back into the reference's padded attributes."""
+        tr = self.traj
+        b = tr.eng.b
+        n = b.natoms
+        with torch.no_grad():
+            molecule.coordinates.data.copy_(b.atom_to_padded(tr.x[:n]))
+            molecule.velocities = b.atom_to_padded(tr.v[:n].clone())
+            molecule.acc = b.atom_to_padded(tr.a[:n].clone())
+            molecule.force = b.atom_to_padded(tr.f[:n].clone())
+            en = tr.energies()
+            molecule.Etot, molecule.Eelec, molecule.Enuc = en["Etot"], en["Eelec"], en["Enuc"]
+            molecule.Hf, molecule.Eiso = en["Hf"], en["Eiso"]
+            molecule.dm = b.to_padded(tr.D)

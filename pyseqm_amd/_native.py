@@ -33,20 +33,24 @@ _SIGS = {
     "pyseqm_atom_setup": [c_void_p, c_void_p],
     "pyseqm_pair_integrals": [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                               c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
+    "pyseqm_pair_forces": [c_int, c_void_p, c_void_p, c_void_p, c_double, c_void_p, c_void_p, c_void_p],
     "pyseqm_fock": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                     c_void_p, c_void_p, c_void_p, c_void_p],
-    "pyseqm_pad_for_eigh": [c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p],
+    "pyseqm_pad_for_eigh": [c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_int, c_void_p],
     "pyseqm_density_from_evecs": [c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p],
     "pyseqm_mix": [c_void_p, c_int, c_int, c_double, c_void_p, c_void_p, c_void_p, c_void_p],
     "pyseqm_copy_active": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p],
     "pyseqm_scf_check": [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_double,
-                         c_void_p, c_void_p, c_void_p],
+                         c_void_p, c_void_p, c_void_p, c_int, c_void_p],
     "pyseqm_gemm": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p],
-    "pyseqm_diis_residual": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
-    "pyseqm_diis_dots": [c_void_p, c_int, c_void_p, c_longlong, c_int, c_int, c_int, c_void_p, c_void_p],
+    "pyseqm_diis_residual": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                             c_int, c_void_p],
+    "pyseqm_diis_dots": [c_void_p, c_int, c_void_p, c_longlong, c_int, c_int, c_int, c_void_p, c_void_p, c_int,
+                         c_void_p],
     "pyseqm_diis_combine": [c_void_p, c_int, c_int, c_void_p, c_longlong, c_int, c_void_p, c_int, c_void_p,
                             c_void_p],
-    "pyseqm_sp2_init": [c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
+    "pyseqm_sp2_init": [c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
+                        c_void_p],
     "pyseqm_sp2_iterate": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                            c_int, c_double, c_int, c_void_p, c_void_p],
     "pyseqm_dgemm_sym": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
@@ -54,8 +58,17 @@ _SIGS = {
     "pyseqm_sp2_x_to_f64": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                             c_void_p],
     "pyseqm_sp2_unpad": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
+    "pyseqm_pair_forces_xl": [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_void_p, c_void_p,
+                              c_void_p],
+    "pyseqm_xl_energy": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p],
 }
-_OPTIONAL = {}
+# entry points without a layout argument: name -> full argtypes
+_RAW_SIGS = {
+    "pyseqm_xl_propagate": [c_longlong, c_void_p, c_void_p, c_void_p, c_longlong, c_int, c_void_p, c_double,
+                            c_double, c_int, c_void_p],
+    "pyseqm_md_verlet": [c_int, c_int, c_double, c_double, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                         c_void_p],
+}
 _lib = None
 
 
@@ -75,12 +88,16 @@ def lib():
             f = getattr(L, name)
             f.argtypes = [_P] + args
             f.restype = c_int
+        for name, args in _RAW_SIGS.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = c_int
         _lib = L
     return _lib
 
 
 def exported_symbols():
-    return list(_SIGS)
+    return list(_SIGS) + list(_RAW_SIGS)
 
 
 def stream_ptr(device=None):
@@ -101,6 +118,20 @@ def call(name, layout, *args):
     rc = getattr(lib(), name)(ctypes.byref(layout), *args)
     if rc != 0:
         raise NativeError(f"{name} failed: {_CODES.get(rc, rc)}")
+
+
+def call_raw(name, *args):
+    rc = getattr(lib(), name)(*args)
+    if rc != 0:
+        raise NativeError(f"{name} failed: {_CODES.get(rc, rc)}")
+
+
+MAX_RED_BLOCKS = 256
+
+
+def red_blocks(nmax):
+    """Blocks per molecule of the two-stage reductions (enough to fill the GPU for large n)."""
+    return int(min(MAX_RED_BLOCKS, max(1, (nmax * nmax + 8191) // 8192)))
 
 
 def require_device(device):

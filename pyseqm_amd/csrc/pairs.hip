@@ -154,10 +154,15 @@ __device__ void local_xh(double r, double da, double qa1, double r0a, double r1a
 }
 
 // Rotation with R v = e_x built from the quaternion of v (two_elec_two_center_int.py:1576-1631),
-// including its antipodal fallback (|1 + v_x| < 1e-7 -> 180 deg turn about z).
-__device__ inline void quat_frame(double vx, double vy, double vz, double R[3][3]) {
+// including its antipodal fallback (|1 + v_x| < 1e-7 -> constant 180 deg turn about z).
+// mode 0: as the reference; 1: never take the fallback (smooth in v); 2: always the fallback.
+// Finite differences use 1 or 2 chosen from the undisplaced geometry, which reproduces the
+// reference's autograd derivative (zero frame derivative inside the fallback cone).
+__device__ inline bool quat_flips(double vx) { return fabs(1.0 + vx) < 1.0e-7; }
+
+__device__ inline void quat_frame(double vx, double vy, double vz, double R[3][3], int mode = 0) {
   double qy = vz, qz = -vy, qw = 1.0 + vx;
-  if (fabs(qw) < 1.0e-7) {
+  if (mode == 2 || (mode == 0 && fabs(qw) < 1.0e-7)) {
     qy = 0.0;
     qz = 1.0;
     qw = 0.0;
@@ -389,16 +394,106 @@ struct PairGeom {
   double r;     // bohr
 };
 
-__device__ inline PairGeom pair_geom(const double* coords, int i, int j) {
+__device__ inline PairGeom geom_from_vector(double dx, double dy, double dz) {
   PairGeom g;
-  const double dx = coords[3 * j] - coords[3 * i], dy = coords[3 * j + 1] - coords[3 * i + 1],
-               dz = coords[3 * j + 2] - coords[3 * i + 2];
   const double d = sqrt(dx * dx + dy * dy + dz * dz);
   g.x[0] = dx / d;
   g.x[1] = dy / d;
   g.x[2] = dz / d;
   g.r = d * (1.0 / kA0);
   return g;
+}
+
+__device__ inline PairGeom pair_geom(const double* coords, int i, int j) {
+  return geom_from_vector(coords[3 * j] - coords[3 * i], coords[3 * j + 1] - coords[3 * i + 1],
+                          coords[3 * j + 2] - coords[3 * i + 2]);
+}
+
+// Molecular-frame two-electron integrals of a heavy-heavy pair, column by column:
+// fn(k, l, (mu nu|la si)) for k = tril(mu, nu) on atom i and l = tril(la, si) on atom j.
+template <class Fn>
+__device__ inline void xx_integrals(const PairGeom& g, const double* pi, const double* pj, Fn&& fn, int fmode = 0) {
+  double ri[22];
+  local_xx(g.r, pi[PAR_DD], pj[PAR_DD], pi[PAR_QQ], pj[PAR_QQ], pi[PAR_RHO0], pj[PAR_RHO0], pi[PAR_RHO1],
+           pj[PAR_RHO1], pi[PAR_RHO2], pj[PAR_RHO2], ri);
+  double R[3][3];
+  quat_frame(-g.x[0], -g.x[1], -g.x[2], R, fmode);
+  double T[10][10], Lm[10][10];
+  pair_map(R, T);
+  local_tensor(ri, Lm);
+#pragma unroll
+  for (int l = 0; l < 10; ++l) {
+    double v[10];
+#pragma unroll
+    for (int p = 0; p < 10; ++p) {
+      double s = 0.0;
+#pragma unroll
+      for (int q = 0; q < 10; ++q) s += Lm[p][q] * T[l][q];
+      v[p] = s;
+    }
+#pragma unroll
+    for (int k = 0; k < 10; ++k) {
+      double s = 0.0;
+#pragma unroll
+      for (int p = 0; p < 10; ++p) s += T[k][p] * v[p];
+      fn(k, l, s);
+    }
+  }
+}
+
+// Heavy-H integrals (mu nu|s s), k = tril(mu, nu) on the heavy atom i.
+template <class Fn>
+__device__ inline void xh_integrals(const PairGeom& g, const double* pi, const double* pj, Fn&& fn, int fmode = 0) {
+  double ri[4];
+  local_xh(g.r, pi[PAR_DD], pi[PAR_QQ], pi[PAR_RHO0], pi[PAR_RHO1], pi[PAR_RHO2], pj[PAR_RHO0], ri);
+  double R[3][3];
+  quat_frame(-g.x[0], -g.x[1], -g.x[2], R, fmode);
+  double T[10][10];
+  pair_map(R, T);
+#pragma unroll
+  for (int k = 0; k < 10; ++k) fn(k, T[k][0] * ri[0] + T[k][1] * ri[1] + T[k][2] * ri[2] + (T[k][5] + T[k][9]) * ri[3]);
+}
+
+__device__ inline double hh_integral(const PairGeom& g, const double* pi, const double* pj) {
+  return kEV / sqrt(g.r * g.r + (pi[PAR_RHO0] + pj[PAR_RHO0]) * (pi[PAR_RHO0] + pj[PAR_RHO0]));
+}
+
+// Off-diagonal core block H_ij[a][b] = S_ij[a][b] (beta_a(i) + beta_b(j)) / 2 for NI x NJ orbitals.
+template <int NI, int NJ>
+__device__ inline void core_block(const PairGeom& g, const double* pi, const double* pj, double Hb[4][4],
+                                  int fmode = 0) {
+  double di[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) di[a][b] = 0.0;
+  if (g.r <= kOverlapCut) {
+    const int jc = jcall_of((int)pi[PAR_QN], (int)pj[PAR_QN]);  // validated on the host
+    double S[5];
+    overlap_local(jc, g.r, pi[PAR_ZS], pi[PAR_ZP], pj[PAR_ZS], pj[PAR_ZP], NI == 4, NJ == 4, S);
+    double Rp[3][3];
+    quat_frame(g.x[0], g.x[1], g.x[2], Rp, fmode);
+    di[0][0] = S[0];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      di[1 + m][0] = S[1] * Rp[0][m];
+      di[0][1 + m] = -S[2] * Rp[0][m];
+    }
+    if (NJ == 4) {
+      const double D[3] = {-S[3], S[4], S[4]};
+#pragma unroll
+      for (int m = 0; m < 3; ++m)
+#pragma unroll
+        for (int n = 0; n < 3; ++n)
+          di[1 + m][1 + n] = Rp[0][m] * D[0] * Rp[0][n] + Rp[1][m] * D[1] * Rp[1][n] + Rp[2][m] * D[2] * Rp[2][n];
+    }
+  }
+  const double bi[4] = {pi[PAR_BS], pi[PAR_BP], pi[PAR_BP], pi[PAR_BP]};
+  const double bj[4] = {pj[PAR_BS], pj[PAR_BP], pj[PAR_BP], pj[PAR_BP]};
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) Hb[a][b] = (a < NI && b < NJ) ? di[a][b] * 0.5 * (bi[a] + bj[b]) : 0.0;
 }
 
 // ------------------------------------------------------------------- tile kernels
@@ -420,6 +515,8 @@ __global__ __launch_bounds__(64) void k_pair_tiles(pyseqm_layout_t L, int method
   const bool valid = lane_pair(t, lane, i, j);
   constexpr int NR = KIND == HH ? 1 : 10;
   constexpr int NC = KIND == XX ? 10 : 1;
+  constexpr int NI = KIND == HH ? 1 : 4;
+  constexpr int NJ = KIND == XX ? 4 : 1;
   double rowv[NR], colv[NC];
 #pragma unroll
   for (int k = 0; k < NR; ++k) rowv[k] = 0.0;
@@ -433,104 +530,42 @@ __global__ __launch_bounds__(64) void k_pair_tiles(pyseqm_layout_t L, int method
     const double* pi = par + (size_t)i * PYSEQM_NPAR;
     const double* pj = par + (size_t)j * PYSEQM_NPAR;
     const PairGeom g = pair_geom(coords, i, j);
-    const int Zi = (int)pi[PAR_Z], Zj = (int)pj[PAR_Z];
+    const double ti = pi[PAR_TORE], tj = pj[PAR_TORE];
+    double gam = 0.0;
+    if constexpr (KIND == XX) {
+      xx_integrals(g, pi, pj, [&](int k, int l, double w) {
+        wt[(size_t)(k * 10 + l) * 64] = w;
+        if (l == 0) rowv[k] = -tj * w;  // e1b: electrons of i, core of j
+        if (k == 0) colv[l] = -ti * w;  // e2a: electrons of j, core of i
+        if (k == 0 && l == 0) gam = w;
+      });
+    } else if constexpr (KIND == XH) {
+      xh_integrals(g, pi, pj, [&](int k, double w) {
+        wt[(size_t)k * 64] = w;
+        rowv[k] = -tj * w;
+        if (k == 0) gam = w;
+      });
+      colv[0] = -ti * gam;
+    } else {
+      gam = hh_integral(g, pi, pj);
+      wt[0] = gam;
+      rowv[0] = -tj * gam;
+      colv[0] = -ti * gam;
+    }
+    double Hb[4][4];
+    core_block<NI, NJ>(g, pi, pj, Hb);
     const long long mo = L.mol_mat_off[t.mol];
     const int norb = L.mol_norb[t.mol];
     const int ai = L.atom_ao[i], aj = L.atom_ao[j];
-    double R[3][3];
-    quat_frame(-g.x[0], -g.x[1], -g.x[2], R);
-    double gam;
-    if constexpr (KIND == XX) {
-      double ri[22];
-      local_xx(g.r, pi[PAR_DD], pj[PAR_DD], pi[PAR_QQ], pj[PAR_QQ], pi[PAR_RHO0], pj[PAR_RHO0], pi[PAR_RHO1],
-               pj[PAR_RHO1], pi[PAR_RHO2], pj[PAR_RHO2], ri);
-      double T[10][10], Lm[10][10];
-      pair_map(R, T);
-      local_tensor(ri, Lm);
-      double w00 = 0.0;
-#pragma unroll
-      for (int l = 0; l < 10; ++l) {
-        double v[10];
-#pragma unroll
-        for (int p = 0; p < 10; ++p) {
-          double s = 0.0;
-#pragma unroll
-          for (int q = 0; q < 10; ++q) s += Lm[p][q] * T[l][q];
-          v[p] = s;
-        }
-#pragma unroll
-        for (int k = 0; k < 10; ++k) {
-          double s = 0.0;
-#pragma unroll
-          for (int p = 0; p < 10; ++p) s += T[k][p] * v[p];
-          wt[(size_t)(k * 10 + l) * 64] = s;
-          if (l == 0 && k == 0) w00 = s;
-          if (l == 0) rowv[k] = -pj[PAR_TORE] * s;   // e1b: electrons of i, core of j
-          if (k == 0) colv[l] = -pi[PAR_TORE] * s;   // e2a: electrons of j, core of i
-        }
-      }
-      gam = w00;
-    } else if constexpr (KIND == XH) {
-      double ri[4];
-      local_xh(g.r, pi[PAR_DD], pi[PAR_QQ], pi[PAR_RHO0], pi[PAR_RHO1], pi[PAR_RHO2], pj[PAR_RHO0], ri);
-      double T[10][10];
-      pair_map(R, T);
-#pragma unroll
-      for (int k = 0; k < 10; ++k) {
-        const double s = T[k][0] * ri[0] + T[k][1] * ri[1] + T[k][2] * ri[2] + (T[k][5] + T[k][9]) * ri[3];
-        wt[(size_t)k * 64] = s;
-        rowv[k] = -pj[PAR_TORE] * s;
-      }
-      gam = ri[0];
-      colv[0] = -pi[PAR_TORE] * gam;
-    } else {
-      gam = kEV / sqrt(g.r * g.r + (pi[PAR_RHO0] + pj[PAR_RHO0]) * (pi[PAR_RHO0] + pj[PAR_RHO0]));
-      wt[0] = gam;
-      rowv[0] = -pj[PAR_TORE] * gam;
-      colv[0] = -pi[PAR_TORE] * gam;
-    }
-
-    // overlap block and off-diagonal core Hamiltonian
-    constexpr int NI = KIND == HH ? 1 : 4;
-    constexpr int NJ = KIND == XX ? 4 : 1;
-    double di[4][4];
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-      for (int b = 0; b < 4; ++b) di[a][b] = 0.0;
-    if (g.r <= kOverlapCut) {
-      const int jc = jcall_of((int)pi[PAR_QN], (int)pj[PAR_QN]);  // validated on the host
-      double S[5];
-      overlap_local(jc, g.r, pi[PAR_ZS], pi[PAR_ZP], pj[PAR_ZS], pj[PAR_ZP], NI == 4, NJ == 4, S);
-      double Rp[3][3];
-      quat_frame(g.x[0], g.x[1], g.x[2], Rp);
-      di[0][0] = S[0];
-#pragma unroll
-      for (int m = 0; m < 3; ++m) {
-        di[1 + m][0] = S[1] * Rp[0][m];
-        di[0][1 + m] = -S[2] * Rp[0][m];
-      }
-      if (NJ == 4) {
-        const double D[3] = {-S[3], S[4], S[4]};
-#pragma unroll
-        for (int m = 0; m < 3; ++m)
-#pragma unroll
-          for (int n = 0; n < 3; ++n) di[1 + m][1 + n] = Rp[0][m] * D[0] * Rp[0][n] + Rp[1][m] * D[1] * Rp[1][n] +
-                                                         Rp[2][m] * D[2] * Rp[2][n];
-      }
-    }
-    const double bi[4] = {pi[PAR_BS], pi[PAR_BP], pi[PAR_BP], pi[PAR_BP]};
-    const double bj[4] = {pj[PAR_BS], pj[PAR_BP], pj[PAR_BP], pj[PAR_BP]};
     double* Hm = H + mo;
 #pragma unroll
     for (int a = 0; a < NI; ++a)
 #pragma unroll
       for (int b = 0; b < NJ; ++b) {
-        const double h = di[a][b] * 0.5 * (bi[a] + bj[b]);
-        Hm[(size_t)(ai + a) * norb + aj + b] = h;
-        Hm[(size_t)(aj + b) * norb + ai + a] = h;
+        Hm[(size_t)(ai + a) * norb + aj + b] = Hb[a][b];
+        Hm[(size_t)(aj + b) * norb + ai + a] = Hb[a][b];
       }
-    en = core_repulsion(method, Zi, Zj, g.r, gam, pi, pj);
+    en = core_repulsion(method, (int)pi[PAR_Z], (int)pj[PAR_Z], g.r, gam, pi, pj);
     if (enuc_pair) enuc_pair[pair_ref[(size_t)tile * 64 + lane]] = en;
   } else {
 #pragma unroll
@@ -539,6 +574,163 @@ __global__ __launch_bounds__(64) void k_pair_tiles(pyseqm_layout_t L, int method
   write_partials(part, tile, lane, rowv, colv);
   const double es = wave_sum(en);
   if (lane == 0) enuc_tile[tile] = es;
+}
+
+// Hellmann-Feynman pair energy at fixed density: the part of 0.5 sum P (H + F) + E_nuc that
+// depends on the geometry of pair (i, j) (basics.py:1315-1336 differentiates the same energy with
+// scf_backward = 0): 2 sum P_ij H_ij + sum P_ii e1b + sum P_jj e2a + sum P_ii P_jj (..|..)
+// - 1/2 sum P_ij P_ij (..|..) + E_nuc(i, j).
+// With XL = true the energy is the extended-Lagrangian one, sum D F(P) - 1/2 (F(P) - H) P + E_nuc
+// (energy.py:76-88, differentiated at fixed D and P by xlbomd.py:536-551): H and the electron-core
+// terms contract with D, the two-electron terms are B(D, P) - 1/2 B(P, P) for the symmetric
+// bilinear form B(X, Y) = sum X G(Y); D = P gives the energy above.
+struct PairDensity {
+  double pa[10], pb[10];  // diagonal blocks, tril, off-diagonal entries doubled
+  double pab[4][4];       // block (i, j)
+};
+
+template <int KIND, bool XL>
+__device__ double pair_energy(int method, const PairGeom& g, const double* pi, const double* pj,
+                              const PairDensity& P, const PairDensity& D, int wmode, int smode) {
+  constexpr int NI = KIND == HH ? 1 : 4;
+  constexpr int NJ = KIND == XX ? 4 : 1;
+  const double ti = pi[PAR_TORE], tj = pj[PAR_TORE];
+  double e = 0.0, gam = 0.0;
+  // Coulomb weight of w_kl: P_a P_b (plain) or D_a P_b + P_a D_b - P_a P_b (XL)
+  auto coul = [&](int k, int l) {
+    return XL ? D.pa[k] * P.pb[l] + P.pa[k] * D.pb[l] - P.pa[k] * P.pb[l] : P.pa[k] * P.pb[l];
+  };
+  // exchange weight: 1/2 P_ab (plain) or D_ab - 1/2 P_ab (XL), times K(P)_ab
+  auto xw = [&](int a, int b) { return XL ? D.pab[a][b] - 0.5 * P.pab[a][b] : 0.5 * P.pab[a][b]; };
+  if constexpr (KIND == XX) {
+    double K[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) K[a][b] = 0.0;
+    xx_integrals(g, pi, pj, [&](int k, int l, double w) {
+      const int mu = kTriI[k], nu = kTriJ[k], la = kTriI[l], si = kTriJ[l];
+      e += coul(k, l) * w;
+      if (l == 0) e -= tj * D.pa[k] * w;
+      if (k == 0) e -= ti * D.pb[l] * w;
+      if (k == 0 && l == 0) gam = w;
+      K[mu][la] += w * P.pab[nu][si];
+      if (mu != nu) K[nu][la] += w * P.pab[mu][si];
+      if (la != si) K[mu][si] += w * P.pab[nu][la];
+      if (mu != nu && la != si) K[nu][si] += w * P.pab[mu][la];
+    }, wmode);
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) e -= xw(a, b) * K[a][b];
+  } else if constexpr (KIND == XH) {
+    double K[4] = {0.0, 0.0, 0.0, 0.0};
+    xh_integrals(g, pi, pj, [&](int k, double w) {
+      const int mu = kTriI[k], nu = kTriJ[k];
+      e += coul(k, 0) * w - tj * D.pa[k] * w;
+      if (k == 0) {
+        gam = w;
+        e -= ti * D.pb[0] * w;
+      }
+      K[mu] += w * P.pab[nu][0];
+      if (mu != nu) K[nu] += w * P.pab[mu][0];
+    }, wmode);
+#pragma unroll
+    for (int a = 0; a < 4; ++a) e -= xw(a, 0) * K[a];
+  } else {
+    gam = hh_integral(g, pi, pj);
+    e = (coul(0, 0) - tj * D.pa[0] - ti * D.pb[0] - xw(0, 0) * P.pab[0][0]) * gam;
+  }
+  double Hb[4][4];
+  core_block<NI, NJ>(g, pi, pj, Hb, smode);
+#pragma unroll
+  for (int a = 0; a < NI; ++a)
+#pragma unroll
+    for (int b = 0; b < NJ; ++b) e += 2.0 * D.pab[a][b] * Hb[a][b];
+  return e + core_repulsion(method, (int)pi[PAR_Z], (int)pj[PAR_Z], g.r, gam, pi, pj);
+}
+
+template <int NI, int NJ>
+__device__ void load_pair_density(const double* Pm, int n, int ai, int aj, PairDensity& D) {
+#pragma unroll
+  for (int k = 0; k < 10; ++k) {
+    const int mu = kTriI[k], nu = kTriJ[k];
+    const double f = mu == nu ? 1.0 : 2.0;
+    D.pa[k] = (mu < NI && nu < NI) ? f * Pm[(size_t)(ai + mu) * n + ai + nu] : 0.0;
+    D.pb[k] = (mu < NJ && nu < NJ) ? f * Pm[(size_t)(aj + mu) * n + aj + nu] : 0.0;
+  }
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) D.pab[a][b] = (a < NI && b < NJ) ? Pm[(size_t)(ai + a) * n + aj + b] : 0.0;
+}
+
+// Forces: dE_pair/dR (R = r_j - r_i, Angstrom) by a 4th-order central difference of the pair
+// energy at fixed density (step h); +dE/dR on atom i, -dE/dR on atom j, reduced per tile.
+// XL: Dm is the density contracted with H (the XL-BOMD D), P the propagated field.
+template <int KIND, bool XL>
+__global__ __launch_bounds__(64) void k_pair_forces(pyseqm_layout_t L, int method, int tile_base,
+                                                     const double* __restrict__ coords,
+                                                     const double* __restrict__ par, const double* __restrict__ P,
+                                                     const double* __restrict__ Dm, double h,
+                                                     double* __restrict__ part) {
+  const int tile = tile_base + blockIdx.x;
+  const int lane = threadIdx.x;
+  const Tile t = load_tile(L.tiles, tile);
+  int i, j;
+  const bool valid = lane_pair(t, lane, i, j);
+  double rowv[3] = {0.0, 0.0, 0.0}, colv[3] = {0.0, 0.0, 0.0};
+  if (valid) {
+    const double* pi = par + (size_t)i * PYSEQM_NPAR;
+    const double* pj = par + (size_t)j * PYSEQM_NPAR;
+    const long long mo = L.mol_mat_off[t.mol];
+    const int n = L.mol_norb[t.mol];
+    const int ai = L.atom_ao[i], aj = L.atom_ao[j];
+    constexpr int NI = KIND == HH ? 1 : 4;
+    constexpr int NJ = KIND == XX ? 4 : 1;
+    PairDensity Pd, Dd;
+    load_pair_density<NI, NJ>(P + mo, n, ai, aj, Pd);
+    if constexpr (XL) load_pair_density<NI, NJ>(Dm + mo, n, ai, aj, Dd);
+    const PairDensity& Dr = XL ? Dd : Pd;
+    const double R0[3] = {coords[3 * j] - coords[3 * i], coords[3 * j + 1] - coords[3 * i + 1],
+                          coords[3 * j + 2] - coords[3 * i + 2]};
+    const double c1 = 8.0 / (12.0 * h), c2 = -1.0 / (12.0 * h);
+    const PairGeom g0 = geom_from_vector(R0[0], R0[1], R0[2]);
+    const int wmode = quat_flips(-g0.x[0]) ? 2 : 1;  // frame of -x_ij (integrals)
+    const int smode = quat_flips(g0.x[0]) ? 2 : 1;   // frame of +x_ij (overlaps)
+#pragma unroll 1
+    for (int x = 0; x < 3; ++x) {
+      double gx = 0.0;
+#pragma unroll 1
+      for (int sgn = -2; sgn <= 2; ++sgn) {
+        if (sgn == 0) continue;
+        double R[3] = {R0[0], R0[1], R0[2]};
+        R[x] += sgn * h;
+        const double e =
+            pair_energy<KIND, XL>(method, geom_from_vector(R[0], R[1], R[2]), pi, pj, Pd, Dr, wmode, smode);
+        const double c = (sgn == 1 || sgn == -1) ? c1 : c2;
+        gx += (sgn > 0 ? c : -c) * e;
+      }
+      rowv[x] = gx;    // force on i = -dE/dr_i = +dE/dR
+      colv[x] = -gx;   // force on j
+    }
+  }
+  write_partials(part, tile, lane, rowv, colv);
+}
+
+__global__ void k_atom_forces(pyseqm_layout_t L, const double* __restrict__ part, double* __restrict__ F) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= L.natoms) return;
+  double f[3] = {0.0, 0.0, 0.0};
+  for (int e = L.csr_off[a]; e < L.csr_off[a + 1]; ++e) {
+    const double* src = part + (size_t)L.csr_idx[e] * kPartStride;
+    f[0] += src[0];
+    f[1] += src[1];
+    f[2] += src[2];
+  }
+  F[3 * a] = f[0];
+  F[3 * a + 1] = f[1];
+  F[3 * a + 2] = f[2];
 }
 
 // Diagonal core-Hamiltonian blocks: U on the diagonal plus the electron-core attractions
@@ -573,20 +765,22 @@ __global__ void k_hcore_diag(pyseqm_layout_t L, const double* __restrict__ par, 
   }
 }
 
-// Per-molecule sum of per-tile values (and optionally per-atom values), fixed order.
+// Per-molecule sum of per-tile values (and optionally per-atom values): one wave per molecule,
+// lanes stride over the molecule's tiles / atoms, fixed lane assignment and shuffle tree.
 __global__ void k_mol_sum(pyseqm_layout_t L, const double* __restrict__ tile_v, const double* __restrict__ atom_v,
                           const unsigned char* __restrict__ active, double* __restrict__ out) {
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= L.nmol) return;
+  const int m = blockIdx.x;
   if (active && !active[m]) return;
+  const int lane = threadIdx.x;
   double s = 0.0;
   for (int kind = 0; kind < 3; ++kind) {
     const int* off = L.mol_tile_off + kind * (L.nmol + 1);
-    for (int t = off[m]; t < off[m + 1]; ++t) s += tile_v[t];
+    for (int t = off[m] + lane; t < off[m + 1]; t += 64) s += tile_v[t];
   }
   if (atom_v)
-    for (int a = L.mol_atom_off[m]; a < L.mol_atom_off[m + 1]; ++a) s += atom_v[a];
-  out[m] = s;
+    for (int a = L.mol_atom_off[m] + lane; a < L.mol_atom_off[m + 1]; a += 64) s += atom_v[a];
+  s = wave_sum(s);
+  if (lane == 0) out[m] = s;
 }
 
 }  // namespace
@@ -594,9 +788,42 @@ __global__ void k_mol_sum(pyseqm_layout_t L, const double* __restrict__ tile_v, 
 namespace pyseqm {
 void launch_mol_sum(const pyseqm_layout_t* L, const double* tile_v, const double* atom_v,
                     const unsigned char* active, double* out, hipStream_t s) {
-  if (L->nmol > 0) k_mol_sum<<<(L->nmol + 127) / 128, 128, 0, s>>>(*L, tile_v, atom_v, active, out);
+  if (L->nmol > 0) k_mol_sum<<<L->nmol, 64, 0, s>>>(*L, tile_v, atom_v, active, out);
 }
 }  // namespace pyseqm
+
+namespace {
+template <bool XL>
+int launch_pair_forces(const pyseqm_layout_t* L, int method, const double* coords, const double* atom_par,
+                       const double* P, const double* D, double h, double* part, double* forces, void* stream) {
+  if (!L || method < PYSEQM_MNDO || method > PYSEQM_PM3 || !(h > 0.0)) return PYSEQM_EINVAL;
+  hipStream_t s = (hipStream_t)stream;
+  int base = 0;
+  if (L->ntile[0])
+    k_pair_forces<XX, XL><<<L->ntile[0], 64, 0, s>>>(*L, method, base, coords, atom_par, P, D, h, part);
+  base += L->ntile[0];
+  if (L->ntile[1])
+    k_pair_forces<XH, XL><<<L->ntile[1], 64, 0, s>>>(*L, method, base, coords, atom_par, P, D, h, part);
+  base += L->ntile[1];
+  if (L->ntile[2])
+    k_pair_forces<HH, XL><<<L->ntile[2], 64, 0, s>>>(*L, method, base, coords, atom_par, P, D, h, part);
+  if (L->natoms) k_atom_forces<<<(L->natoms + 127) / 128, 128, 0, s>>>(*L, part, forces);
+  return hipGetLastError() == hipSuccess ? PYSEQM_OK : PYSEQM_ELAUNCH;
+}
+}  // namespace
+
+extern "C" int pyseqm_pair_forces(const pyseqm_layout_t* L, int method, const double* coords,
+                                  const double* atom_par, const double* P, double h, double* part, double* forces,
+                                  void* stream) {
+  return launch_pair_forces<false>(L, method, coords, atom_par, P, nullptr, h, part, forces, stream);
+}
+
+extern "C" int pyseqm_pair_forces_xl(const pyseqm_layout_t* L, int method, const double* coords,
+                                     const double* atom_par, const double* D, const double* P, double h,
+                                     double* part, double* forces, void* stream) {
+  if (!D || !P) return PYSEQM_EINVAL;
+  return launch_pair_forces<true>(L, method, coords, atom_par, P, D, h, part, forces, stream);
+}
 
 extern "C" int pyseqm_atom_setup(const pyseqm_layout_t* L, double* atom_par, void* stream) {
   if (!L || L->natoms < 0) return PYSEQM_EINVAL;

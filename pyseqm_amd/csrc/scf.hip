@@ -7,8 +7,7 @@
 //   mixing (scf_loop.py:266-267, 1052-1056) and the 4-way convergence test get_error
 //     (scf_loop.py:106-147),
 //   the Pulay/DIIS residual FP - PF, its max-norm, the B-matrix row and the extrapolated Fock
-//     matrix (scf_loop.py:997-1035),
-//   the XL-BOMD density propagation (MolecularDynamics.py:1418-1427).
+//     matrix (scf_loop.py:997-1035).
 #include "common.h"
 
 using namespace pyseqm;
@@ -44,46 +43,6 @@ __device__ inline double block_max(double v, double* sh) {
     for (int k = 0; k < (int)(blockDim.x >> 6); ++k) s = fmax(s, sh[k]);
   __syncthreads();
   return s;
-}
-
-// Packed F of active molecule -> zero-padded (nmax x nmax) matrix whose padded diagonal is
-// hN + (1 + 0.005 t) dE, t = 1, 2, ... above the Gershgorin range [h1, hN] (diag.py:172-192),
-// so that padded eigenpairs sort above every real one.  One block per molecule.
-__global__ void k_pad_for_eigh(pyseqm_layout_t L, const double* __restrict__ F, const int* __restrict__ idx,
-                               int nmax, double* __restrict__ out) {
-  __shared__ double sh[kEl / 64];
-  const int k = blockIdx.x;
-  const int m = idx[k];
-  const int n = L.mol_norb[m];
-  const double* Fm = F + L.mol_mat_off[m];
-  double* O = out + (size_t)k * nmax * nmax;
-  double hi = -1.0e300, lo = 1.0e300;
-  for (int r = threadIdx.x; r < n; r += blockDim.x) {
-    double s = 0.0;
-    for (int c = 0; c < n; ++c) s += fabs(Fm[(size_t)r * n + c]);
-    const double d = Fm[(size_t)r * n + r];
-    const double rad = s - fabs(d);
-    hi = fmax(hi, d + rad);
-    lo = fmin(lo, d - rad);
-  }
-  const double hN = block_max(hi, sh);
-  const double mlo = block_max(-lo, sh);
-  __shared__ double hs[2];
-  if (threadIdx.x == 0) {
-    hs[0] = hN;
-    hs[1] = hN + mlo;  // dE = hN - h1
-  }
-  __syncthreads();
-  const double top = hs[0], dE = hs[1];
-  for (size_t e = threadIdx.x; e < (size_t)nmax * nmax; e += blockDim.x) {
-    const int r = (int)(e / nmax), c = (int)(e % nmax);
-    double v = 0.0;
-    if (r < n && c < n)
-      v = Fm[(size_t)r * n + c];
-    else if (r == c)
-      v = top + (1.0 + 0.005 * (r - n + 1)) * dE;
-    O[e] = v;
-  }
 }
 
 // P = 2 sum_{k < nocc} V[:,k] V[:,k]^T (diag.py:230-232) from the padded eigenvectors (columns),
@@ -132,17 +91,18 @@ __global__ void k_mix(pyseqm_layout_t L, const int* __restrict__ idx, double a, 
 
 // Convergence test of get_error (scf_loop.py:106-147): |dE| <= eps, max|FP-PF| <= diis_tol
 // (50 eps in the reference; the mixed-precision density step passes its precision floor),
-// ||dP||_F / (4 natoms) <= 2 eps, max|dP| <= 15 eps.  Writes notconv[m] and the metrics.
-__global__ void k_scf_check(pyseqm_layout_t L, const int* __restrict__ idx, const double* __restrict__ P,
-                            const double* __restrict__ Pold, const double* __restrict__ e_new,
-                            double* __restrict__ e_old, const double* __restrict__ diis_err, double eps,
-                            double diis_tol, unsigned char* __restrict__ notconv, double* __restrict__ metrics) {
+// ||dP||_F / (4 natoms) <= 2 eps, max|dP| <= 15 eps.  Two stages: per-block partial sums of
+// dP^2 and max|dP| over nb blocks per molecule, then one thread per molecule (fixed order).
+__global__ void k_check_partial(pyseqm_layout_t L, const int* __restrict__ idx, int nb,
+                                const double* __restrict__ P, const double* __restrict__ Pold,
+                                double* __restrict__ red) {
   __shared__ double sh[kEl / 64];
-  const int m = idx[blockIdx.x];
-  const int n = L.mol_norb[m];
+  const int k = blockIdx.y;
+  const int m = idx[k];
+  const size_t n2 = (size_t)L.mol_norb[m] * L.mol_norb[m];
   const size_t off = L.mol_mat_off[m];
   double ss = 0.0, mx = 0.0;
-  for (size_t e = threadIdx.x; e < (size_t)n * n; e += blockDim.x) {
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n2; e += (size_t)nb * blockDim.x) {
     const double d = P[off + e] - Pold[off + e];
     ss += d * d;
     mx = fmax(mx, fabs(d));
@@ -150,19 +110,35 @@ __global__ void k_scf_check(pyseqm_layout_t L, const int* __restrict__ idx, cons
   const double s2 = block_sum(ss, sh);
   const double m2 = block_max(mx, sh);
   if (threadIdx.x == 0) {
-    const int nat = L.mol_atom_off[m + 1] - L.mol_atom_off[m];
-    const double dm = sqrt(s2) / (4.0 * nat);
-    const double de = e_new[m] - e_old[m];
-    const double dd = diis_err ? diis_err[m] : 0.0;
-    const bool bad = fabs(de) > eps || dd > diis_tol || dm > 2.0 * eps || m2 > 15.0 * eps;
-    notconv[m] = bad ? 1 : 0;
-    if (bad) e_old[m] = e_new[m];
-    if (metrics) {
-      metrics[4 * m + 0] = de;
-      metrics[4 * m + 1] = dd;
-      metrics[4 * m + 2] = dm;
-      metrics[4 * m + 3] = m2;
-    }
+    red[((size_t)k * nb + blockIdx.x) * 2] = s2;
+    red[((size_t)k * nb + blockIdx.x) * 2 + 1] = m2;
+  }
+}
+
+__global__ void k_check_final(pyseqm_layout_t L, const int* __restrict__ idx, int nact, int nb,
+                              const double* __restrict__ red, const double* __restrict__ e_new,
+                              double* __restrict__ e_old, const double* __restrict__ diis_err, double eps,
+                              double diis_tol, unsigned char* __restrict__ notconv, double* __restrict__ metrics) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nact) return;
+  const int m = idx[k];
+  double s2 = 0.0, m2 = 0.0;
+  for (int b = 0; b < nb; ++b) {
+    s2 += red[((size_t)k * nb + b) * 2];
+    m2 = fmax(m2, red[((size_t)k * nb + b) * 2 + 1]);
+  }
+  const int nat = L.mol_atom_off[m + 1] - L.mol_atom_off[m];
+  const double dm = sqrt(s2) / (4.0 * nat);
+  const double de = e_new[m] - e_old[m];
+  const double dd = diis_err ? diis_err[m] : 0.0;
+  const bool bad = fabs(de) > eps || dd > diis_tol || dm > 2.0 * eps || m2 > 15.0 * eps;
+  notconv[m] = bad ? 1 : 0;
+  if (bad) e_old[m] = e_new[m];
+  if (metrics) {
+    metrics[4 * m + 0] = de;
+    metrics[4 * m + 1] = dd;
+    metrics[4 * m + 2] = dm;
+    metrics[4 * m + 3] = m2;
   }
 }
 
@@ -216,39 +192,68 @@ __global__ __launch_bounds__(256) void k_gemm64(pyseqm_layout_t L, const int* __
     }
 }
 
-// C = X - X^T (X = F P, so C = FP - PF) into ring slot; diis_err[m] = max|C|.
-__global__ void k_antisym(pyseqm_layout_t L, const int* __restrict__ idx, const double* __restrict__ X,
-                          double* __restrict__ Cout, double* __restrict__ diis_err) {
+// C = X - X^T (X = F P, so C = FP - PF) into the ring slot, with per-block partial max|C|.
+__global__ void k_antisym(pyseqm_layout_t L, const int* __restrict__ idx, int nb, const double* __restrict__ X,
+                          double* __restrict__ Cout, double* __restrict__ red) {
   __shared__ double sh[kEl / 64];
-  const int m = idx[blockIdx.x];
+  const int k = blockIdx.y;
+  const int m = idx[k];
   const int n = L.mol_norb[m];
   const size_t off = L.mol_mat_off[m];
   double mx = 0.0;
-  for (size_t e = threadIdx.x; e < (size_t)n * n; e += blockDim.x) {
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < (size_t)n * n; e += (size_t)nb * blockDim.x) {
     const int r = (int)(e / n), c = (int)(e % n);
     const double v = X[off + e] - X[off + (size_t)c * n + r];
     Cout[off + e] = v;
     mx = fmax(mx, fabs(v));
   }
   const double r = block_max(mx, sh);
-  if (threadIdx.x == 0) diis_err[m] = r;
+  if (threadIdx.x == 0) red[(size_t)k * nb + blockIdx.x] = r;
+}
+
+__global__ void k_antisym_final(const int* __restrict__ idx, int nact, int nb, const double* __restrict__ red,
+                                double* __restrict__ diis_err) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nact) return;
+  double mx = 0.0;
+  for (int b = 0; b < nb; ++b) mx = fmax(mx, red[(size_t)k * nb + b]);
+  diis_err[idx[k]] = mx;
 }
 
 // B[m][slot][j] = <C_slot, C_j> over the packed upper triangle = 0.5 * full Frobenius product
-// (C is antisymmetric), j = 0..count-1 (scf_loop.py:1003-1013).  Block per (j, molecule).
-__global__ void k_diis_dots(pyseqm_layout_t L, const int* __restrict__ idx, const double* __restrict__ ring,
-                            size_t ring_stride, int slot, int nfock, double* __restrict__ Bmat) {
+// (C is antisymmetric), j = 0..count-1 (scf_loop.py:1003-1013).  Each block reads its chunk of
+// C_slot once and forms all `count` products; per-block partials, then a fixed-order final sum.
+__global__ void k_diis_dots(pyseqm_layout_t L, const int* __restrict__ idx, int nb, const double* __restrict__ ring,
+                            size_t ring_stride, int slot, int count, double* __restrict__ red) {
   __shared__ double sh[kEl / 64];
-  const int j = blockIdx.x;
-  const int m = idx[blockIdx.y];
-  const int n = L.mol_norb[m];
+  const int k = blockIdx.y;
+  const int m = idx[k];
+  const size_t n2 = (size_t)L.mol_norb[m] * L.mol_norb[m];
   const size_t off = L.mol_mat_off[m];
   const double* a = ring + (size_t)slot * ring_stride + off;
-  const double* b = ring + (size_t)j * ring_stride + off;
+  double acc[16];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) acc[j] = 0.0;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n2; e += (size_t)nb * blockDim.x) {
+    const double av = a[e];
+#pragma unroll
+    for (int j = 0; j < 16; ++j)
+      if (j < count) acc[j] += av * ring[(size_t)j * ring_stride + off + e];
+  }
+  for (int j = 0; j < count; ++j) {
+    const double r = block_sum(acc[j], sh);
+    if (threadIdx.x == 0) red[((size_t)k * nb + blockIdx.x) * 16 + j] = r;
+  }
+}
+
+__global__ void k_diis_dots_final(const int* __restrict__ idx, int nact, int nb, int slot, int count, int nfock,
+                                  const double* __restrict__ red, double* __restrict__ Bmat) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nact * count) return;
+  const int k = t / count, j = t % count;
   double s = 0.0;
-  for (size_t e = threadIdx.x; e < (size_t)n * n; e += blockDim.x) s += a[e] * b[e];
-  const double r = block_sum(s, sh);
-  if (threadIdx.x == 0) Bmat[((size_t)m * (nfock + 1) + slot) * (nfock + 1) + j] = 0.5 * r;
+  for (int b = 0; b < nb; ++b) s += red[((size_t)k * nb + b) * 16 + j];
+  Bmat[((size_t)idx[k] * (nfock + 1) + slot) * (nfock + 1) + j] = 0.5 * s;
 }
 
 // F = sum_j c[m][j] F_j over the stored Fock matrices (scf_loop.py:1033-1035).
@@ -288,14 +293,6 @@ inline int status() { return hipGetLastError() == hipSuccess ? PYSEQM_OK : PYSEQ
 
 }  // namespace
 
-extern "C" int pyseqm_pad_for_eigh(const pyseqm_layout_t* L, const double* F, const int* idx, int nact, int nmax,
-                                   double* out, void* stream) {
-  if (!L || nact < 0 || nmax <= 0) return PYSEQM_EINVAL;
-  if (nact == 0) return PYSEQM_OK;
-  k_pad_for_eigh<<<nact, kEl, 0, (hipStream_t)stream>>>(*L, F, idx, nmax, out);
-  return status();
-}
-
 extern "C" int pyseqm_density_from_evecs(const pyseqm_layout_t* L, const double* V, int nmax, const int* idx,
                                          int nact, double* P, void* stream) {
   if (!L || nact < 0 || nmax <= 0) return PYSEQM_EINVAL;
@@ -324,11 +321,13 @@ extern "C" int pyseqm_copy_active(const pyseqm_layout_t* L, const int* idx, int 
 extern "C" int pyseqm_scf_check(const pyseqm_layout_t* L, const int* idx, int nact, const double* P,
                                 const double* Pold, const double* e_new, double* e_old, const double* diis_err,
                                 double eps, double diis_tol, unsigned char* notconv, double* metrics,
-                                void* stream) {
-  if (!L || nact < 0) return PYSEQM_EINVAL;
+                                double* red, int nb, void* stream) {
+  if (!L || nact < 0 || nb < 1 || nb > PYSEQM_MAX_RED_BLOCKS) return PYSEQM_EINVAL;
   if (nact == 0) return PYSEQM_OK;
-  k_scf_check<<<nact, kEl, 0, (hipStream_t)stream>>>(*L, idx, P, Pold, e_new, e_old, diis_err, eps, diis_tol,
-                                                     notconv, metrics);
+  hipStream_t s = (hipStream_t)stream;
+  k_check_partial<<<dim3(nb, nact), kEl, 0, s>>>(*L, idx, nb, P, Pold, red);
+  k_check_final<<<ceil_div(nact, 64), 64, 0, s>>>(*L, idx, nact, nb, red, e_new, e_old, diis_err, eps, diis_tol,
+                                                  notconv, metrics);
   return status();
 }
 
@@ -347,19 +346,28 @@ extern "C" int pyseqm_dgemm_sym(const pyseqm_layout_t* L, const int* idx, int na
 
 // X = F P on the FP64 MFMA GEMM (P symmetric), then C = X - X^T and max|C|.
 extern "C" int pyseqm_diis_residual(const pyseqm_layout_t* L, const int* idx, int nact, int nmax, const double* F,
-                                    const double* P, double* X, double* Cout, double* diis_err, void* stream) {
+                                    const double* P, double* X, double* Cout, double* diis_err, double* red, int nb,
+                                    void* stream) {
+  if (nb < 1 || nb > PYSEQM_MAX_RED_BLOCKS) return PYSEQM_EINVAL;
   int rc = pyseqm_dgemm_sym(L, idx, nact, nmax, L->mol_mat_off, nullptr, F, P, X, 0, stream);
   if (rc != PYSEQM_OK || nact == 0) return rc;
-  k_antisym<<<nact, kEl, 0, (hipStream_t)stream>>>(*L, idx, X, Cout, diis_err);
+  hipStream_t s = (hipStream_t)stream;
+  k_antisym<<<dim3(nb, nact), kEl, 0, s>>>(*L, idx, nb, X, Cout, red);
+  k_antisym_final<<<ceil_div(nact, 64), 64, 0, s>>>(idx, nact, nb, red, diis_err);
   return status();
 }
 
 extern "C" int pyseqm_diis_dots(const pyseqm_layout_t* L, const int* idx, int nact, const double* ring,
-                                long long ring_stride, int slot, int count, int nfock, double* Bmat, void* stream) {
-  if (!L || nact < 0 || count <= 0 || count > nfock || slot < 0 || slot >= nfock) return PYSEQM_EINVAL;
+                                long long ring_stride, int slot, int count, int nfock, double* Bmat, double* red,
+                                int nb, void* stream) {
+  if (!L || nact < 0 || count <= 0 || count > nfock || nfock > 16 || slot < 0 || slot >= nfock || nb < 1 ||
+      nb > PYSEQM_MAX_RED_BLOCKS)
+    return PYSEQM_EINVAL;
   if (nact == 0) return PYSEQM_OK;
-  k_diis_dots<<<dim3(count, nact), kEl, 0, (hipStream_t)stream>>>(*L, idx, ring, (size_t)ring_stride, slot, nfock,
-                                                                   Bmat);
+  hipStream_t s = (hipStream_t)stream;
+  k_diis_dots<<<dim3(nb, nact), kEl, 0, s>>>(*L, idx, nb, ring, (size_t)ring_stride, slot, count, red);
+  k_diis_dots_final<<<ceil_div((long long)nact * count, 64), 64, 0, s>>>(idx, nact, nb, slot, count, nfock, red,
+                                                                         Bmat);
   return status();
 }
 

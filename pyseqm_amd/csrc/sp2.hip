@@ -36,40 +36,71 @@ __device__ inline double wmax(double v) {
 }
 
 // ------------------------------------------------------------------ init
-// X0 (FP32, padded) and the FP64 trace / Frobenius norm of the rounded X0.  One block per molecule.
-__global__ void k_sp2_init(pyseqm_layout_t L, const double* __restrict__ F, const int* __restrict__ idx,
-                           const long long* __restrict__ xoff, const int* __restrict__ xld, float* __restrict__ X,
-                           double* __restrict__ st) {
-  __shared__ double sh[4][4];
-  const int m = idx[blockIdx.x];
+// Pass 1: Gershgorin bounds hN = max_i (a_ii + r_i), h1 = min_i (a_ii - r_i) (SP2.py:36-43); one
+// wave per row, nb blocks per molecule, per-block max/min written to red[(k*nb+b)*4 + 0/1].
+__global__ void k_gersh(pyseqm_layout_t L, const double* __restrict__ F, const int* __restrict__ idx, int nb,
+                        double* __restrict__ red) {
+  __shared__ double sh[2][4];
+  const int k = blockIdx.y;
+  const int m = idx[k];
   const int n = L.mol_norb[m];
-  const int ld = xld[m];
   const double* Fm = F + L.mol_mat_off[m];
-  float* Xm = X + xoff[m];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   double hi = -1.0e300, lo = 1.0e300;
-  for (int r = threadIdx.x; r < n; r += blockDim.x) {
-    double s = 0.0;
-    for (int c = 0; c < n; ++c) s += fabs(Fm[(size_t)r * n + c]);
+  for (int r = blockIdx.x * 4 + w; r < n; r += nb * 4) {
+    double sabs = 0.0;
+    for (int c = lane; c < n; c += 64) sabs += fabs(Fm[(size_t)r * n + c]);
+    sabs = wsum(sabs);
     const double d = Fm[(size_t)r * n + r];
-    hi = fmax(hi, d + (s - fabs(d)));
-    lo = fmin(lo, d - (s - fabs(d)));
+    hi = fmax(hi, d + (sabs - fabs(d)));
+    lo = fmin(lo, d - (sabs - fabs(d)));
   }
-  hi = wmax(hi);
-  lo = -wmax(-lo);
   if (lane == 0) {
     sh[0][w] = hi;
     sh[1][w] = lo;
   }
   __syncthreads();
-  double hN = sh[0][0], h1 = sh[1][0];
-  for (int k = 1; k < (int)(blockDim.x >> 6); ++k) {
-    hN = fmax(hN, sh[0][k]);
-    h1 = fmin(h1, sh[1][k]);
+  if (threadIdx.x == 0) {
+    double a = sh[0][0], b = sh[1][0];
+    for (int q = 1; q < 4; ++q) {
+      a = fmax(a, sh[0][q]);
+      b = fmin(b, sh[1][q]);
+    }
+    red[((size_t)k * nb + blockIdx.x) * 4] = a;
+    red[((size_t)k * nb + blockIdx.x) * 4 + 1] = b;
   }
-  const double inv = 1.0 / (hN - h1);
+}
+
+__device__ inline void gersh_bounds(const double* red, int k, int nb, double* sh2) {
+  if (threadIdx.x == 0) {
+    double a = -1.0e300, b = 1.0e300;
+    for (int q = 0; q < nb; ++q) {
+      a = fmax(a, red[((size_t)k * nb + q) * 4]);
+      b = fmin(b, red[((size_t)k * nb + q) * 4 + 1]);
+    }
+    sh2[0] = a;
+    sh2[1] = b;
+  }
+  __syncthreads();
+}
+
+// Pass 2: X0 = (hN I - F) / (hN - h1) in FP32 (padded block), per-block FP64 trace / Frobenius
+// partials of the rounded X0 in red[(k*nb+b)*4 + 2/3].
+__global__ void k_sp2_x0(pyseqm_layout_t L, const double* __restrict__ F, const int* __restrict__ idx, int nb,
+                         const long long* __restrict__ xoff, const int* __restrict__ xld, float* __restrict__ X,
+                         double* __restrict__ red) {
+  __shared__ double sh2[2];
+  __shared__ double sh[2][4];
+  const int k = blockIdx.y;
+  const int m = idx[k];
+  const int n = L.mol_norb[m];
+  const int ld = xld[m];
+  const double* Fm = F + L.mol_mat_off[m];
+  float* Xm = X + xoff[m];
+  gersh_bounds(red, k, nb, sh2);
+  const double hN = sh2[0], inv = 1.0 / (sh2[0] - sh2[1]);
   double tr = 0.0, fro = 0.0;
-  for (size_t e = threadIdx.x; e < (size_t)ld * ld; e += blockDim.x) {
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < (size_t)ld * ld; e += (size_t)nb * blockDim.x) {
     const int r = (int)(e / ld), c = (int)(e % ld);
     float v = 0.0f;
     if (r < n && c < n) v = (float)(((r == c ? hN : 0.0) - Fm[(size_t)r * n + c]) * inv);
@@ -79,26 +110,60 @@ __global__ void k_sp2_init(pyseqm_layout_t L, const double* __restrict__ F, cons
   }
   tr = wsum(tr);
   fro = wsum(fro);
-  __syncthreads();
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   if (lane == 0) {
-    sh[2][w] = tr;
-    sh[3][w] = fro;
+    sh[0][w] = tr;
+    sh[1][w] = fro;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    double t = 0.0, f = 0.0;
-    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) {
-      t += sh[2][k];
-      f += sh[3][k];
-    }
-    double* s = st + (size_t)m * S_NST;
-    const double N = L.mol_nocc[m];
-    s[S_TR] = t;
-    s[S_FRO] = f;
-    s[S_E0] = s[S_E1] = s[S_E2] = fabs(t - N);
-    s[S_BRANCH] = fabs(f - N) < fabs(2.0 * t - f - N) ? 1.0 : 0.0;
-    s[S_DONE] = 0.0;
-    s[S_IT] = 0.0;
+    red[((size_t)k * nb + blockIdx.x) * 4 + 2] = sh[0][0] + sh[0][1] + sh[0][2] + sh[0][3];
+    red[((size_t)k * nb + blockIdx.x) * 4 + 3] = sh[1][0] + sh[1][1] + sh[1][2] + sh[1][3];
+  }
+}
+
+// Pass 3: per-molecule SP2 state (fixed-order sums of the partials).
+__global__ void k_sp2_state0(const int* __restrict__ idx, int nact, int nb, const int* __restrict__ nocc,
+                             const double* __restrict__ red, double* __restrict__ st) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nact) return;
+  const int m = idx[k];
+  double t = 0.0, f = 0.0;
+  for (int q = 0; q < nb; ++q) {
+    t += red[((size_t)k * nb + q) * 4 + 2];
+    f += red[((size_t)k * nb + q) * 4 + 3];
+  }
+  double* s = st + (size_t)m * S_NST;
+  const double N = nocc[m];
+  s[S_TR] = t;
+  s[S_FRO] = f;
+  s[S_E0] = s[S_E1] = s[S_E2] = fabs(t - N);
+  s[S_BRANCH] = fabs(f - N) < fabs(2.0 * t - f - N) ? 1.0 : 0.0;
+  s[S_DONE] = 0.0;
+  s[S_IT] = 0.0;
+}
+
+// Zero-padded (nmax x nmax) copy of F with the padded diagonal above the Gershgorin range
+// (hN + (1 + 0.005 t) dE, diag.py:172-192) for a batched eigensolver.
+__global__ void k_pad_fill(pyseqm_layout_t L, const double* __restrict__ F, const int* __restrict__ idx, int nb,
+                           int nmax, const double* __restrict__ red, double* __restrict__ out) {
+  __shared__ double sh2[2];
+  const int k = blockIdx.y;
+  const int m = idx[k];
+  const int n = L.mol_norb[m];
+  const double* Fm = F + L.mol_mat_off[m];
+  gersh_bounds(red, k, nb, sh2);
+  const double top = sh2[0], dE = sh2[0] - sh2[1];
+  double* O = out + (size_t)k * nmax * nmax;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < (size_t)nmax * nmax;
+       e += (size_t)nb * blockDim.x) {
+    const int r = (int)(e / nmax), c = (int)(e % nmax);
+    double v = 0.0;
+    if (r < n && c < n)
+      v = Fm[(size_t)r * n + c];
+    else if (r == c)
+      v = top + (1.0 + 0.005 * (r - n + 1)) * dE;
+    O[e] = v;
   }
 }
 
@@ -407,10 +472,24 @@ inline int eblocks(long long n2) {
 }  // namespace
 
 extern "C" int pyseqm_sp2_init(const pyseqm_layout_t* L, const double* F, const int* idx, int nact,
-                               const long long* xoff, const int* xld, float* X, double* state, void* stream) {
-  if (!L || nact < 0) return PYSEQM_EINVAL;
+                               const long long* xoff, const int* xld, float* X, double* state, double* red, int nb,
+                               void* stream) {
+  if (!L || nact < 0 || nb < 1 || nb > PYSEQM_MAX_RED_BLOCKS) return PYSEQM_EINVAL;
   if (nact == 0) return PYSEQM_OK;
-  k_sp2_init<<<nact, 256, 0, (hipStream_t)stream>>>(*L, F, idx, xoff, xld, X, state);
+  hipStream_t s = (hipStream_t)stream;
+  k_gersh<<<dim3(nb, nact), 256, 0, s>>>(*L, F, idx, nb, red);
+  k_sp2_x0<<<dim3(nb, nact), 256, 0, s>>>(*L, F, idx, nb, xoff, xld, X, red);
+  k_sp2_state0<<<(nact + 63) / 64, 64, 0, s>>>(idx, nact, nb, L->mol_nocc, red, state);
+  return st_ok();
+}
+
+extern "C" int pyseqm_pad_for_eigh(const pyseqm_layout_t* L, const double* F, const int* idx, int nact, int nmax,
+                                   double* out, double* red, int nb, void* stream) {
+  if (!L || nact < 0 || nmax <= 0 || nb < 1 || nb > PYSEQM_MAX_RED_BLOCKS) return PYSEQM_EINVAL;
+  if (nact == 0) return PYSEQM_OK;
+  hipStream_t s = (hipStream_t)stream;
+  k_gersh<<<dim3(nb, nact), 256, 0, s>>>(*L, F, idx, nb, red);
+  k_pad_fill<<<dim3(nb, nact), 256, 0, s>>>(*L, F, idx, nb, nmax, red, out);
   return st_ok();
 }
 

@@ -49,6 +49,7 @@ class SCFEngine:
         self.tile_v = torch.empty(max(nt, 1), **f64)
         self.atom_v = torch.empty(max(b.natoms, 1), **f64)
         self.enuc = torch.zeros(b.nmol, **f64)
+        self.red = torch.zeros(max(b.nmol, 1) * nat.MAX_RED_BLOCKS * 16, **f64)  # reduction scratch
         self.eelec = torch.zeros(b.nmol, **f64)
         # isolated-atom energies and atomic heats of formation (geometry independent)
         Z = b.Z
@@ -107,7 +108,8 @@ class SCFEngine:
         b = self.b
         nmax = int(b.norb[self._idx_host].max())
         Fp = torch.empty(nact, nmax, nmax, dtype=torch.float64, device=self.device)
-        nat.call("pyseqm_pad_for_eigh", b.L, nat.ptr(F), nat.ptr(idx), nact, nmax, nat.ptr(Fp), self.stream)
+        nat.call("pyseqm_pad_for_eigh", b.L, nat.ptr(F), nat.ptr(idx), nact, nmax, nat.ptr(Fp), nat.ptr(self.red),
+                 nat.red_blocks(nmax), self.stream)
         e, V = torch.linalg.eigh(Fp)
         V = V.contiguous()
         nat.call("pyseqm_density_from_evecs", b.L, nat.ptr(V), nmax, nat.ptr(idx), nact, nat.ptr(Pout),
@@ -187,7 +189,8 @@ class SCFEngine:
             act = self._act_mask(nc_host)
             self.fock(P, F, active=act, eelec=e_new)
             nat.call("pyseqm_scf_check", b.L, nat.ptr(idx), nact, nat.ptr(P), nat.ptr(Pold), nat.ptr(e_new),
-                     nat.ptr(e_old), None, float(eps), 0.0, nat.ptr(notconv), None, self.stream)
+                     nat.ptr(e_old), None, float(eps), 0.0, nat.ptr(notconv), None, nat.ptr(self.red),
+                     nat.red_blocks(nmax), self.stream)
             nc_host = notconv.cpu().numpy().astype(bool)
         return P, it, nc_host
 
@@ -221,10 +224,11 @@ class SCFEngine:
             counter = (counter + 1) % NFOCK
             Fslot = Fring[counter]
             nat.call("pyseqm_copy_active", b.L, nat.ptr(idx), nact, nmax, nat.ptr(F), nat.ptr(Fslot), self.stream)
+            nb = nat.red_blocks(nmax)
             nat.call("pyseqm_diis_residual", b.L, nat.ptr(idx), nact, nmax, nat.ptr(F), nat.ptr(P), nat.ptr(X),
-                     nat.ptr(Cring[counter]), nat.ptr(diis_err), self.stream)
+                     nat.ptr(Cring[counter]), nat.ptr(diis_err), nat.ptr(self.red), nb, self.stream)
             nat.call("pyseqm_diis_dots", b.L, nat.ptr(idx), nact, nat.ptr(Cring), total, counter, cF, NFOCK,
-                     nat.ptr(EMAT), self.stream)
+                     nat.ptr(EMAT), nat.ptr(self.red), nb, self.stream)
             reset = False
             if cF >= 2:
                 il = idx.long()
@@ -252,8 +256,7 @@ class SCFEngine:
             self.fock(P, F, active=act, eelec=e_new)
             nat.call("pyseqm_scf_check", b.L, nat.ptr(idx), nact, nat.ptr(P), nat.ptr(Pold), nat.ptr(e_new),
                      nat.ptr(e_old), nat.ptr(diis_err), float(eps), self.diis_tol, nat.ptr(notconv),
-                     nat.ptr(self._metrics),
-                     self.stream)
+                     nat.ptr(self._metrics), nat.ptr(self.red), nb, self.stream)
             nc_host = notconv.cpu().numpy().astype(bool)
             if self.trace is not None:
                 self.trace.append(self._metrics.cpu().numpy().copy())
@@ -272,6 +275,15 @@ class SCFEngine:
         hf = etot - self.eiso + self.eheat
         return dict(F=F, Eelec=eelec, Enuc=self.enuc.clone(), Etot=etot, Eiso=self.eiso.clone(), Hf=hf)
 
+    def forces(self, P, h=1.0e-3):
+        """Forces (natoms, 3) in eV/Angstrom at fixed density P (Hellmann-Feynman, the reference's
+        autograd force with scf_backward = 0, basics.py:1315-1336)."""
+        b = self.b
+        out = torch.zeros(max(b.natoms, 1), 3, dtype=torch.float64, device=self.device)
+        nat.call("pyseqm_pair_forces", b.L, nat.METHODS[self.method], nat.ptr(self.coords), nat.ptr(self.atom_par),
+                 nat.ptr(P), float(h), nat.ptr(self.part), nat.ptr(out), self.stream)
+        return out[:b.natoms]
+
     def eig(self, F):
         """Orbital energies in the reference layout e (nmol, 4*molsize), zero padded, and the packed
         eigenvectors (nmol, nmax, nmax), sym_eig_trunc(eig_only=True) (diag.py:110-241)."""
@@ -280,7 +292,8 @@ class SCFEngine:
         idx = torch.arange(b.nmol, dtype=torch.int32, device=self.device)
         nmax = b.nmax
         Fp = torch.empty(b.nmol, nmax, nmax, dtype=torch.float64, device=self.device)
-        nat.call("pyseqm_pad_for_eigh", b.L, nat.ptr(F), nat.ptr(idx), b.nmol, nmax, nat.ptr(Fp), self.stream)
+        nat.call("pyseqm_pad_for_eigh", b.L, nat.ptr(F), nat.ptr(idx), b.nmol, nmax, nat.ptr(Fp),
+                 nat.ptr(self.red), nat.red_blocks(nmax), self.stream)
         e0, V = torch.linalg.eigh(Fp)
         e = torch.zeros(b.nmol, 4 * b.molsize, dtype=torch.float64, device=self.device)
         mask = torch.arange(nmax, device=self.device)[None, :] < torch.as_tensor(b.norb, device=self.device)[:, None]

@@ -227,6 +227,14 @@ class PackedBatch:
         out[flat] = Xp[mm, rr, cc]
         return out
 
+    def atoms_from_padded(self, v):
+        """(nmol, molsize, ...) reference layout -> per-atom (natoms, ...); (natoms, ...) passes through."""
+        v = torch.as_tensor(v)
+        if v.dim() >= 2 and tuple(v.shape[:2]) == (self.nmol, self.molsize) and not (
+                v.dim() == 2 and v.shape[0] == self.natoms):
+            return v.reshape((-1,) + tuple(v.shape[2:]))[torch.as_tensor(self.real_flat, device=v.device)]
+        return v
+
     def atom_to_padded(self, v, fill=0.0):
         """Per-atom (natoms, ...) -> (nmol, molsize, ...) in the reference's padded atom layout."""
         shape = (self.nmol * self.molsize,) + tuple(v.shape[1:])

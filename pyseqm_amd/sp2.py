@@ -49,7 +49,7 @@ def density_sp2(eng, F, idx, nact, Pout, eps=SP2_EPS, max_iter=SP2_MAX_ITER):
     s = eng.stream
     L = b.L
     nat.call("pyseqm_sp2_init", L, nat.ptr(F), nat.ptr(idx), nact, nat.ptr(ws.xoff), nat.ptr(ws.xld),
-             nat.ptr(ws.Xa), nat.ptr(ws.state), s)
+             nat.ptr(ws.Xa), nat.ptr(ws.state), nat.ptr(eng.red), nat.red_blocks(nmax), s)
     ws.ndone.zero_()
     bufs = (ws.Xa, ws.Xb)
     if ws.timing is not None:

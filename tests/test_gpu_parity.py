@@ -153,3 +153,30 @@ def test_molecule_alone_equals_in_batch():
         e1 = SCFEngine(sp[m:m + 1], c[m:m + 1], "AM1", DEV)
         h1 = e1.energies(e1.scf(eps=1e-8).P)["Hf"].cpu().numpy()
         assert abs(h1[0] - hf[m]) < 1e-7
+
+
+@pytest.mark.parametrize("files,method,eps,gname", [
+    (["methane.xyz"], "MNDO", 1e-6, "smoke_single_point_MNDO"),
+    (["methane.xyz"], "AM1", 1e-6, "smoke_single_point_AM1"),
+    (["methane.xyz"], "PM3", 1e-6, "smoke_single_point_PM3"),
+    (["methanal.1.xyz", "methanal.2.xyz", "methanal.3.xyz"], "AM1", 1e-7, "ground_force_batch_methanal"),
+    (["methane.xyz", "benzene.xyz"], "AM1", 1e-7, "ground_force_batch_mixed"),
+])
+def test_forces_match_goldens(files, method, eps, gname):
+    sp, c = read_xyz(files)
+    eng = SCFEngine(sp, c, method, DEV)
+    r = eng.scf(eps=eps, converger=(1,))
+    f = eng.b.atom_to_padded(eng.forces(r.P)).cpu().numpy()
+    g = np.array(golden(gname)["force"]).reshape(f.shape)
+    np.testing.assert_allclose(f, g, atol=1e-6, rtol=0)
+
+
+@pytest.mark.parametrize("name,method", [("sulfur", "PM3"), ("water8", "MNDO")])
+def test_forces_match_oracle(name, method):
+    sp, c = _inputs(name)
+    eng = SCFEngine(sp, c, method, DEV)
+    r = eng.scf(eps=1e-8, converger=(2,))
+    f = eng.b.atom_to_padded(eng.forces(r.P)).cpu().numpy()
+    Po = eng.b.to_padded(r.P).cpu().numpy()
+    fo = R.forces_fd(sp, c, method, Po)
+    np.testing.assert_allclose(f, fo, atol=1e-6, rtol=0)

@@ -248,3 +248,42 @@ def test_pulay_and_adaptive_reach_same_fixed_point():
     np.testing.assert_allclose(ra["Etot"], rp["Etot"], atol=1e-7, rtol=0)
     np.testing.assert_allclose(rl["Etot"], rp["Etot"], atol=1e-7, rtol=0)
     assert rp["iterations"] < ra["iterations"]
+
+
+# ---------------------------------------------------------------- XL-BOMD
+def test_xl_energy_is_shadow_energy():
+    """E_xl(D, P) = E(D) - 1/2 B(D - P, D - P) with E(X) = 1/2 sum X (H + F(X)) and
+    B(X, X) = sum X G(X), G = F - H linear; D = P gives the SCF energy."""
+    sp, c = read_xyz(["methanal.1.xyz", "benzene.xyz"])
+    b = R.parse(sp, c, "AM1")
+    H, T, _, _, _ = R.hcore(b)
+    P = R.initial_density(b)
+    rng = np.random.default_rng(3)
+    dP = rng.normal(scale=0.05, size=P.shape)
+    dP = dP + dP.transpose(0, 2, 1)
+    mask = (np.abs(P).sum(axis=2, keepdims=True) > 0) & (np.abs(P).sum(axis=1, keepdims=True) > 0)
+    D = P + dP * mask
+    F = R.fock(b, P, H, T)
+    Ex = R.elec_energy_xl(D, P, F, H)
+    FD = R.fock(b, D, H, T)
+    X = D - P
+    GX = R.fock(b, X, H, T) - H
+    np.testing.assert_allclose(Ex, R.elec_energy(D, FD, H) - 0.5 * np.einsum("nij,nij->n", X, GX), atol=1e-9)
+    np.testing.assert_allclose(R.elec_energy_xl(P, P, F, H), R.elec_energy(P, F, H), atol=1e-10)
+
+
+@pytest.mark.parametrize("k", [3, 4, 5, 6, 7, 8, 9])
+def test_xl_propagation_extrapolates_linear_sequences(k):
+    """The XL coefficient sets satisfy sum c_i = 0 and sum i c_i = 0, so a density history that is
+    linear in time, with D(n) = P(n), is extrapolated exactly, over every ring position."""
+    rng = np.random.default_rng(k)
+    X, Y = rng.normal(size=(2, 5, 5))
+    m = k + 1
+    Pt = np.zeros((m, 5, 5))
+    # seed the ring as after m steps of the linear sequence P(n) = X + n Y, n = 0..m-1
+    for n in range(m):
+        Pt[m - 1 - (n % m)] = X + n * Y
+    P = X + (m - 1) * Y
+    for step in range(m, 3 * m):
+        P, Pt = R.xl_propagate(P, P, Pt, k, step)
+        np.testing.assert_allclose(P, X + step * Y, atol=1e-9 * (1 + step))
