@@ -9,9 +9,16 @@ One "step" = one converged single point of the whole batch from coordinates alre
 device: pair integrals + core Hamiltonian, the SCF loop to convergence, the final Fock build and
 energies (Hf, Etot, Eelec, Enuc).  value = molecules converged per second.
 
-Also reported: the Fock-build roofline (algorithmic HBM bytes per launch / average launch time
-measured with HIP events on the launch stream) and the oracle's FP64 CPU time on a bounded
-sample of the same workload.
+Also reported, in the same JSON line:
+  roofline               the Fock build (algorithmic HBM bytes per launch / average launch time,
+                         HIP events on the launch stream);
+  sp2_mixed_same_inputs  the mixed-precision SP2 density step on the same batch: throughput,
+                         speedup over the FP64 diagonalisation path and |dHf| per atom;
+  roofline_sp2           the SP2 X^2 GEMMs on (H2O)_256 x 8 (MFMA-bound);
+  serving, p50_ms/p99_ms an open-loop Poisson SCF request stream (half AM1 alkanes, half PM3 water
+                         clusters) through the continuous-batching server beside 256 graph-stepped
+                         XL-BOMD trajectories: arrival->result latency percentiles, SCF/s;
+  cpu_baseline           the oracle's FP64 CPU time on a bounded sample of the same workload.
 """
 import argparse
 import json
@@ -44,18 +51,131 @@ def make_workload(cfg, seed):
     return c["method"], sp, xyz
 
 
-def cpu_baseline(method, sp, xyz, nsample, eps, conv):
+def cpu_baseline(method, sp, xyz, nsample, eps, conv, target_s=15.0):
+    """The oracle (FP64 numpy restatement of the reference algorithm) on the host, one thread,
+    on the first molecules of the same batch; the sample grows until it takes ~target_s."""
     from threadpoolctl import threadpool_limits
 
     from oracle import seqm_ref as R
+    done, dt, conv_ok = 0, 0.0, True
     with threadpool_limits(limits=1):
-        t0 = time.perf_counter()
-        r = R.single_point(sp[:nsample], xyz[:nsample], method, eps=eps, converger=conv)
-        dt = time.perf_counter() - t0
-    return dict(value=nsample / dt, unit="molecules/s", cores=1, kind="port",
-                sample=f"first {nsample} molecules of the same batch, oracle FP64 numpy single point "
-                       f"(same method/eps/mixer), 1 thread, {dt:.2f} s",
-                converged=bool(not r["notconverged"].any()))
+        n = nsample
+        while done < len(sp) and dt < target_s:
+            n = min(n, len(sp) - done)
+            t0 = time.perf_counter()
+            r = R.single_point(sp[done:done + n], xyz[done:done + n], method, eps=eps, converger=conv)
+            dt += time.perf_counter() - t0
+            conv_ok &= bool(not r["notconverged"].any())
+            done += n
+    return dict(value=done / dt, unit="molecules/s", cores=1, kind="port",
+                sample=f"first {done} molecules of the same batch, oracle FP64 numpy single point "
+                       f"(same method/eps/mixer, FP64 eigh), 1 thread, {dt:.1f} s",
+                converged=conv_ok)
+
+
+def serve(args, dev):
+    """Open-loop Poisson SCF stream (half AM1 alkanes, half PM3 water clusters) through the
+    continuous-batching server, beside 256 graph-stepped XL-BOMD trajectories."""
+    import threading
+
+    import torch
+
+    from pyseqm_amd import workloads as W
+    from pyseqm_amd.serving import Request, SCFServer, XLService, percentiles
+    stream = W.serving_stream(rate=args.serve_rate, window=args.serve_window, seed=args.seed,
+                              water_max=args.serve_water_max)
+    xl = None
+    if args.xl_trajectories > 0:
+        sp, xyz = W.alkanes(args.xl_trajectories, seed=args.seed + 7)
+        xl = XLService(sp, xyz, "AM1", dev, k=6, dt=0.5)
+        for _ in range(3):
+            xl.step()
+        xl.latencies.clear()
+    srv = SCFServer(dev, lanes=args.serve_lanes, density="sp2", eps=args.serve_eps)
+    stop = threading.Event()
+    th = None
+    if xl is not None:
+        th = threading.Thread(target=xl.run_until, args=(stop,), daemon=True)
+        th.start()
+    reqs = []
+    t0 = time.perf_counter()
+    for k, (ta, sp_, x_, method) in enumerate(stream):
+        now = time.perf_counter() - t0
+        if now < ta:
+            time.sleep(ta - now)
+        reqs.append(srv.submit(Request(k, sp_, x_, method, t_arrival=t0 + ta)))
+    t_submit_end = time.perf_counter()
+    deadline = t_submit_end + args.serve_drain
+    for r in reqs:
+        r.event.wait(timeout=max(0.0, deadline - time.perf_counter()))
+    t_end = time.perf_counter()
+    stop.set()
+    if th is not None:
+        th.join()
+    done = [r for r in reqs if r.t_done is not None]
+    lat = [r.latency * 1e3 for r in done]
+    pending = len(reqs) - len(done)
+    # a request still pending at the end counts with its elapsed time (a lower bound)
+    lat_all = lat + [(t_end - r.t_arrival) * 1e3 for r in reqs if r.t_done is None]
+    with srv.cv:
+        srv.queue.clear()
+    srv.close()
+    out = {
+        "offered_rate": args.serve_rate, "window_s": args.serve_window, "requests": len(reqs),
+        "completed": len(done), "pending_at_end": pending,
+        "converged": int(sum(1 for r in done if r.result and r.result["converged"])),
+        "scf_per_s": len(done) / (t_end - t0), "wall_s": t_end - t0,
+        "latency_ms": percentiles(lat_all), "latency_ms_completed": percentiles(lat),
+        "batches": srv.batches, "lanes": args.serve_lanes, "hbm_budget_GiB": srv.hbm_budget / 2**30,
+        "peak_in_flight_GiB": srv.peak_in_use / 2**30, "density": "sp2 (bf16x3 + f64 McWeeny)",
+        "eps_eV": args.serve_eps, "mix": "50% AM1 alkanes n~U[2,30], 50% PM3 (H2O)_N N~U[4,%d]" % args.serve_water_max,
+    }
+    if xl is not None:
+        n = len(xl.latencies)
+        out["xl_bomd"] = {"trajectories": int(xl.traj.eng.b.nmol), "steps": n, "k": 6, "dt_fs": 0.5,
+                          "step_ms": percentiles(np.array(xl.latencies) * 1e3),
+                          "trajectory_steps_per_s": n * xl.traj.eng.b.nmol / (t_end - t0),
+                          "graph_captured": bool(xl.graphs)}
+    return out
+
+
+def sp2_roofline(eng, step):
+    """Events around the SP2 GEMM loops of one SCF step: 2 n^3 flops per X^2 per active molecule."""
+    import torch
+
+    from pyseqm_amd import sp2 as sp2mod
+    eng._sp2ws.timing = []
+    step()
+    torch.cuda.synchronize()
+    t_ms = sum(e0.elapsed_time(e1) for e0, e1, *_ in eng._sp2ws.timing)
+    flops = sum(sp2mod.gemm_flops(eng, ix, it) for _, _, ix, _, it in eng._sp2ws.timing)
+    eng._sp2ws.timing = None
+    ach = flops / (t_ms * 1e-3) / 1e12
+    peak = 2500.0 / 3.0
+    return {"kernel": "k_sp2_bf16x3 (SP2 X^2, bf16x3 MFMA, FP32 accumulate, fused branch/trace)",
+            "bound": "mfma", "achieved": ach, "peak": peak, "unit": "TFLOP/s", "frac": ach / peak,
+            "traffic": None, "algorithmic_flops": flops, "gemm_loop_ms": t_ms,
+            "note": "peak = dense bf16 2.5 PFLOP/s / 3 products (bf16x3 split); flops = 2 n^3 per SP2 iteration "
+                    "per active molecule; time = SP2 iteration loops incl. the fused trace/branch epilogue"}
+
+
+def fock_roofline(eng, P, reps):
+    import torch
+    F = torch.empty_like(P)
+    s = torch.cuda.current_stream()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    eng.fock(P, F)
+    ev0.record(s)
+    for _ in range(reps):
+        eng.fock(P, F)
+    ev1.record(s)
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1) / reps
+    nbytes, (nxx, nxh, nhh) = fock_algorithmic_bytes(eng.b)
+    ach = nbytes / (ms * 1e-3) / 1e9
+    return {"kernel": "pyseqm_fock (k_fock_tiles<XX/XH/HH> + k_fock_diag, fused Eelec)", "bound": "hbm",
+            "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None,
+            "algorithmic_bytes": nbytes, "launch_ms": ms, "pairs": {"XX": nxx, "XH": nxh, "HH": nhh}}
 
 
 def main():
@@ -66,12 +186,22 @@ def main():
     ap.add_argument("--config", default="C2")
     ap.add_argument("--density", default="eigh", choices=["eigh", "sp2"])
     ap.add_argument("--eps", type=float, default=1e-8)
+    ap.add_argument("--sp2-eps", type=float, default=1e-5,
+                    help="SCF energy tolerance of the mixed-precision comparison run")
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--cpu-sample", type=int, default=16)
+    ap.add_argument("--cpu-sample", type=int, default=8)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--fock-reps", type=int, default=20)
-    ap.add_argument("--compare", action="store_true",
-                    help="also run the FP64 diagonalisation path once: speedup and |dHf|/atom")
+    ap.add_argument("--no-compare", action="store_true", help="skip the SP2-mixed run on the same inputs")
+    ap.add_argument("--sp2-roofline-config", default="C3", help="workload of the SP2 GEMM roofline ('' = skip)")
+    ap.add_argument("--serve-window", type=float, default=10.0, help="seconds of Poisson arrivals (0 = skip)")
+    ap.add_argument("--serve-rate", type=float, default=2000.0)
+    ap.add_argument("--serve-water-max", type=int, default=256)
+    ap.add_argument("--serve-lanes", type=int, default=3)
+    ap.add_argument("--serve-eps", type=float, default=1e-5)
+    ap.add_argument("--serve-drain", type=float, default=20.0)
+    ap.add_argument("--xl-trajectories", type=int, default=256)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (this path is "
@@ -84,20 +214,21 @@ def main():
 
     method, sp, xyz = make_workload(args.config, args.seed)
     conv = (2,)
-    if args.density == "sp2" and args.eps == 1e-8:
-        args.eps = 1e-5  # energy tolerance of the mixed-precision path (its precision floor is ~1e-8 eV/atom)
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     eng = SCFEngine(sp, None, method, dev)
     coords = torch.as_tensor(xyz, dtype=torch.float64, device=dev)
     nmol = eng.b.nmol
+    eps = args.eps if args.density == "eigh" else args.sp2_eps
 
-    def step():
-        eng.set_coordinates(coords)
-        r = eng.scf(eps=args.eps, converger=conv, density=args.density)
-        en = eng.energies(r.P)
-        return r, en
+    def make_step(e, density, tol):
+        def step():
+            e.set_coordinates(coords)
+            r = e.scf(eps=tol, converger=conv, density=density)
+            return r, e.energies(r.P)
+        return step
 
+    step = make_step(eng, args.density, eps)
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -111,37 +242,7 @@ def main():
     dt = time.perf_counter() - t0
     ms = dt / args.steps * 1e3
     value = nconv / dt
-
-    # SP2 GEMM roofline (one more timed step with events around the SP2 GEMM loops)
-    sp2_roof = None
-    if args.density == "sp2":
-        from pyseqm_amd import sp2 as sp2mod
-        eng._sp2ws.timing = []
-        step()
-        torch.cuda.synchronize()
-        t_ms = sum(e0.elapsed_time(e1) for e0, e1, *_ in eng._sp2ws.timing)
-        flops = sum(sp2mod.gemm_flops(eng, ix, it) for _, _, ix, _, it in eng._sp2ws.timing)
-        eng._sp2ws.timing = None
-        ach = flops / (t_ms * 1e-3) / 1e12
-        peak = 2500.0 / 3.0
-        sp2_roof = {"kernel": "k_sp2_bf16x3 (SP2 X^2, bf16x3 MFMA, FP32 accumulate, fused branch/trace)",
-                    "bound": "mfma", "achieved": ach, "peak": peak, "unit": "TFLOP/s", "frac": ach / peak,
-                    "traffic": None, "algorithmic_flops": flops, "gemm_loop_ms": t_ms,
-                    "note": "peak = dense bf16 2.5 PFLOP/s / 3 products; flops = 2 n^3 per SP2 iteration"}
-    # Fock-build roofline: HIP events on the stream the kernels are launched on
-    P = r.P
-    F = torch.empty_like(P)
-    s = torch.cuda.current_stream()
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    eng.fock(P, F)
-    ev0.record(s)
-    for _ in range(args.fock_reps):
-        eng.fock(P, F)
-    ev1.record(s)
-    torch.cuda.synchronize()
-    fock_ms = ev0.elapsed_time(ev1) / args.fock_reps
-    nbytes, (nxx, nxh, nhh) = fock_algorithmic_bytes(eng.b)
-    achieved = nbytes / (fock_ms * 1e-3) / 1e9
+    hf_main = en["Hf"].clone()
 
     line = {
         "metric": "converged SCF/s", "value": value, "unit": "molecules/s", "n_gpus": 1,
@@ -149,34 +250,53 @@ def main():
         "scaling": "weak", "vs_baseline": None,
         "dtype": "f64" if args.density == "eigh" else "bf16x3 (fp32 accumulate) + f64 McWeeny", "data": "synthetic",
         "config": {"workload": f"{args.config}: {nmol} {method} molecules "
-                               f"({'alkanes n~U[2,30]' if args.config == 'C2' else args.config}), "
-                               f"density step {args.density}, Pulay DIIS, eps {args.eps:g} eV",
+                               f"({'alkanes CnH2n+2 n~U[2,30]' if args.config == 'C2' else args.config}), "
+                               f"density step {args.density}, Pulay DIIS, eps {eps:g} eV",
                    "method": method, "molecules": nmol, "orbitals_total": int(eng.b.norb.sum()),
-                   "norb_max": int(eng.b.nmax), "scf_iterations": iters,
+                   "norb_max": int(eng.b.nmax), "scf_iterations": iters, "seed": args.seed,
                    "parallelism": "single GPU, batch over molecules"},
-        "roofline": {"kernel": "pyseqm_fock (Fock build + fused Eelec)", "bound": "hbm",
-                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "algorithmic_bytes": nbytes, "launch_ms": fock_ms,
-                     "pairs": {"XX": nxx, "XH": nxh, "HH": nhh}},
+        "roofline": fock_roofline(eng, r.P, args.fock_reps),
     }
-    if sp2_roof is not None:
-        line["roofline_fock"] = line["roofline"]
-        line["roofline"] = sp2_roof
-    if args.compare:
+    if not args.no_compare and args.density == "eigh":
         e2 = SCFEngine(sp, None, method, dev)
-        e2.set_coordinates(coords)
+        step2 = make_step(e2, "sp2", args.sp2_eps)
+        step2()
         torch.cuda.synchronize()
-        tc = time.perf_counter()
-        r2 = e2.scf(eps=1e-8, converger=conv, density="eigh")
-        en2 = e2.energies(r2.P)
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            r2, en2 = step2()
         torch.cuda.synchronize()
-        tc = time.perf_counter() - tc
-        dhf = (en["Hf"] - en2["Hf"]).abs().cpu().numpy() / eng.b.nat
-        line["compare_fp64_eigh"] = {"ms_per_step": tc * 1e3, "speedup": tc * 1e3 / ms,
-                                     "max_abs_dHf_per_atom_eV": float(dhf.max()),
-                                     "scf_iterations": r2.iterations}
+        ms2 = (time.perf_counter() - t1) / args.steps * 1e3
+        dhf = (en2["Hf"] - hf_main).abs().cpu().numpy() / eng.b.nat
+        line["sp2_mixed_same_inputs"] = {
+            "value": int((~r2.notconverged).sum()) / (ms2 * 1e-3), "ms_per_step": ms2, "speedup_vs_fp64_eigh": ms / ms2,
+            "max_abs_dHf_per_atom_eV": float(dhf.max()), "mean_abs_dHf_per_atom_eV": float(dhf.mean()),
+            "scf_iterations": r2.iterations, "eps_eV": args.sp2_eps}
+    if args.sp2_roofline_config:
+        m3, sp3, xyz3 = make_workload(args.sp2_roofline_config, args.seed)
+        e3 = SCFEngine(sp3, None, m3, dev)
+        c3 = torch.as_tensor(xyz3, dtype=torch.float64, device=dev)
+
+        def step3():
+            e3.set_coordinates(c3)
+            return e3.scf(eps=args.sp2_eps, converger=conv, density="sp2")
+        step3()
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        r3 = step3()
+        torch.cuda.synchronize()
+        t3 = time.perf_counter() - t3
+        roof = sp2_roofline(e3, step3)
+        roof["workload"] = f"{args.sp2_roofline_config}: {e3.b.nmol} x {int(e3.b.nmax)} orbitals, SCF {t3 * 1e3:.1f} ms, " \
+                           f"{r3.iterations} SCF iterations"
+        line["roofline_sp2"] = roof
+    if args.serve_window > 0:
+        sv = serve(args, dev)
+        line["serving"] = sv
+        line["p50_ms"] = sv["latency_ms"]["p50"]
+        line["p99_ms"] = sv["latency_ms"]["p99"]
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(method, sp, xyz, args.cpu_sample, args.eps, conv)
+        line["cpu_baseline"] = cpu_baseline(method, sp, xyz, args.cpu_sample, args.eps, conv, args.cpu_seconds)
     print(json.dumps(line), flush=True)
     return 0
 

@@ -92,6 +92,12 @@ int pyseqm_pair_integrals(const pyseqm_layout_t* L, int method, const double* co
  * part: tile partial scratch as in pyseqm_pair_integrals. */
 int pyseqm_pair_forces(const pyseqm_layout_t* L, int method, const double* coords, const double* atom_par,
                        const double* P, double h, double* part, double* forces, void* stream);
+/* Core-core repulsion E_nuc(i, j) of an explicit pair list, pair_nuclear_energy
+ * (seqm/seqm_functions/energy.py:91-134): idxi/idxj [npairs] atom rows of atom_par (PAR_Z, PAR_TORE,
+ * PAR_ALPHA and the Gaussian K/L/M slots are read), rij [npairs] in bohr, gam [npairs] = (ss|ss). */
+int pyseqm_core_repulsion_pairs(long long npairs, int method, const int* idxi, const int* idxj, const double* rij,
+                                const double* gam, const double* atom_par, double* enuc_pair, void* stream);
+
 /* XL-BOMD forces: -d/dR of E(D, P) = sum D F(P) - 1/2 sum (F(P) - H) P + E_nuc at fixed D and P
  * (the energy of EnergyXL, energy.py:76-88, differentiated by ForceXL's backward,
  * seqm/dynamics/xlbomd.py:536-551).  D: density from F(P); P: propagated density field. */

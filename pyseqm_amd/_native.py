@@ -66,6 +66,8 @@ _SIGS = {
 _RAW_SIGS = {
     "pyseqm_xl_propagate": [c_longlong, c_void_p, c_void_p, c_void_p, c_longlong, c_int, c_void_p, c_double,
                             c_double, c_int, c_void_p],
+    "pyseqm_core_repulsion_pairs": [c_longlong, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                    c_void_p],
     "pyseqm_md_verlet": [c_int, c_int, c_double, c_double, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                          c_void_p],
 }

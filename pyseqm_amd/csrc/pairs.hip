@@ -235,14 +235,22 @@ __device__ inline void aint7(double x, double a[7]) {
   for (int k = 1; k < 7; ++k) a[k] = a[0] + k * a[k - 1] / x;
 }
 
-__device__ inline void bint7(double x, double b[7]) {
+// Branch of bint7 at argument x: 2 recurrence (|x| > 0.5), 1 truncated series, 0 x ~ 0.
+__device__ inline int bint_branch(double x) {
   const double ax = fabs(x);
-  if (ax > 0.5) {
+  return ax > 0.5 ? 2 : (ax > 1.0e-6 ? 1 : 0);
+}
+
+// forced >= 0 evaluates that branch regardless of x (finite differences of a pair energy keep the
+// branch of the geometry they differentiate, as the reference's autograd does).
+__device__ inline void bint7(double x, double b[7], int forced = -1) {
+  const int br = forced >= 0 ? forced : bint_branch(x);
+  if (br == 2) {
     const double tx = exp(x) / x, tmx = -exp(-x) / x;
     b[0] = tx + tmx;
 #pragma unroll
     for (int k = 1; k < 7; ++k) b[k] = ((k & 1) ? -tx : tx) + tmx + k * b[k - 1] / x;
-  } else if (ax > 1.0e-6) {
+  } else if (br == 1) {
     const double x2 = x * x, x3 = x2 * x, x4 = x2 * x2, x5 = x4 * x, x6 = x4 * x2;
     b[0] = 2.0 + x2 / 3.0 + x4 / 60.0 + x6 / 2520.0;
     b[2] = 2.0 / 3.0 + x2 / 5.0 + x4 / 84.0 + x6 / 3240.0;
@@ -266,17 +274,26 @@ __device__ inline int jcall_of(int qi, int qj) {
   return 0;
 }
 
+// Branches of the B integrals for the four exponent pairs (s_i s_j, p_i s_j, s_i p_j, p_i p_j),
+// 2 bits each.
+__device__ inline int overlap_branches(double r, double zas, double zap, double zbs, double zbp) {
+  return bint_branch(0.5 * r * (zas - zbs)) | (bint_branch(0.5 * r * (zap - zbs)) << 2) |
+         (bint_branch(0.5 * r * (zas - zbp)) << 4) | (bint_branch(0.5 * r * (zap - zbp)) << 6);
+}
+
 // S[0..4] = S111 (s|s), S211 (p_sig(i)|s(j)), S121, S221, S222 in the diatomic frame
 // (diat_overlap_PM6_SP.py:84-345).  za/zb = (zeta_s, zeta_p) of atoms i / j.
+// bsel >= 0: B-integral branches (overlap_branches) to use instead of those of r.
 __device__ void overlap_local(int jc, double r, double zas, double zap, double zbs, double zbp,
-                              bool need_p_i, bool need_p_j, double S[5]) {
+                              bool need_p_i, bool need_p_j, double S[5], int bsel = -1) {
   S[0] = S[1] = S[2] = S[3] = S[4] = 0.0;
   double A[7], B[7];
   const double s3 = 1.7320508075688772;
   const double s10 = 3.1622776601683795, s30 = 5.477225575051661;
   auto AB = [&](double z1, double z2) {
+    const int ci = (z1 == zas ? 0 : 1) | (z2 == zbs ? 0 : 2);
     aint7(0.5 * r * (z1 + z2), A);
-    bint7(0.5 * r * (z1 - z2), B);
+    bint7(0.5 * r * (z1 - z2), B, bsel >= 0 ? (bsel >> (2 * ci)) & 3 : -1);
   };
   if (jc == 2) {
     AB(zas, zbs);
@@ -461,7 +478,7 @@ __device__ inline double hh_integral(const PairGeom& g, const double* pi, const 
 // Off-diagonal core block H_ij[a][b] = S_ij[a][b] (beta_a(i) + beta_b(j)) / 2 for NI x NJ orbitals.
 template <int NI, int NJ>
 __device__ inline void core_block(const PairGeom& g, const double* pi, const double* pj, double Hb[4][4],
-                                  int fmode = 0) {
+                                  int fmode = 0, int bsel = -1) {
   double di[4][4];
 #pragma unroll
   for (int a = 0; a < 4; ++a)
@@ -470,7 +487,7 @@ __device__ inline void core_block(const PairGeom& g, const double* pi, const dou
   if (g.r <= kOverlapCut) {
     const int jc = jcall_of((int)pi[PAR_QN], (int)pj[PAR_QN]);  // validated on the host
     double S[5];
-    overlap_local(jc, g.r, pi[PAR_ZS], pi[PAR_ZP], pj[PAR_ZS], pj[PAR_ZP], NI == 4, NJ == 4, S);
+    overlap_local(jc, g.r, pi[PAR_ZS], pi[PAR_ZP], pj[PAR_ZS], pj[PAR_ZP], NI == 4, NJ == 4, S, bsel);
     double Rp[3][3];
     quat_frame(g.x[0], g.x[1], g.x[2], Rp, fmode);
     di[0][0] = S[0];
@@ -591,7 +608,7 @@ struct PairDensity {
 
 template <int KIND, bool XL>
 __device__ double pair_energy(int method, const PairGeom& g, const double* pi, const double* pj,
-                              const PairDensity& P, const PairDensity& D, int wmode, int smode) {
+                              const PairDensity& P, const PairDensity& D, int wmode, int smode, int bsel) {
   constexpr int NI = KIND == HH ? 1 : 4;
   constexpr int NJ = KIND == XX ? 4 : 1;
   const double ti = pi[PAR_TORE], tj = pj[PAR_TORE];
@@ -642,7 +659,7 @@ __device__ double pair_energy(int method, const PairGeom& g, const double* pi, c
     e = (coul(0, 0) - tj * D.pa[0] - ti * D.pb[0] - xw(0, 0) * P.pab[0][0]) * gam;
   }
   double Hb[4][4];
-  core_block<NI, NJ>(g, pi, pj, Hb, smode);
+  core_block<NI, NJ>(g, pi, pj, Hb, smode, bsel);
 #pragma unroll
   for (int a = 0; a < NI; ++a)
 #pragma unroll
@@ -698,6 +715,7 @@ __global__ __launch_bounds__(64) void k_pair_forces(pyseqm_layout_t L, int metho
     const PairGeom g0 = geom_from_vector(R0[0], R0[1], R0[2]);
     const int wmode = quat_flips(-g0.x[0]) ? 2 : 1;  // frame of -x_ij (integrals)
     const int smode = quat_flips(g0.x[0]) ? 2 : 1;   // frame of +x_ij (overlaps)
+    const int bsel = overlap_branches(g0.r, pi[PAR_ZS], pi[PAR_ZP], pj[PAR_ZS], pj[PAR_ZP]);
 #pragma unroll 1
     for (int x = 0; x < 3; ++x) {
       double gx = 0.0;
@@ -707,7 +725,7 @@ __global__ __launch_bounds__(64) void k_pair_forces(pyseqm_layout_t L, int metho
         double R[3] = {R0[0], R0[1], R0[2]};
         R[x] += sgn * h;
         const double e =
-            pair_energy<KIND, XL>(method, geom_from_vector(R[0], R[1], R[2]), pi, pj, Pd, Dr, wmode, smode);
+            pair_energy<KIND, XL>(method, geom_from_vector(R[0], R[1], R[2]), pi, pj, Pd, Dr, wmode, smode, bsel);
         const double c = (sgn == 1 || sgn == -1) ? c1 : c2;
         gx += (sgn > 0 ? c : -c) * e;
       }
@@ -791,6 +809,30 @@ void launch_mol_sum(const pyseqm_layout_t* L, const double* tile_v, const double
   if (L->nmol > 0) k_mol_sum<<<L->nmol, 64, 0, s>>>(*L, tile_v, atom_v, active, out);
 }
 }  // namespace pyseqm
+
+namespace {
+// Core-core repulsion of an explicit pair list (energy.py:91-134): one thread per pair, gam given.
+__global__ void k_core_repulsion_pairs(long long npairs, int method, const int* __restrict__ idxi,
+                                       const int* __restrict__ idxj, const double* __restrict__ rij,
+                                       const double* __restrict__ gam, const double* __restrict__ par,
+                                       double* __restrict__ out) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= npairs) return;
+  const double* pi = par + (size_t)idxi[k] * PYSEQM_NPAR;
+  const double* pj = par + (size_t)idxj[k] * PYSEQM_NPAR;
+  out[k] = core_repulsion(method, (int)pi[PAR_Z], (int)pj[PAR_Z], rij[k], gam[k], pi, pj);
+}
+}  // namespace
+
+extern "C" int pyseqm_core_repulsion_pairs(long long npairs, int method, const int* idxi, const int* idxj,
+                                           const double* rij, const double* gam, const double* atom_par,
+                                           double* enuc_pair, void* stream) {
+  if (npairs < 0 || method < PYSEQM_MNDO || method > PYSEQM_PM3) return PYSEQM_EINVAL;
+  if (npairs == 0) return PYSEQM_OK;
+  k_core_repulsion_pairs<<<(unsigned)((npairs + 255) / 256), 256, 0, (hipStream_t)stream>>>(
+      npairs, method, idxi, idxj, rij, gam, atom_par, enuc_pair);
+  return hipGetLastError() == hipSuccess ? PYSEQM_OK : PYSEQM_ELAUNCH;
+}
 
 namespace {
 template <bool XL>

@@ -65,6 +65,28 @@ class SCFEngine:
         if coordinates is not None:
             self.set_coordinates(coordinates)
 
+    def set_parameters(self, params):
+        """Override per-atom MOPAC parameters ({name: (natoms,) tensor}, names of the parameter
+        files, e.g. learned parameters of Pack_Parameters, basics.py:406-448) and re-derive the
+        multipoles; the isolated-atom energies follow the new U/G values."""
+        from .seqm_functions.parameters import COLUMNS
+        b = self.b
+        for name, v in params.items():
+            if name in COLUMNS and v is not None:
+                self.atom_par[:, COLUMNS.index(name)] = torch.as_tensor(v, dtype=torch.float64,
+                                                                        device=self.device).reshape(-1)[:b.natoms]
+        nat.call("pyseqm_atom_setup", b.L, nat.ptr(self.atom_par), self.stream)
+        Z = torch.as_tensor(b.Z, device=self.device).long()
+        ap = self.atom_par
+        eiso = (ap[:, 0] * torch.as_tensor(USSC, device=self.device)[Z] + ap[:, 1] * torch.as_tensor(UPPC, device=self.device)[Z]
+                + ap[:, 6] * torch.as_tensor(GSSC, device=self.device)[Z] + ap[:, 8] * torch.as_tensor(GPPC, device=self.device)[Z]
+                + ap[:, 7] * torch.as_tensor(GSPC, device=self.device)[Z] + ap[:, 9] * torch.as_tensor(GP2C, device=self.device)[Z]
+                + ap[:, 10] * torch.as_tensor(HSPC, device=self.device)[Z])
+        self.eiso = torch.zeros(b.nmol, dtype=torch.float64, device=self.device).index_add_(
+            0, torch.as_tensor(b.atom_mol, device=self.device).long(), eiso)
+        if self.coords is not None:
+            self.set_coordinates(self.coords)
+
     # ------------------------------------------------------------- integrals
     def set_coordinates(self, coordinates, enuc_pair=False):
         """coordinates: (nmol, molsize, 3) Angstrom in the reference's padded layout, or (natoms, 3)."""
@@ -93,11 +115,10 @@ class SCFEngine:
         """Diagonal guess (scf_loop.py:2066-2081): heavy atoms tore/4 on s and p, H 1.0."""
         b = self.b
         P = torch.zeros(max(b.total, 1), dtype=torch.float64, device=self.device)
-        didx, dmol = b.diag_index()
-        vals = []
-        for a in range(b.natoms):
-            vals.extend([TORE[b.Z[a]] / 4.0] * 4 if b.Z[a] > 1 else [1.0])
+        didx, _ = b.diag_index()
         # diag_index is molecule-major in orbital order, the same order as the atoms
+        k = np.where(b.Z > 1, 4, 1)
+        vals = np.repeat(np.where(b.Z > 1, np.take(TORE, b.Z) / 4.0, 1.0), k)
         P[torch.as_tensor(didx, device=self.device)] = torch.as_tensor(vals, dtype=torch.float64,
                                                                         device=self.device)
         return P
@@ -118,12 +139,12 @@ class SCFEngine:
             return e, V, nmax
         return None
 
-    def density(self, F, idx, nact, Pout):
+    def density(self, F, idx, nact, Pout, fixed_iters=None):
         if self.density_kind == "eigh":
             self.density_eigh(F, idx, nact, Pout)
         else:
             from . import sp2
-            sp2.density_sp2(self, F, idx, nact, Pout)
+            sp2.density_sp2(self, F, idx, nact, Pout, fixed_iters=fixed_iters)
 
     # ------------------------------------------------------------------- SCF
     def _active(self, notconv_host):

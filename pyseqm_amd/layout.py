@@ -47,6 +47,31 @@ def _region(mol, a0, r_start, nr, c_start, nc, tri):
     return t[keep]
 
 
+class MatrixLayout:
+    """Layout of nmol dense n x n blocks stored back to back (a (nmol, n, n) tensor), for the
+    matrix-only kernels (density steps, energies) on inputs given in the reference's dense
+    layouts.  norb[m] = n for every block; occupations per block."""
+
+    def __init__(self, nmol, n, nocc, device):
+        self.nmol, self.n, self.device = int(nmol), int(n), torch.device(device)
+        i32 = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=torch.int32, device=self.device)
+        nocc = np.broadcast_to(np.asarray(torch.as_tensor(nocc).cpu(), dtype=np.int64).reshape(-1), (self.nmol,))
+        self.norb = np.full(self.nmol, self.n, dtype=np.int64)
+        self.nocc = nocc.copy()
+        self.d = {"mol_norb": i32(self.norb), "mol_nocc": i32(self.nocc),
+                  "mol_mat_off": torch.arange(self.nmol, dtype=torch.int64, device=self.device) * (self.n * self.n),
+                  "mol_atom_off": torch.zeros(self.nmol + 1, dtype=torch.int32, device=self.device)}
+        L = _native.Layout()
+        L.nmol, L.natoms, L.npairs = self.nmol, 0, 0
+        for k in range(3):
+            L.ntile[k] = 0
+        for name in ("mol_norb", "mol_nocc", "mol_mat_off", "mol_atom_off"):
+            setattr(L, name, self.d[name].data_ptr())
+        self.L = L
+        self.total = self.nmol * self.n * self.n
+        self.nmax = self.n
+
+
 class PackedBatch:
     """Host description + device tensors of one packed batch."""
 
@@ -170,6 +195,27 @@ class PackedBatch:
             setattr(L, name, self.d[name].data_ptr())
         self.L = L
 
+    @classmethod
+    def from_counts(cls, nheavy, nhydro, nocc, molsize, device):
+        """Batch with the orbital layout of (nheavy, nhydro) atoms per molecule (heavy atoms first,
+        as pack/unpack assume, pack.py:64-96) and the given occupations; the element of each heavy
+        atom is irrelevant to the layout (carbon stands in)."""
+        nh = np.asarray(torch.as_tensor(nheavy).cpu(), dtype=np.int64).reshape(-1)
+        nH = np.asarray(torch.as_tensor(nhydro).cpu(), dtype=np.int64).reshape(-1)
+        sp = np.zeros((len(nh), int(molsize)), dtype=np.int64)
+        for m in range(len(nh)):
+            sp[m, :nh[m]] = 6
+            sp[m, nh[m]:nh[m] + nH[m]] = 1
+        tore = np.asarray(TORE)[sp].sum(1).astype(np.int64)
+        b = cls(sp, "AM1", device, charges=tore % 2)
+        b.set_nocc(nocc)
+        return b
+
+    def set_nocc(self, nocc):
+        self.nocc = np.asarray(torch.as_tensor(nocc).cpu(), dtype=np.int64).reshape(-1).copy()
+        self.d["mol_nocc"] = torch.as_tensor(self.nocc, dtype=torch.int32, device=self.device)
+        self.L.mol_nocc = self.d["mol_nocc"].data_ptr()
+
     # ------------------------------------------------------------- helpers
     @property
     def ntiles(self):
@@ -207,12 +253,10 @@ class PackedBatch:
 
     def diag_index(self):
         """Flat packed index of every diagonal element and its molecule."""
-        idx, mol = [], []
-        for m in range(self.nmol):
-            n = int(self.norb[m])
-            idx.append(self.mol_mat_off[m] + np.arange(n) * (n + 1))
-            mol.append(np.full(n, m))
-        return np.concatenate(idx), np.concatenate(mol)
+        mol = np.repeat(np.arange(self.nmol), self.norb)
+        first = np.concatenate([[0], np.cumsum(self.norb)])[:-1]
+        k = np.arange(len(mol)) - first[mol]
+        return self.mol_mat_off[mol] + k * (self.norb[mol] + 1), mol
 
     def to_padded(self, X):
         """Packed matrices (flat) -> reference layout (nmol, 4*molsize, 4*molsize) on the device."""

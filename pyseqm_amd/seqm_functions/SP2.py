@@ -1,0 +1,34 @@
+"""SP2(a, nocc, eps, factor): trace-correcting purification of dense (N, D, D) Fock matrices
+(seqm/seqm_functions/SP2.py:9-85) on the mixed-precision MFMA kernels.
+
+Every D x D block is one system, zero rows included, exactly as the reference treats its input.
+X runs in FP32 with X^2 on bf16x3 MFMA (FP32 accumulation); the stop rule is the reference's
+reduced-precision one (trace error below max(eps, 1e-2) and no longer improving, SP2.py:21-28,
+74-78); the returned density gets one FP64 McWeeny step, factor * (3 X^2 - 2 X^3)."""
+import numpy as np
+import torch
+
+from .. import _native as nat
+from ..layout import MatrixLayout
+from ..sp2 import SP2_EPS, density_sp2
+
+
+class _MatrixEngine:
+    def __init__(self, b):
+        self.b = b
+        self.device = b.device
+        self.stream = nat.stream_ptr(b.device)
+        self.red = torch.empty(b.nmol * nat.MAX_RED_BLOCKS * 16, dtype=torch.float64, device=b.device)
+        self._idx_host = np.arange(b.nmol)
+
+
+def SP2(a, nocc, eps=1.0e-4, factor=2.0):
+    dev = nat.require_device(a.device)
+    N, D, _ = a.shape
+    b = MatrixLayout(N, D, nocc, dev)
+    eng = _MatrixEngine(b)
+    F = a.detach().to(torch.float64).contiguous().reshape(-1)
+    P = torch.empty_like(F)
+    idx = torch.arange(N, dtype=torch.int32, device=dev)
+    density_sp2(eng, F, idx, N, P, eps=max(float(eps), SP2_EPS))
+    return (0.5 * factor * P).reshape(N, D, D).to(a.dtype)

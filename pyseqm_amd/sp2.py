@@ -40,7 +40,10 @@ class SP2Workspace:
         self.timing = None  # list of (start event, end event, idx, nact) of the SP2 GEMM loops
 
 
-def density_sp2(eng, F, idx, nact, Pout, eps=SP2_EPS, max_iter=SP2_MAX_ITER):
+def density_sp2(eng, F, idx, nact, Pout, eps=SP2_EPS, max_iter=SP2_MAX_ITER, fixed_iters=None):
+    """fixed_iters: launch exactly that many SP2 iterations and never read the stop counter on the
+    host (molecules that stopped make the remaining launches no-ops); the launch sequence is then
+    free of host synchronisation and can be captured in a graph."""
     b = eng.b
     ws = getattr(eng, "_sp2ws", None)
     if ws is None:
@@ -56,11 +59,11 @@ def density_sp2(eng, F, idx, nact, Pout, eps=SP2_EPS, max_iter=SP2_MAX_ITER):
         ev0 = torch.cuda.Event(enable_timing=True)
         ev1 = torch.cuda.Event(enable_timing=True)
         ev0.record()
-    for it in range(max_iter):
+    for it in range(max_iter if fixed_iters is None else int(fixed_iters)):
         nat.call("pyseqm_sp2_iterate", L, nat.ptr(idx), nact, nmax, nat.ptr(ws.xoff), nat.ptr(ws.xld),
                  nat.ptr(bufs[it % 2]), nat.ptr(bufs[(it + 1) % 2]), nat.ptr(ws.state), nat.ptr(ws.part),
                  ws.part_stride, float(eps), int(max_iter), nat.ptr(ws.ndone), s)
-        if (it + 1) % POLL == 0 and int(ws.ndone.item()) >= nact:
+        if fixed_iters is None and (it + 1) % POLL == 0 and int(ws.ndone.item()) >= nact:
             break
     if ws.timing is not None:
         ev1.record()

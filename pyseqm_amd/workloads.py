@@ -121,3 +121,36 @@ CONFIGS = {
     "C3": dict(method="PM3", make=lambda seed=0: water_clusters(256, 8, seed)),
     "C4": dict(method="PM3", make=lambda seed=0: water_clusters(1024, 1, seed)),
 }
+
+
+def _rotate(x, rng):
+    q = rng.normal(size=4)
+    a, b, c, d = q / np.linalg.norm(q)
+    Rm = np.array([[a * a + b * b - c * c - d * d, 2 * (b * c - a * d), 2 * (b * d + a * c)],
+                   [2 * (b * c + a * d), a * a - b * b + c * c - d * d, 2 * (c * d - a * b)],
+                   [2 * (b * d - a * c), 2 * (c * d + a * b), a * a - b * b - c * c + d * d]])
+    return x @ Rm.T
+
+
+def serving_stream(rate=2000.0, window=10.0, seed=0, water_min=4, water_max=256, alkane_max=30, pool=48,
+                   noise=0.02):
+    """Open-loop Poisson request stream: exponential inter-arrival times of mean 1/rate over
+    `window` seconds; half AM1 alkane single points (n ~ U{2..alkane_max}), half PM3 water clusters
+    (N ~ U{water_min..water_max}).  Water clusters come from a pool of `pool` random-sequential-
+    addition clusters (sizes drawn once), each request re-oriented at random and perturbed by
+    N(0, noise); returns [(t_arrival, species, coords, method)] sorted by arrival."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, size=int(rate * window * 1.2) + 16))
+    t = t[t < window]
+    sizes = rng.integers(water_min, water_max + 1, size=pool)
+    wpool = [water_cluster(int(n), seed=1000 + k) for k, n in enumerate(sizes)]
+    apool = {n: alkane(n) for n in range(2, alkane_max + 1)}
+    out = []
+    for ta in t:
+        if rng.random() < 0.5:
+            s, x = apool[int(rng.integers(2, alkane_max + 1))]
+            out.append((float(ta), s, x + rng.normal(0.0, 0.03, x.shape), "AM1"))
+        else:
+            s, x = wpool[int(rng.integers(0, pool))]
+            out.append((float(ta), s, _rotate(x, rng) + rng.normal(0.0, noise, x.shape), "PM3"))
+    return out

@@ -1,0 +1,109 @@
+"""The seqm_functions entry points with the reference's signatures (hcore, two_elec_two_center_int,
+fock, sym_eig_trunc, SP2, elec_energy, pair_nuclear_energy) on the GPU, fed the oracle's inputs in
+the reference's layouts and compared with the oracle's outputs.
+
+Bars: integrals / Fock / energies 1e-12 relative to the matrix scale (FP64, summation order);
+FP64 density step 1e-9; mixed-precision SP2 density 1e-5 (bf16x3 + one FP64 McWeeny step)."""
+import numpy as np
+import pytest
+import torch
+
+import pyseqm_amd
+from oracle import seqm_ref as R
+from pyseqm_amd.seqm_functions.SP2 import SP2
+from pyseqm_amd.seqm_functions.diag import sym_eig_trunc
+from pyseqm_amd.seqm_functions.energy import elec_energy, pair_nuclear_energy
+from pyseqm_amd.seqm_functions.fock import fock
+from pyseqm_amd.seqm_functions.hcore import hcore
+from pyseqm_amd.seqm_functions.pack import pack, unpack
+from pyseqm_amd.seqm_functions.two_elec_two_center_int import two_elec_two_center_int
+from tests.test_gpu_parity import _inputs
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda"
+T = lambda a, **k: torch.as_tensor(a, device=DEV, **k)
+
+
+def _close(a, b, rtol):
+    a = a.detach().cpu().numpy() if torch.is_tensor(a) else np.asarray(a)
+    b = np.asarray(b)
+    scale = max(np.abs(b).max(), 1.0)
+    err = np.abs(a - b).max()
+    assert err <= rtol * scale, (err, scale)
+
+
+@pytest.mark.parametrize("name,method", [("sulfur", "PM3"), ("water8", "AM1"), ("methanal+benzene", "MNDO")])
+def test_integrals_and_fock_entry_points(name, method):
+    sp, c = _inputs(name)
+    ob = R.parse(sp, c, method)
+    H, Tt, di, gam, mp = R.hcore(ob)
+    w_o = R.tensor_to_w(Tt)
+    prm = ob.prm
+    Z = T(ob.Z)
+    g = {k: T(prm[k][ob.Z]) for k in ("zeta_s", "zeta_p", "g_ss", "g_pp", "g_sp", "g_p2", "h_sp")}
+    # two_elec_two_center_int on the oracle's pair list
+    w, e1b, e2a, r0i, r0j, _, _ = two_elec_two_center_int(
+        None, T(ob.idxi), T(ob.idxj), T(ob.ni), T(ob.nj), T(ob.xij), T(ob.rij), Z, g["zeta_s"], g["zeta_p"], None,
+        None, None, None, g["g_ss"], g["g_pp"], g["g_p2"], g["h_sp"], None, None, None, None, None, method)
+    _close(w, w_o, 1e-12)
+    tore = R.TORE
+    for k, (mu, nu) in enumerate(zip((0, 0, 1, 0, 1, 2, 0, 1, 2, 3), (0, 1, 1, 2, 2, 2, 3, 3, 3, 3))):
+        _close(e1b[:, mu, nu], -tore[ob.nj] * w_o[:, k, 0], 1e-12)
+        _close(e2a[:, mu, nu], -tore[ob.ni] * w_o[:, 0, k], 1e-12)
+    # hcore through a Molecule
+    p = {"method": method, "scf_eps": 1e-8, "scf_converger": [2]}
+    mol = pyseqm_amd.Molecule(pyseqm_amd.Constants().to(DEV), p, T(c), T(sp))
+    M, w2, rho0xi, rho0xj, _, _ = hcore(mol)
+    Hp = M.reshape(ob.nmol, ob.molsize, ob.molsize, 4, 4).transpose(2, 3).reshape(H.shape)
+    _close(Hp, np.triu(H), 1e-12)
+    _close(w2, w_o, 1e-12)
+    # fock with the reference's arguments
+    rng = np.random.default_rng(1)
+    Pr = rng.normal(size=H.shape)
+    Pr = Pr + Pr.transpose(0, 2, 1)
+    for a in np.nonzero(ob.Z == 1)[0]:
+        o = 4 * ob.atom_loc[a]
+        Pr[ob.atom_mol[a], o + 1:o + 4, :] = 0.0
+        Pr[ob.atom_mol[a], :, o + 1:o + 4] = 0.0
+    F = fock(ob.nmol, ob.molsize, T(Pr), M, T(ob.maskd), T(ob.mask), T(ob.idxi), T(ob.idxj), w2, None, g["g_ss"],
+             g["g_pp"], g["g_sp"], g["g_p2"], g["h_sp"], method, g["zeta_s"], g["zeta_p"], None, Z, None, None)
+    Fo = R.fock(ob, Pr, H, Tt)
+    _close(F, Fo, 1e-12)
+    _close(elec_energy(T(Pr), F, T(np.triu(H))), R.elec_energy(Pr, Fo, H), 1e-12)
+    # core-core repulsion
+    alpha = T(prm["alpha"][ob.Z])
+    ng = {"MNDO": 0, "AM1": 4, "PM3": 2}[method]
+    par = (alpha,) if ng == 0 else (alpha,) + tuple(
+        torch.stack([T(prm[f"Gaussian{k}_{x}"][ob.Z]) for k in range(1, ng + 1)], 1) for x in "KLM")
+    en = pair_nuclear_energy(Z, None, ob.nmol, T(ob.ni), T(ob.nj), T(ob.idxi), T(ob.idxj), T(ob.rij), rho0xi, rho0xj,
+                             None, None, w2[:, 0, 0], method=method, parameters=par)
+    _close(en, R.pair_nuclear_energy(ob, gam), 1e-12)
+
+
+def test_density_entry_points():
+    sp, c = _inputs("methanal+benzene")
+    r = R.single_point(sp, c, "PM3", eps=1e-9, converger=(2,))
+    ob, F = r["batch"], r["F"]
+    e_o, P_o, _ = R.density_eigh(ob, F)
+    e, P, v = sym_eig_trunc(T(F), T(ob.nHeavy), T(ob.nHydro), T(ob.nocc))
+    _close(P, P_o, 1e-9)
+    _close(e, e_o, 1e-9)
+    # pack -> SP2 -> unpack (the reference's SP2 call pattern, scf_loop.py:66-78) on same-size systems
+    sp6, c6 = _inputs("benzene4")
+    r6 = R.single_point(sp6, c6, "PM3", eps=1e-9, converger=(2,))
+    b6, F6 = r6["batch"], r6["F"]
+    nh, nH = T(b6.nHeavy), T(b6.nHydro)
+    Pk = SP2(pack(T(F6), nh, nH), T(b6.nocc), 1e-4)
+    Ps = unpack(Pk, nh, nH, F6.shape[-1]).cpu().numpy()
+    Po = r6["P"]
+    # the mixed-precision density differs from the projector at first order in its elements but
+    # only at second order in the energy; idempotency and trace are restored by the FP64 step
+    assert np.abs(Ps - Po).max() < 2e-4
+    H6 = R.hcore(b6)[0]
+    dE = np.abs(R.elec_energy(Ps, R.fock(b6, Ps, H6, R.hcore(b6)[1]), H6) - R.elec_energy(Po, F6, H6)) / 12
+    assert dE.max() < 1e-6, dE
+    for m in range(4):
+        idx = R.orbital_index(b6, m)
+        p = Ps[m][np.ix_(idx, idx)]
+        assert np.abs(p @ p - 2 * p).max() < 1e-6
+        assert abs(np.trace(p) - 2 * b6.nocc[m]) < 1e-8

@@ -47,6 +47,8 @@ def _inputs(name):
         return W.water_clusters(8, 2, seed=3)
     if name == "alkanes":
         return W.alkanes(6, seed=5, nmax=9)
+    if name == "benzene4":
+        return W.benzene(4, seed=2)
     raise KeyError(name)
 
 

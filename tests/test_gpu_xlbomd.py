@@ -66,7 +66,7 @@ def test_xl_energy_and_forces_match_oracle(name, method):
     H, T, _, gam, _ = R.hcore(ob)
     Fo = R.fock(ob, Pp, H, T)
     np.testing.assert_allclose(b.to_padded(tr.F).cpu().numpy(), Fo, atol=1e-12 * np.abs(Fo).max(), rtol=0)
-    Do, _ = R.density_eigh(ob, Fo)
+    _, Do, _ = R.density_eigh(ob, Fo)
     np.testing.assert_allclose(Dp, Do, atol=1e-8, rtol=0)
     eo = R.elec_energy_xl(Dp, Pp, Fo, H)
     np.testing.assert_allclose(tr.eelec.cpu().numpy(), eo, atol=1e-10 * np.abs(eo).max(), rtol=0)

@@ -83,3 +83,31 @@ def test_parser_errors_match_reference():
         PackedBatch(sp, "AM1", "cpu", charges=[1])
     with pytest.raises(ValueError, match="not supported"):
         PackedBatch(np.array([[35, 1]]), "AM1", "cpu")
+
+
+def test_pack_unpack_match_orbital_selection():
+    """pack keeps exactly the real orbitals of each molecule in reference order (heavy blocks,
+    then hydrogen s rows); unpack is its inverse on real rows and zero elsewhere."""
+    import torch
+    from oracle import seqm_ref as R
+    from pyseqm_amd.seqm_functions.pack import pack, unpack
+    sp, c = read_xyz(["methanal.1.xyz", "benzene.xyz", "methane.xyz"])
+    ob = R.parse(sp, c, "AM1")
+    rng = np.random.default_rng(0)
+    n = 4 * ob.molsize
+    X = rng.normal(size=(ob.nmol, n, n))
+    Xp = pack(torch.as_tensor(X), torch.as_tensor(ob.nHeavy), torch.as_tensor(ob.nHydro)).numpy()
+    N = int(ob.norb.max())
+    assert Xp.shape == (ob.nmol, N, N)
+    for m in range(ob.nmol):
+        idx = R.orbital_index(ob, m)
+        k = len(idx)
+        np.testing.assert_array_equal(Xp[m, :k, :k], X[m][np.ix_(idx, idx)])
+        assert not Xp[m, k:].any() and not Xp[m, :, k:].any()
+    Xu = unpack(torch.as_tensor(Xp), torch.as_tensor(ob.nHeavy), torch.as_tensor(ob.nHydro), n).numpy()
+    for m in range(ob.nmol):
+        idx = R.orbital_index(ob, m)
+        np.testing.assert_array_equal(Xu[m][np.ix_(idx, idx)], X[m][np.ix_(idx, idx)])
+        mask = np.ones((n, n), bool)
+        mask[np.ix_(idx, idx)] = False
+        assert not Xu[m][mask].any()
