@@ -178,6 +178,36 @@ def fock_roofline(eng, P, reps):
             "algorithmic_bytes": nbytes, "launch_ms": ms, "pairs": {"XX": nxx, "XH": nxh, "HH": nhh}}
 
 
+def kernel_only(args, method, sp, xyz, dev):
+    """One kernel group on the workload, --steps times, for counter collection."""
+    import torch
+
+    from pyseqm_amd.engine import SCFEngine
+    eng = SCFEngine(sp, xyz, method, dev)
+    r = eng.scf(eps=args.sp2_eps, converger=(2,), density="sp2" if args.kernel_only == "sp2" else "eigh")
+    torch.cuda.synchronize()
+    if args.kernel_only == "fock":
+        F = torch.empty_like(r.P)
+        for _ in range(args.steps):
+            eng.fock(r.P, F)
+        nbytes, _ = fock_algorithmic_bytes(eng.b)
+        info = {"algorithmic_bytes_per_launch": nbytes}
+    elif args.kernel_only == "pairs":
+        c = torch.as_tensor(xyz, device=dev)
+        for _ in range(args.steps):
+            eng.set_coordinates(c)
+        info = {}
+    else:
+        for _ in range(args.steps):
+            eng.set_coordinates(torch.as_tensor(xyz, device=dev))
+            eng.scf(eps=args.sp2_eps, converger=(2,), density="sp2")
+        info = {}
+    torch.cuda.synchronize()
+    print(json.dumps({"kernel_only": args.kernel_only, "config": args.config, "launches": args.steps, **info}),
+          flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -202,20 +232,37 @@ def main():
     ap.add_argument("--serve-eps", type=float, default=1e-5)
     ap.add_argument("--serve-drain", type=float, default=20.0)
     ap.add_argument("--xl-trajectories", type=int, default=256)
+    ap.add_argument("--kernel-only", choices=["fock", "pairs", "sp2"], default=None,
+                    help="profiling aid: run only this kernel group on the workload (--steps launches) and exit")
+    ap.add_argument("--copy-calibration", type=float, default=0.0,
+                    help="profiling aid: copy this many MiB device-to-device --steps times and exit (the plain "
+                         "copy kernel whose FETCH_SIZE / WRITE_SIZE calibrate the traffic counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (this path is "
-              "data-parallel over molecules; see the sharding notes in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported. The batched SCF is "
+              "data-parallel over molecules and requests (one process per GPU, no exchange); only the "
+              "one-GPU figure is measured here.", file=sys.stderr)
         return 2
 
     import torch
 
     from pyseqm_amd.engine import SCFEngine
 
-    method, sp, xyz = make_workload(args.config, args.seed)
-    conv = (2,)
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if args.copy_calibration > 0:
+        nb = int(args.copy_calibration * 2**20) // 8 * 8
+        src = torch.ones(nb // 8, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(args.steps):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_per_launch": nb, "launches": args.steps}), flush=True)
+        return 0
+    method, sp, xyz = make_workload(args.config, args.seed)
+    conv = (2,)
+    if args.kernel_only:
+        return kernel_only(args, method, sp, xyz, dev)
     eng = SCFEngine(sp, None, method, dev)
     coords = torch.as_tensor(xyz, dtype=torch.float64, device=dev)
     nmol = eng.b.nmol

@@ -41,8 +41,9 @@ class Molecular_Dynamics_Basic(torch.nn.Module):
 
     def initialize_velocity(self, molecule, vel_com=True):
         """Maxwell-Boltzmann velocities rescaled to exactly Temp (MolecularDynamics.py:717-744).
-        The normal deviates are drawn on the CPU generator (seeded by ``run(seed=)``) in the padded
-        layout, so a given seed gives the same velocities on every device."""
+        The normal deviates come from the generator of the molecule's device, in the padded
+        layout (torch.randn_like of the coordinates, seeded by ``run(seed=)``), as in the
+        reference."""
         if torch.is_tensor(molecule.velocities):
             if vel_com:
                 self._zero_com(molecule, translate_to_origin=False, restore_kinetic_energy=False)
@@ -51,8 +52,7 @@ class Molecular_Dynamics_Basic(torch.nn.Module):
             molecule.velocities = torch.zeros_like(molecule.coordinates)
             return molecule.velocities
         scale = torch.sqrt(self.Temp * molecule.mass_inverse) * VEL_SCALE
-        g = torch.randn(tuple(molecule.coordinates.shape), dtype=molecule.coordinates.dtype)
-        molecule.velocities = g.to(molecule.coordinates.device) * scale
+        molecule.velocities = torch.randn_like(molecule.coordinates.detach()) * scale
         Ek = self._kinetic_energy(molecule)
         alpha = torch.sqrt(self.Temp / self._calc_temperature(Ek))
         molecule.velocities.mul_(alpha.reshape(-1, 1, 1))
@@ -126,6 +126,8 @@ class Molecular_Dynamics_Basic(torch.nn.Module):
             **kwargs):
         if seed is not None:
             torch.manual_seed(int(seed))
+            if torch.cuda.is_available():
+                torch.cuda.manual_seed_all(int(seed))
         self.initialize(molecule, remove_com=remove_com, learned_parameters=learned_parameters, steps=steps)
         if not reuse_P:
             molecule.dm = None

@@ -78,4 +78,20 @@ __device__ inline void write_partials(double* part, int tile, int lane, const do
   }
 }
 
+// Sum of an atom's tile partial slots csr_idx[e0..e1): lane l adds slots e0 + l, e0 + l + 64, ...
+// in order, then a fixed shuffle tree; every lane ends with the totals (deterministic).
+template <int N>
+__device__ inline void gather_slots(const double* part, const int* csr_idx, int e0, int e1, int lane,
+                                    double (&acc)[N]) {
+#pragma unroll
+  for (int k = 0; k < N; ++k) acc[k] = 0.0;
+  for (int e = e0 + lane; e < e1; e += 64) {
+    const double* src = part + (size_t)csr_idx[e] * kPartStride;
+#pragma unroll
+    for (int k = 0; k < N; ++k) acc[k] += src[k];
+  }
+#pragma unroll
+  for (int k = 0; k < N; ++k) acc[k] = wave_sum(acc[k]);
+}
+
 }  // namespace pyseqm

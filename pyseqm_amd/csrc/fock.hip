@@ -141,12 +141,133 @@ __global__ __launch_bounds__(64) void k_fock_tiles(pyseqm_layout_t L, int tile_b
   if (lane == 0) etile[tile] = es;
 }
 
+// The integral contraction of k_fock_xx2 for the (mu nu) pairs k = K0 .. K0+4 (static indices).
+template <int K0>
+__device__ inline void xx_half(const double* __restrict__ wt, const double* Pm, int n, int ai, int aj,
+                               const double (&pab)[4][4], double (&rowv)[5], double (&colv)[10], double (&K)[4][4]) {
+  double pa[5], pb[10];
+#pragma unroll
+  for (int kk = 0; kk < 5; ++kk) {
+    const int k = K0 + kk;
+    pa[kk] = (kTriI[k] == kTriJ[k] ? 1.0 : 2.0) * Pm[(size_t)(ai + kTriI[k]) * n + ai + kTriJ[k]];
+  }
+#pragma unroll
+  for (int l = 0; l < 10; ++l)
+    pb[l] = (kTriI[l] == kTriJ[l] ? 1.0 : 2.0) * Pm[(size_t)(aj + kTriI[l]) * n + aj + kTriJ[l]];
+#pragma unroll
+  for (int kk = 0; kk < 5; ++kk) {
+    const int k = K0 + kk;
+    const int mu = kTriI[k], nu = kTriJ[k];
+#pragma unroll
+    for (int l = 0; l < 10; ++l) {
+      const int la = kTriI[l], si = kTriJ[l];
+      const double v = wt[(size_t)(k * 10 + l) * 64];
+      colv[l] += v * pa[kk];
+      rowv[kk] += v * pb[l];
+      K[mu][la] += v * pab[nu][si];
+      if (mu != nu) K[nu][la] += v * pab[mu][si];
+      if (la != si) K[mu][si] += v * pab[nu][la];
+      if (mu != nu && la != si) K[nu][si] += v * pab[mu][la];
+    }
+  }
+}
+
+// Heavy-heavy tiles with two waves per tile: wave q contracts the integrals (kl) with k in its half
+// of the 10 (mu nu) pairs, so each wave streams 50 of the 100 integral columns; wave 1 hands its
+// partial Coulomb column sums, exchange block and row sums to wave 0 through LDS, and wave 0 writes
+// F, the tile partials and the energy.  Same arithmetic as k_fock_tiles<XX>, summed in two halves.
+__global__ __launch_bounds__(128) void k_fock_xx2(pyseqm_layout_t L, const double* __restrict__ w,
+                                                  const double* __restrict__ P, const double* __restrict__ H,
+                                                  double* __restrict__ F, double* __restrict__ part,
+                                                  double* __restrict__ etile,
+                                                  const unsigned char* __restrict__ active) {
+  __shared__ double xch[31][64];
+  const int tile = blockIdx.x;
+  const int q = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const Tile t = load_tile(L.tiles, tile);
+  if (active && !active[t.mol]) return;  // uniform over the block
+  int i, j;
+  const bool valid = lane_pair(t, lane, i, j);
+  double rowv[5], colv[10], K[4][4];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) rowv[k] = 0.0;
+#pragma unroll
+  for (int k = 0; k < 10; ++k) colv[k] = 0.0;
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) K[a][b] = 0.0;
+  long long mo = 0;
+  int n = 0, ai = 0, aj = 0;
+  double pab[4][4];
+  if (valid) {
+    mo = L.mol_mat_off[t.mol];
+    n = L.mol_norb[t.mol];
+    ai = L.atom_ao[i];
+    aj = L.atom_ao[j];
+    const double* Pm = P + mo;
+    const double* wt = w + (size_t)tile * 6400 + lane;
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) pab[a][b] = Pm[(size_t)(ai + a) * n + aj + b];
+    if (q == 0)
+      xx_half<0>(wt, Pm, n, ai, aj, pab, rowv, colv, K);
+    else
+      xx_half<5>(wt, Pm, n, ai, aj, pab, rowv, colv, K);
+  }
+  if (q == 1) {
+#pragma unroll
+    for (int l = 0; l < 10; ++l) xch[l][lane] = colv[l];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) xch[10 + 4 * a + b][lane] = K[a][b];
+#pragma unroll
+    for (int kk = 0; kk < 5; ++kk) xch[26 + kk][lane] = rowv[kk];
+  }
+  __syncthreads();
+  if (q == 1) return;
+  double row10[10];
+#pragma unroll
+  for (int kk = 0; kk < 5; ++kk) {
+    row10[kk] = rowv[kk];
+    row10[5 + kk] = xch[26 + kk][lane];
+  }
+#pragma unroll
+  for (int l = 0; l < 10; ++l) colv[l] += xch[l][lane];
+  double e = 0.0;
+  if (valid) {
+    const double* Hm = H + mo;
+    double* Fm = F + mo;
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const double kt = K[a][b] + xch[10 + 4 * a + b][lane];
+        const double h = Hm[(size_t)(ai + a) * n + aj + b];
+        const double f = h - 0.5 * kt;
+        Fm[(size_t)(ai + a) * n + aj + b] = f;
+        Fm[(size_t)(aj + b) * n + ai + a] = f;
+        e += pab[a][b] * (h + f);
+      }
+  }
+  write_partials(part, tile, lane, row10, colv);
+  const double es = wave_sum(e);
+  if (lane == 0) etile[tile] = es;
+}
+
 // Diagonal blocks: H_ii + one-center G (fock.py:196-224) + Coulomb partials; energy share.
-__global__ void k_fock_diag(pyseqm_layout_t L, const double* __restrict__ par, const double* __restrict__ P,
-                            const double* __restrict__ H, const double* __restrict__ part, double* __restrict__ F,
-                            double* __restrict__ eatom, const unsigned char* __restrict__ active) {
-  const int a = blockIdx.x * blockDim.x + threadIdx.x;
-  if (a >= L.natoms) return;
+// One wave per atom: the lanes split the atom's partial slots, then lanes 0..15 write the block.
+__global__ __launch_bounds__(256) void k_fock_diag(pyseqm_layout_t L, const double* __restrict__ par,
+                                                   const double* __restrict__ P, const double* __restrict__ H,
+                                                   const double* __restrict__ part, double* __restrict__ F,
+                                                   double* __restrict__ eatom,
+                                                   const unsigned char* __restrict__ active) {
+  const int a = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (a >= L.natoms) return;  // uniform over the wave
   const int m = L.atom_mol[a];
   if (active && !active[m]) return;
   const int n = L.mol_norb[m];
@@ -156,18 +277,15 @@ __global__ void k_fock_diag(pyseqm_layout_t L, const double* __restrict__ par, c
   double* Fm = F + mo;
   const int ao = L.atom_ao[a];
   const double* p = par + (size_t)a * PYSEQM_NPAR;
+  const int e0 = L.csr_off[a], e1 = L.csr_off[a + 1];
   if (L.atom_Z[a] > 1) {
     double acc[10], pd[10], hd[10];
+    gather_slots(part, L.csr_idx, e0, e1, lane, acc);
 #pragma unroll
     for (int k = 0; k < 10; ++k) {
       pd[k] = Pm[(size_t)(ao + kTriI[k]) * n + ao + kTriJ[k]];
       hd[k] = Hm[(size_t)(ao + kTriI[k]) * n + ao + kTriJ[k]];
-      acc[k] = hd[k];
-    }
-    for (int e = L.csr_off[a]; e < L.csr_off[a + 1]; ++e) {
-      const double* src = part + (size_t)L.csr_idx[e] * kPartStride;
-#pragma unroll
-      for (int k = 0; k < 10; ++k) acc[k] += src[k];
+      acc[k] += hd[k];
     }
     const double gss = p[PAR_GSS], gsp = p[PAR_GSP], gpp = p[PAR_GPP], gp2 = p[PAR_GP2], hsp = p[PAR_HSP];
     const double pss = pd[0], ppt = pd[2] + pd[5] + pd[9];
@@ -183,20 +301,29 @@ __global__ void k_fock_diag(pyseqm_layout_t L, const double* __restrict__ par, c
     acc[4] += pd[4] * fo;
     acc[7] += pd[7] * fo;
     acc[8] += pd[8] * fo;
-    double e = 0.0;
+    if (lane == 0) {
+      double e = 0.0;
 #pragma unroll
-    for (int k = 0; k < 10; ++k) e += (kTriI[k] == kTriJ[k] ? 0.5 : 1.0) * pd[k] * (hd[k] + acc[k]);
-    eatom[a] = e;
+      for (int k = 0; k < 10; ++k) e += (kTriI[k] == kTriJ[k] ? 0.5 : 1.0) * pd[k] * (hd[k] + acc[k]);
+      eatom[a] = e;
+    }
+    if (lane < 16) {
+      const int mu = lane >> 2, nu = lane & 3;
+      double v = acc[0];
 #pragma unroll
-    for (int mu = 0; mu < 4; ++mu)
-#pragma unroll
-      for (int nu = 0; nu < 4; ++nu) Fm[(size_t)(ao + mu) * n + ao + nu] = acc[tri_index(mu, nu)];
+      for (int k = 1; k < 10; ++k)
+        if (k == tri_index(mu, nu)) v = acc[k];
+      Fm[(size_t)(ao + mu) * n + ao + nu] = v;
+    }
   } else {
-    const double pd = Pm[(size_t)ao * n + ao], hd = Hm[(size_t)ao * n + ao];
-    double acc = hd + 0.5 * pd * p[PAR_GSS];
-    for (int e = L.csr_off[a]; e < L.csr_off[a + 1]; ++e) acc += part[(size_t)L.csr_idx[e] * kPartStride];
-    Fm[(size_t)ao * n + ao] = acc;
-    eatom[a] = 0.5 * pd * (hd + acc);
+    double acc[1];
+    gather_slots(part, L.csr_idx, e0, e1, lane, acc);
+    if (lane == 0) {
+      const double pd = Pm[(size_t)ao * n + ao], hd = Hm[(size_t)ao * n + ao];
+      const double f = acc[0] + hd + 0.5 * pd * p[PAR_GSS];
+      Fm[(size_t)ao * n + ao] = f;
+      eatom[a] = 0.5 * pd * (hd + f);
+    }
   }
 }
 
@@ -209,13 +336,12 @@ extern "C" int pyseqm_fock(const pyseqm_layout_t* L, const double* atom_par, con
   if (!L) return PYSEQM_EINVAL;
   hipStream_t s = (hipStream_t)stream;
   int base = 0;
-  if (L->ntile[0]) k_fock_tiles<XX><<<L->ntile[0], 64, 0, s>>>(*L, base, w_xx, P, H, F, part, etile, active);
+  if (L->ntile[0]) k_fock_xx2<<<L->ntile[0], 128, 0, s>>>(*L, w_xx, P, H, F, part, etile, active);
   base += L->ntile[0];
   if (L->ntile[1]) k_fock_tiles<XH><<<L->ntile[1], 64, 0, s>>>(*L, base, w_xh, P, H, F, part, etile, active);
   base += L->ntile[1];
   if (L->ntile[2]) k_fock_tiles<HH><<<L->ntile[2], 64, 0, s>>>(*L, base, w_hh, P, H, F, part, etile, active);
-  if (L->natoms)
-    k_fock_diag<<<(L->natoms + 127) / 128, 128, 0, s>>>(*L, atom_par, P, H, part, F, eatom, active);
+  if (L->natoms) k_fock_diag<<<(L->natoms + 3) / 4, 256, 0, s>>>(*L, atom_par, P, H, part, F, eatom, active);
   pyseqm::launch_mol_sum(L, etile, eatom, active, eelec, s);
   return hipGetLastError() == hipSuccess ? PYSEQM_OK : PYSEQM_ELAUNCH;
 }

@@ -736,50 +736,48 @@ __global__ __launch_bounds__(64) void k_pair_forces(pyseqm_layout_t L, int metho
   write_partials(part, tile, lane, rowv, colv);
 }
 
-__global__ void k_atom_forces(pyseqm_layout_t L, const double* __restrict__ part, double* __restrict__ F) {
-  const int a = blockIdx.x * blockDim.x + threadIdx.x;
-  if (a >= L.natoms) return;
-  double f[3] = {0.0, 0.0, 0.0};
-  for (int e = L.csr_off[a]; e < L.csr_off[a + 1]; ++e) {
-    const double* src = part + (size_t)L.csr_idx[e] * kPartStride;
-    f[0] += src[0];
-    f[1] += src[1];
-    f[2] += src[2];
-  }
-  F[3 * a] = f[0];
-  F[3 * a + 1] = f[1];
-  F[3 * a + 2] = f[2];
+__global__ __launch_bounds__(256) void k_atom_forces(pyseqm_layout_t L, const double* __restrict__ part,
+                                                     double* __restrict__ F) {
+  const int a = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (a >= L.natoms) return;  // uniform over the wave
+  double f[3];
+  gather_slots(part, L.csr_idx, L.csr_off[a], L.csr_off[a + 1], lane, f);
+  if (lane < 3) F[3 * a + lane] = lane == 0 ? f[0] : (lane == 1 ? f[1] : f[2]);
 }
 
 // Diagonal core-Hamiltonian blocks: U on the diagonal plus the electron-core attractions
 // collected from every tile the atom appears in (hcore.py:127-150), fixed summation order.
-__global__ void k_hcore_diag(pyseqm_layout_t L, const double* __restrict__ par, const double* __restrict__ part,
-                             double* __restrict__ H) {
-  const int a = blockIdx.x * blockDim.x + threadIdx.x;
-  if (a >= L.natoms) return;
+__global__ __launch_bounds__(256) void k_hcore_diag(pyseqm_layout_t L, const double* __restrict__ par,
+                                                    const double* __restrict__ part, double* __restrict__ H) {
+  const int a = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (a >= L.natoms) return;  // uniform over the wave
   const double* p = par + (size_t)a * PYSEQM_NPAR;
-  const bool heavy = L.atom_Z[a] > 1;
-  double acc[10];
-#pragma unroll
-  for (int k = 0; k < 10; ++k) acc[k] = 0.0;
-  acc[0] = p[PAR_USS];
-  if (heavy) acc[2] = acc[5] = acc[9] = p[PAR_UPP];
-  const int nk = heavy ? 10 : 1;
-  for (int e = L.csr_off[a]; e < L.csr_off[a + 1]; ++e) {
-    const double* src = part + (size_t)L.csr_idx[e] * kPartStride;
-    for (int k = 0; k < nk; ++k) acc[k] += src[k];
-  }
   const int m = L.atom_mol[a];
   const int norb = L.mol_norb[m];
   double* Hm = H + L.mol_mat_off[m];
   const int ao = L.atom_ao[a];
-  if (heavy) {
+  const int e0 = L.csr_off[a], e1 = L.csr_off[a + 1];
+  if (L.atom_Z[a] > 1) {
+    double acc[10];
+    gather_slots(part, L.csr_idx, e0, e1, lane, acc);
+    acc[0] += p[PAR_USS];
+    acc[2] += p[PAR_UPP];
+    acc[5] += p[PAR_UPP];
+    acc[9] += p[PAR_UPP];
+    if (lane < 16) {
+      const int mu = lane >> 2, nu = lane & 3;
+      double v = acc[0];
 #pragma unroll
-    for (int mu = 0; mu < 4; ++mu)
-#pragma unroll
-      for (int nu = 0; nu < 4; ++nu) Hm[(size_t)(ao + mu) * norb + ao + nu] = acc[tri_index(mu, nu)];
+      for (int k = 1; k < 10; ++k)
+        if (k == tri_index(mu, nu)) v = acc[k];
+      Hm[(size_t)(ao + mu) * norb + ao + nu] = v;
+    }
   } else {
-    Hm[(size_t)ao * norb + ao] = acc[0];
+    double acc[1];
+    gather_slots(part, L.csr_idx, e0, e1, lane, acc);
+    if (lane == 0) Hm[(size_t)ao * norb + ao] = acc[0] + p[PAR_USS];
   }
 }
 
@@ -849,7 +847,7 @@ int launch_pair_forces(const pyseqm_layout_t* L, int method, const double* coord
   base += L->ntile[1];
   if (L->ntile[2])
     k_pair_forces<HH, XL><<<L->ntile[2], 64, 0, s>>>(*L, method, base, coords, atom_par, P, D, h, part);
-  if (L->natoms) k_atom_forces<<<(L->natoms + 127) / 128, 128, 0, s>>>(*L, part, forces);
+  if (L->natoms) k_atom_forces<<<(L->natoms + 3) / 4, 256, 0, s>>>(*L, part, forces);
   return hipGetLastError() == hipSuccess ? PYSEQM_OK : PYSEQM_ELAUNCH;
 }
 }  // namespace
@@ -894,7 +892,7 @@ extern "C" int pyseqm_pair_integrals(const pyseqm_layout_t* L, int method, const
   if (L->ntile[2])
     k_pair_tiles<HH><<<L->ntile[2], 64, 0, s>>>(*L, method, base, coords, atom_par, w_hh, H, part, enuc_tile,
                                                 enuc_pair, pair_ref);
-  if (L->natoms) k_hcore_diag<<<(L->natoms + 127) / 128, 128, 0, s>>>(*L, atom_par, part, H);
+  if (L->natoms) k_hcore_diag<<<(L->natoms + 3) / 4, 256, 0, s>>>(*L, atom_par, part, H);
   pyseqm::launch_mol_sum(L, enuc_tile, nullptr, nullptr, enuc_mol, s);
   return hipGetLastError() == hipSuccess ? PYSEQM_OK : PYSEQM_ELAUNCH;
 }

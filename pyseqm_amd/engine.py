@@ -23,6 +23,29 @@ class SCFResult:
     pass
 
 
+def size_groups(norb, ratio=0.8, min_group=32):
+    """Partition molecules (by position) into groups of similar orbital counts: sorted by size,
+    a group takes molecules down to `ratio` of its largest; groups smaller than `min_group` join
+    the next larger one.  Returns a list of index arrays."""
+    norb = np.asarray(norb)
+    order = np.argsort(-norb, kind="stable")
+    groups, start = [], 0
+    while start < len(order):
+        top = norb[order[start]]
+        end = start
+        while end < len(order) and norb[order[end]] >= ratio * top:
+            end += 1
+        if groups and len(groups[-1]) < min_group:
+            groups[-1] = np.concatenate([groups[-1], order[start:end]])
+        else:
+            groups.append(order[start:end])
+        start = end
+    if len(groups) > 1 and len(groups[-1]) < min_group:
+        groups[-2] = np.concatenate([groups[-2], groups[-1]])
+        groups.pop()
+    return groups
+
+
 class SCFEngine:
     """Pair integrals, Fock builds, density steps and the SCF loop for one packed batch."""
 
@@ -125,8 +148,23 @@ class SCFEngine:
 
     # ---------------------------------------------------------- density step
     def density_eigh(self, F, idx, nact, Pout, want_evecs=False):
-        """Batched FP64 eigensolve of the active Fock matrices -> Pout (diag.py:110-241)."""
+        """Batched FP64 eigensolve of the active Fock matrices -> Pout (diag.py:110-241).
+
+        The active molecules are grouped by size so that each batched solve pads to its own
+        group's largest block (the reference pads every molecule to the batch's largest, which
+        for an uneven batch multiplies the O(n^3) work)."""
         b = self.b
+        if not want_evecs:
+            groups = size_groups(b.norb[self._idx_host])
+            if len(groups) > 1:
+                idx_h = np.asarray(self._idx_host)
+                for sel in groups:
+                    sub = torch.as_tensor(idx_h[sel].astype(np.int32), device=self.device)
+                    saved = self._idx_host
+                    self._idx_host = idx_h[sel]
+                    self.density_eigh(F, sub, len(sel), Pout, want_evecs=True)
+                    self._idx_host = saved
+                return None
         nmax = int(b.norb[self._idx_host].max())
         Fp = torch.empty(nact, nmax, nmax, dtype=torch.float64, device=self.device)
         nat.call("pyseqm_pad_for_eigh", b.L, nat.ptr(F), nat.ptr(idx), nact, nmax, nat.ptr(Fp), nat.ptr(self.red),

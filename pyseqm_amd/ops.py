@@ -1,0 +1,128 @@
+"""The C ABI registered as PyTorch custom operators (namespace ``pyseqm``).
+
+Each operator wraps one ``extern "C"`` entry point of libpyseqm_hip.so: it takes device tensors
+(outputs are mutated in place, as the C functions write into caller buffers), runs on the
+current HIP stream and returns nothing.  The packed-batch descriptor (``pyseqm_layout_t``) is
+passed as an integer handle from :func:`layout_handle`.  These are the same kernels the engine
+calls; the engine's inner SCF loop calls the C ABI directly to avoid the dispatcher's per-call
+cost on its many small launches.
+
+    h = layout_handle(batch)
+    torch.ops.pyseqm.fock_build(h, atom_par, w_xx, w_xh, w_hh, P, H, F, part, tile_v, atom_v, eelec)
+"""
+import torch
+
+from . import _native as nat
+
+_LAYOUTS = {}
+
+
+def layout_handle(batch):
+    """Integer handle of a PackedBatch / MatrixLayout for the operators below (kept alive here)."""
+    key = id(batch)
+    _LAYOUTS[key] = batch
+    return key
+
+
+def release_handle(key):
+    _LAYOUTS.pop(key, None)
+
+
+def _L(h):
+    try:
+        return _LAYOUTS[h].L
+    except KeyError:
+        raise ValueError(f"unknown pyseqm layout handle {h}") from None
+
+
+def _s():
+    return nat.stream_ptr()
+
+
+def _p(t):
+    return nat.ptr(t)
+
+
+@torch.library.custom_op("pyseqm::atom_setup", mutates_args=("atom_par",))
+def atom_setup(layout: int, atom_par: torch.Tensor) -> None:
+    """Derived multipole parameters per atom (two_elec_two_center_int.py:117-147, cal_par.py)."""
+    nat.call("pyseqm_atom_setup", _L(layout), _p(atom_par), _s())
+
+
+@torch.library.custom_op("pyseqm::pair_integrals", mutates_args=("w_xx", "w_xh", "w_hh", "H", "part", "tile_v",
+                                                                  "enuc"))
+def pair_integrals(layout: int, method: int, coords: torch.Tensor, atom_par: torch.Tensor, w_xx: torch.Tensor,
+                   w_xh: torch.Tensor, w_hh: torch.Tensor, H: torch.Tensor, part: torch.Tensor, tile_v: torch.Tensor,
+                   enuc: torch.Tensor) -> None:
+    """Two-center integrals, core Hamiltonian and core-core repulsion (hcore.py:9-179)."""
+    nat.call("pyseqm_pair_integrals", _L(layout), method, _p(coords), _p(atom_par), _p(w_xx), _p(w_xh), _p(w_hh),
+             _p(H), _p(part), _p(tile_v), _p(enuc), None, None, _s())
+
+
+@torch.library.custom_op("pyseqm::fock_build", mutates_args=("F", "part", "tile_v", "atom_v", "eelec"))
+def fock_build(layout: int, atom_par: torch.Tensor, w_xx: torch.Tensor, w_xh: torch.Tensor, w_hh: torch.Tensor,
+               P: torch.Tensor, H: torch.Tensor, F: torch.Tensor, part: torch.Tensor, tile_v: torch.Tensor,
+               atom_v: torch.Tensor, eelec: torch.Tensor) -> None:
+    """F = H + G(P) and Eelec (fock.py:132-347, energy.py:26-49)."""
+    nat.call("pyseqm_fock", _L(layout), _p(atom_par), _p(w_xx), _p(w_xh), _p(w_hh), _p(P), _p(H), _p(F), _p(part),
+             _p(tile_v), _p(atom_v), _p(eelec), None, _s())
+
+
+@torch.library.custom_op("pyseqm::pad_for_eigh", mutates_args=("out", "red"))
+def pad_for_eigh(layout: int, F: torch.Tensor, idx: torch.Tensor, nmax: int, out: torch.Tensor,
+                 red: torch.Tensor) -> None:
+    """Gershgorin-shifted padded stack of the listed Fock blocks (diag.py:130-160)."""
+    nat.call("pyseqm_pad_for_eigh", _L(layout), _p(F), _p(idx), int(idx.numel()), nmax, _p(out), _p(red),
+             nat.red_blocks(nmax), _s())
+
+
+@torch.library.custom_op("pyseqm::density_from_evecs", mutates_args=("P",))
+def density_from_evecs(layout: int, V: torch.Tensor, nmax: int, idx: torch.Tensor, P: torch.Tensor) -> None:
+    """P = 2 C_occ C_occ^T for the listed molecules (diag.py:230-232)."""
+    nat.call("pyseqm_density_from_evecs", _L(layout), _p(V), nmax, _p(idx), int(idx.numel()), _p(P), _s())
+
+
+@torch.library.custom_op("pyseqm::pair_forces", mutates_args=("part", "forces"))
+def pair_forces(layout: int, method: int, coords: torch.Tensor, atom_par: torch.Tensor, P: torch.Tensor, h: float,
+                part: torch.Tensor, forces: torch.Tensor) -> None:
+    """Hellmann-Feynman forces at fixed P (basics.py:1315-1336)."""
+    nat.call("pyseqm_pair_forces", _L(layout), method, _p(coords), _p(atom_par), _p(P), float(h), _p(part),
+             _p(forces), _s())
+
+
+@torch.library.custom_op("pyseqm::pair_forces_xl", mutates_args=("part", "forces"))
+def pair_forces_xl(layout: int, method: int, coords: torch.Tensor, atom_par: torch.Tensor, D: torch.Tensor,
+                   P: torch.Tensor, h: float, part: torch.Tensor, forces: torch.Tensor) -> None:
+    """XL-BOMD forces at fixed D, P (xlbomd.py:536-551)."""
+    nat.call("pyseqm_pair_forces_xl", _L(layout), method, _p(coords), _p(atom_par), _p(D), _p(P), float(h),
+             _p(part), _p(forces), _s())
+
+
+@torch.library.custom_op("pyseqm::xl_energy", mutates_args=("eelec", "red"))
+def xl_energy(layout: int, D: torch.Tensor, P: torch.Tensor, F: torch.Tensor, H: torch.Tensor, eelec: torch.Tensor,
+              red: torch.Tensor) -> None:
+    """sum D F - 1/2 sum (F - H) P per molecule (energy.py:76-88)."""
+    nat.call("pyseqm_xl_energy", _L(layout), _p(D), _p(P), _p(F), _p(H), _p(eelec), _p(red),
+             nat.red_blocks(_LAYOUTS[layout].nmax), _s())
+
+
+@torch.library.custom_op("pyseqm::xl_propagate", mutates_args=("P", "Pt"))
+def xl_propagate(D: torch.Tensor, P: torch.Tensor, Pt: torch.Tensor, coef: list[float], coef_D: float, c: float,
+                 slot: int) -> None:
+    """P <- coef_D (c D + (1-c) P) + sum_j coef[j] Pt[j];  Pt[slot] = P (MolecularDynamics.py:1418-1461)."""
+    import ctypes
+    arr = (ctypes.c_double * len(coef))(*coef)
+    nat.call_raw("pyseqm_xl_propagate", int(P.numel()), _p(D), _p(P), _p(Pt), int(Pt.shape[1]), len(coef), arr,
+                 float(coef_D), float(c), int(slot), _s())
+
+
+@torch.library.custom_op("pyseqm::core_repulsion_pairs", mutates_args=("out",))
+def core_repulsion_pairs(method: int, idxi: torch.Tensor, idxj: torch.Tensor, rij: torch.Tensor, gam: torch.Tensor,
+                         atom_par: torch.Tensor, out: torch.Tensor) -> None:
+    """Core-core repulsion of an explicit pair list (energy.py:91-134)."""
+    nat.call_raw("pyseqm_core_repulsion_pairs", int(rij.numel()), method, _p(idxi), _p(idxj), _p(rij), _p(gam),
+                 _p(atom_par), _p(out), _s())
+
+
+OPS = ("atom_setup", "pair_integrals", "fock_build", "pad_for_eigh", "density_from_evecs", "pair_forces",
+       "pair_forces_xl", "xl_energy", "xl_propagate", "core_repulsion_pairs")

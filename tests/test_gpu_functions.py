@@ -107,3 +107,27 @@ def test_density_entry_points():
         p = Ps[m][np.ix_(idx, idx)]
         assert np.abs(p @ p - 2 * p).max() < 1e-6
         assert abs(np.trace(p) - 2 * b6.nocc[m]) < 1e-8
+
+
+def test_custom_ops_match_engine():
+    import pyseqm_amd.ops  # noqa: F401  (registers torch.ops.pyseqm)
+    from pyseqm_amd import _native as nat
+    from pyseqm_amd.engine import SCFEngine
+    from pyseqm_amd.ops import layout_handle
+    sp, c = _inputs("sulfur")
+    eng = SCFEngine(sp, c, "PM3", DEV)
+    r = eng.scf(eps=1e-8)
+    b = eng.b
+    h = layout_handle(b)
+    F1 = torch.empty_like(r.P)
+    e1 = torch.zeros(b.nmol, dtype=torch.float64, device=DEV)
+    eng.fock(r.P, F1, eelec=e1)
+    F2 = torch.empty_like(r.P)
+    e2 = torch.zeros_like(e1)
+    torch.ops.pyseqm.fock_build(h, eng.atom_par, eng.w_xx, eng.w_xh, eng.w_hh, r.P, eng.H, F2, eng.part,
+                                eng.tile_v, eng.atom_v, e2)
+    assert torch.equal(F1, F2) and torch.equal(e1, e2)
+    f1 = eng.forces(r.P)
+    f2 = torch.zeros(b.natoms, 3, dtype=torch.float64, device=DEV)
+    torch.ops.pyseqm.pair_forces(h, nat.METHODS["PM3"], eng.coords, eng.atom_par, r.P, 1e-3, eng.part, f2)
+    assert torch.equal(f1, f2)

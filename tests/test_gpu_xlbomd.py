@@ -75,8 +75,62 @@ def test_xl_energy_and_forces_match_oracle(name, method):
     np.testing.assert_allclose(f, fo, atol=1e-6, rtol=0)
 
 
+def _oracle_xl(sp, x, v, method, k, nsteps, eps=1e-7, dt=0.5):
+    """XL-BOMD on the oracle: SCF seed density, then propagation, eigh density, XL energy and
+    finite-difference XL forces per step (MolecularDynamics.py:1437-1515)."""
+    from pyseqm_amd.xlbomd import ACC_SCALE, KINETIC_ENERGY_SCALE
+    mass = R.MASS[sp][..., None]
+    r = R.single_point(sp, x, method, eps=eps, converger=(1,))
+    P = r["P"].copy()
+    a = R.forces_fd(sp, x, method, P) / mass * ACC_SCALE
+    Pt = np.repeat(P[None], k + 1, axis=0)
+    D = P.copy()
+    ek = lambda v: float((0.5 * mass * v * v).sum() * KINETIC_ENERGY_SCALE)
+    out = [(ek(v), r["Etot"][0] + ek(v))]
+    for step in range(nsteps):
+        v = v + 0.5 * a * dt
+        x = x + v * dt
+        P, Pt = R.xl_propagate(D, P, Pt, k, step)
+        b = R.parse(sp, x, method)
+        H, Tt, _, _, _ = R.hcore(b)
+        _, D, _ = R.density_eigh(b, R.fock(b, P, H, Tt))
+        E = R.energy_xl_fixed(sp, x, method, D, P)[0]
+        a = R.forces_fd(sp, x, method, P, D=D) / mass * ACC_SCALE
+        v = v + 0.5 * a * dt
+        out.append((ek(v), E + ek(v)))
+    return np.array(out)
+
+
+@pytest.mark.parametrize("k", [6, 4])
+def test_xl_bomd_trajectory_matches_oracle(k):
+    """Same initial velocities on both sides: Ek(t) and Etot(t) of the device trajectory equal the
+    oracle's step by step (1e-6 eV)."""
+    sp, c = read_xyz(["methane.xyz"])
+    rng = np.random.default_rng(5)
+    v0 = rng.normal(scale=0.01, size=c.shape)
+    v0[sp == 0] = 0.0
+    v0 -= v0.mean(axis=1, keepdims=True) * (sp > 0)[..., None]
+    p = {"method": "AM1", "scf_eps": 1.0e-7, "scf_converger": [1]}
+    const = pyseqm_amd.Constants().to(DEV)
+    mol = pyseqm_amd.Molecule(const, p, torch.as_tensor(c, device=DEV), torch.as_tensor(sp, device=DEV))
+    md = pyseqm_amd.XL_BOMD(xl_bomd_params={"k": k}, damp=None, seqm_parameters=p, timestep=0.5, Temp=300.0)
+    md.set_dof(mol)
+    mol.velocities = torch.as_tensor(v0, device=DEV)
+    md._zero_com(mol, translate_to_origin=False, restore_kinetic_energy=False)
+    v_start = mol.velocities.cpu().numpy().copy()
+    md.run(mol, steps=6, reuse_P=True, remove_com=None)
+    T, Ek, Ep, Etot = md.thermo()
+    ref = _oracle_xl(sp, c, v_start, "AM1", k, 6)
+    np.testing.assert_allclose(Ek[:, 0], ref[:, 0], atol=1e-6, rtol=0)
+    np.testing.assert_allclose(Etot[:, 0], ref[:, 1], atol=1e-6, rtol=0)
+
+
 @pytest.mark.parametrize("k,gname", [(6, "md_xl_bomd_methane"), (4, "md_xl_bomd_methane_k4")])
 def test_xl_bomd_methane_trajectory_matches_golden(k, gname):
+    """The reference's own MD test (tests/unit/test_md_suite.py:218-259): 30 steps from
+    Maxwell-Boltzmann velocities drawn on the device with seed 0; Etot(t) within the reference's
+    1e-2, drift within 5e-2.  Step 0 (SCF energy + the kinetic energy fixed by T = 300 K) is
+    compared tightly."""
     torch.manual_seed(0)
     sp, c = read_xyz(["methane.xyz"])
     p = {"method": "AM1", "scf_eps": 1.0e-7, "scf_converger": [1]}
@@ -86,8 +140,10 @@ def test_xl_bomd_methane_trajectory_matches_golden(k, gname):
     md.run(mol, steps=30, reuse_P=True, remove_com=None, seed=0)
     T, Ek, Ep, Etot = md.thermo()
     g = golden(gname)[0]
-    print("max |dEtot| vs golden:", np.abs(Etot[:, 0] - np.array(g["Etot"])).max())
-    np.testing.assert_allclose(Ek[:, 0], g["Ek"], atol=1e-5, rtol=0)
-    np.testing.assert_allclose(Etot[:, 0], g["Etot"], atol=1e-5, rtol=0)
+    print("max |dEtot| vs golden:", np.abs(Etot[:, 0] - np.array(g["Etot"])).max(),
+          "max |dEk|:", np.abs(Ek[:, 0] - np.array(g["Ek"])).max())
+    assert abs(Etot[0, 0] - g["Etot"][0]) < 1e-8
+    np.testing.assert_allclose(Etot[:, 0], g["Etot"], atol=1e-2, rtol=1e-2)
     drift = np.abs(Etot[:, 0] - Etot[0, 0]).max()
     assert drift <= 5e-2
+    assert Ek.min() >= 0.0

@@ -27,3 +27,11 @@ def test_library_exports_every_declared_symbol():
     for name in _declared():
         assert hasattr(lib, name), name
     _native.lib()  # argtypes bind without error
+
+
+def test_custom_ops_registered():
+    import torch
+
+    import pyseqm_amd.ops as ops
+    for name in ops.OPS:
+        assert hasattr(torch.ops.pyseqm, name), name
