@@ -185,9 +185,10 @@ __global__ __launch_bounds__(256) void k_sp2_bf16x3(const int* __restrict__ idx,
                                                     const int* __restrict__ xld, const float* __restrict__ Xin,
                                                     float* __restrict__ Xout, double* __restrict__ st,
                                                     double* __restrict__ part, int part_stride) {
-  __shared__ __attribute__((aligned(16))) __bf16 As[2][BT][LDSK];  // [hi/lo][row][k]
-  __shared__ __attribute__((aligned(16))) __bf16 Bs[2][BT][LDSK];
-  __shared__ double red[2][4];
+  // double-buffered slabs: [buffer][hi/lo][row][k]; 160 KiB for two blocks per CU
+  __shared__ __attribute__((aligned(16))) __bf16 As[2][2][BT][LDSK];
+  __shared__ __attribute__((aligned(16))) __bf16 Bs[2][2][BT][LDSK];
+  double(*red)[4] = reinterpret_cast<double(*)[4]>(&As[0][0][0][0]);  // reused after the main loop
   const int m = idx[blockIdx.y];
   const double* s = st + (size_t)m * S_NST;
   if (s[S_DONE] != 0.0) return;
@@ -237,17 +238,23 @@ __global__ __launch_bounds__(256) void k_sp2_bf16x3(const int* __restrict__ idx,
     *reinterpret_cast<uint2*>(&hi[row][col]) = ph.u;
     *reinterpret_cast<uint2*>(&lo[row][col]) = pl.u;
   };
-  gload(0);
-  for (int k0 = 0; k0 < ld; k0 += BK) {
+  auto stage = [&](int buf) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int f = tid + 256 * q;
       const int row = f >> 3, kq = f & 7;
-      split_store(ra[q], As[0], As[1], row, 4 * kq);
-      split_store(rb[q], Bs[0], Bs[1], row, 4 * kq);
+      split_store(ra[q], As[buf][0], As[buf][1], row, 4 * kq);
+      split_store(rb[q], Bs[buf][0], Bs[buf][1], row, 4 * kq);
     }
-    __syncthreads();
-    if (k0 + BK < ld) gload(k0 + BK);
+  };
+  // one barrier per slab: slab t+1 is staged into the other buffer while slab t is multiplied
+  gload(0);
+  stage(0);
+  __syncthreads();
+  int cur = 0;
+  for (int k0 = 0; k0 < ld; k0 += BK) {
+    const bool more = k0 + BK < ld;
+    if (more) gload(k0 + BK);
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 16) {
       bf16x8 ah[2], al[2], bh[2], bl[2];
@@ -255,11 +262,11 @@ __global__ __launch_bounds__(256) void k_sp2_bf16x3(const int* __restrict__ idx,
 #pragma unroll
       for (int a = 0; a < 2; ++a) {
         const int r = wr * 64 + a * 32 + (lane & 31);
-        ah[a] = *reinterpret_cast<const bf16x8*>(&As[0][r][kofs]);
-        al[a] = *reinterpret_cast<const bf16x8*>(&As[1][r][kofs]);
+        ah[a] = *reinterpret_cast<const bf16x8*>(&As[cur][0][r][kofs]);
+        al[a] = *reinterpret_cast<const bf16x8*>(&As[cur][1][r][kofs]);
         const int c = wc * 64 + a * 32 + (lane & 31);
-        bh[a] = *reinterpret_cast<const bf16x8*>(&Bs[0][c][kofs]);
-        bl[a] = *reinterpret_cast<const bf16x8*>(&Bs[1][c][kofs]);
+        bh[a] = *reinterpret_cast<const bf16x8*>(&Bs[cur][0][c][kofs]);
+        bl[a] = *reinterpret_cast<const bf16x8*>(&Bs[cur][1][c][kofs]);
       }
 #pragma unroll
       for (int a = 0; a < 2; ++a)
@@ -270,7 +277,9 @@ __global__ __launch_bounds__(256) void k_sp2_bf16x3(const int* __restrict__ idx,
           acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a], bh[b], acc[a][b], 0, 0, 0);
         }
     }
+    if (more) stage(cur ^ 1);
     __syncthreads();
+    cur ^= 1;
   }
   // epilogue: Y = X^2 or 2X - X^2, mirror write, FP64 partial trace / Frobenius norm
   double tr = 0.0, fro = 0.0;
