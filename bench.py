@@ -128,7 +128,7 @@ def serve(args, dev):
         "latency_ms": percentiles(lat_all), "latency_ms_completed": percentiles(lat),
         "batches": srv.batches, "lanes": args.serve_lanes, "hbm_budget_GiB": srv.hbm_budget / 2**30,
         "peak_in_flight_GiB": srv.peak_in_use / 2**30, "density": "sp2 (bf16x3 + f64 McWeeny)",
-        "eps_eV": args.serve_eps, "mix": "50% AM1 alkanes n~U[2,30], 50% PM3 (H2O)_N N~U[4,%d]" % args.serve_water_max,
+        "eps_eV": args.serve_eps, "mix": f"50% AM1 alkanes n~U[2,30], 50% PM3 (H2O)_N N~U[4,{args.serve_water_max}]",
     }
     if xl is not None:
         n = len(xl.latencies)
@@ -157,6 +157,17 @@ def sp2_roofline(eng, step):
             "traffic": None, "algorithmic_flops": flops, "gemm_loop_ms": t_ms,
             "note": "peak = dense bf16 2.5 PFLOP/s / 3 products (bf16x3 split); flops = 2 n^3 per SP2 iteration "
                     "per active molecule; time = SP2 iteration loops incl. the fused trace/branch epilogue"}
+
+
+def measured_traffic(config):
+    """Calibrated HBM bytes per Fock build from the committed rocprofv3 PMC summary, if present."""
+    path = os.path.join(ROOT, "profiles", "fock_pmc_traffic.json")
+    try:
+        with open(path) as fh:
+            d = json.load(fh)
+        return d[config]["traffic_bytes"]
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def fock_roofline(eng, P, reps):
@@ -304,6 +315,11 @@ def main():
                    "parallelism": "single GPU, batch over molecules"},
         "roofline": fock_roofline(eng, r.P, args.fock_reps),
     }
+    traffic = measured_traffic(args.config)
+    if traffic is not None:
+        line["roofline"]["traffic"] = traffic
+        line["roofline"]["traffic_source"] = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE per Fock build, calibrated "
+                                              "by a 1 GiB copy (profiles/fock_hbm_traffic.md)")
     if not args.no_compare and args.density == "eigh":
         e2 = SCFEngine(sp, None, method, dev)
         step2 = make_step(e2, "sp2", args.sp2_eps)

@@ -78,20 +78,37 @@ __device__ inline void write_partials(double* part, int tile, int lane, const do
   }
 }
 
-// Sum of an atom's tile partial slots csr_idx[e0..e1): lane l adds slots e0 + l, e0 + l + 64, ...
-// in order, then a fixed shuffle tree; every lane ends with the totals (deterministic).
-template <int N>
-__device__ inline void gather_slots(const double* part, const int* csr_idx, int e0, int e1, int lane,
+// Sum over the G lanes of an aligned lane group (G = 16 or 64), every lane gets the total.
+template <int G>
+__device__ inline double group_sum(double v) {
+#pragma unroll
+  for (int o = 1; o < G; o <<= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+// Sum of an atom's tile partial slots csr_idx[e0..e1) by a group of G lanes: group lane l adds
+// slots e0 + l, e0 + l + G, ... in order, then a fixed shuffle tree; every lane of the group ends
+// with the totals (deterministic).
+template <int G, int N>
+__device__ inline void gather_slots(const double* part, const int* csr_idx, int e0, int e1, int gl,
                                     double (&acc)[N]) {
 #pragma unroll
   for (int k = 0; k < N; ++k) acc[k] = 0.0;
-  for (int e = e0 + lane; e < e1; e += 64) {
+  for (int e = e0 + gl; e < e1; e += G) {
     const double* src = part + (size_t)csr_idx[e] * kPartStride;
 #pragma unroll
     for (int k = 0; k < N; ++k) acc[k] += src[k];
   }
 #pragma unroll
-  for (int k = 0; k < N; ++k) acc[k] = wave_sum(acc[k]);
+  for (int k = 0; k < N; ++k) acc[k] = group_sum<G>(acc[k]);
+}
+
+// Lanes per atom of the per-atom gathers: 64 when atoms carry many partial slots (large
+// molecules), 16 otherwise (average slots per atom = 16 ntiles / natoms, at most 10 per atom
+// for small molecules).
+inline int lanes_per_atom(const pyseqm_layout_t* L) {
+  const long long nt = (long long)L->ntile[0] + L->ntile[1] + L->ntile[2];
+  return (L->natoms > 0 && 16 * nt > 32LL * L->natoms) ? 64 : 16;
 }
 
 }  // namespace pyseqm

@@ -260,14 +260,15 @@ __global__ __launch_bounds__(128) void k_fock_xx2(pyseqm_layout_t L, const doubl
 
 // Diagonal blocks: H_ii + one-center G (fock.py:196-224) + Coulomb partials; energy share.
 // One wave per atom: the lanes split the atom's partial slots, then lanes 0..15 write the block.
+template <int G>
 __global__ __launch_bounds__(256) void k_fock_diag(pyseqm_layout_t L, const double* __restrict__ par,
                                                    const double* __restrict__ P, const double* __restrict__ H,
                                                    const double* __restrict__ part, double* __restrict__ F,
                                                    double* __restrict__ eatom,
                                                    const unsigned char* __restrict__ active) {
-  const int a = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (a >= L.natoms) return;  // uniform over the wave
+  const int a = blockIdx.x * (256 / G) + threadIdx.x / G;
+  const int lane = threadIdx.x % G;
+  if (a >= L.natoms) return;  // uniform over the lane group
   const int m = L.atom_mol[a];
   if (active && !active[m]) return;
   const int n = L.mol_norb[m];
@@ -280,7 +281,7 @@ __global__ __launch_bounds__(256) void k_fock_diag(pyseqm_layout_t L, const doub
   const int e0 = L.csr_off[a], e1 = L.csr_off[a + 1];
   if (L.atom_Z[a] > 1) {
     double acc[10], pd[10], hd[10];
-    gather_slots(part, L.csr_idx, e0, e1, lane, acc);
+    gather_slots<G>(part, L.csr_idx, e0, e1, lane, acc);
 #pragma unroll
     for (int k = 0; k < 10; ++k) {
       pd[k] = Pm[(size_t)(ao + kTriI[k]) * n + ao + kTriJ[k]];
@@ -317,7 +318,7 @@ __global__ __launch_bounds__(256) void k_fock_diag(pyseqm_layout_t L, const doub
     }
   } else {
     double acc[1];
-    gather_slots(part, L.csr_idx, e0, e1, lane, acc);
+    gather_slots<G>(part, L.csr_idx, e0, e1, lane, acc);
     if (lane == 0) {
       const double pd = Pm[(size_t)ao * n + ao], hd = Hm[(size_t)ao * n + ao];
       const double f = acc[0] + hd + 0.5 * pd * p[PAR_GSS];
@@ -341,7 +342,12 @@ extern "C" int pyseqm_fock(const pyseqm_layout_t* L, const double* atom_par, con
   if (L->ntile[1]) k_fock_tiles<XH><<<L->ntile[1], 64, 0, s>>>(*L, base, w_xh, P, H, F, part, etile, active);
   base += L->ntile[1];
   if (L->ntile[2]) k_fock_tiles<HH><<<L->ntile[2], 64, 0, s>>>(*L, base, w_hh, P, H, F, part, etile, active);
-  if (L->natoms) k_fock_diag<<<(L->natoms + 3) / 4, 256, 0, s>>>(*L, atom_par, P, H, part, F, eatom, active);
+  if (L->natoms) {
+    if (lanes_per_atom(L) == 64)
+      k_fock_diag<64><<<(L->natoms + 3) / 4, 256, 0, s>>>(*L, atom_par, P, H, part, F, eatom, active);
+    else
+      k_fock_diag<16><<<(L->natoms + 15) / 16, 256, 0, s>>>(*L, atom_par, P, H, part, F, eatom, active);
+  }
   pyseqm::launch_mol_sum(L, etile, eatom, active, eelec, s);
   return hipGetLastError() == hipSuccess ? PYSEQM_OK : PYSEQM_ELAUNCH;
 }

@@ -736,23 +736,25 @@ __global__ __launch_bounds__(64) void k_pair_forces(pyseqm_layout_t L, int metho
   write_partials(part, tile, lane, rowv, colv);
 }
 
+template <int G>
 __global__ __launch_bounds__(256) void k_atom_forces(pyseqm_layout_t L, const double* __restrict__ part,
                                                      double* __restrict__ F) {
-  const int a = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (a >= L.natoms) return;  // uniform over the wave
+  const int a = blockIdx.x * (256 / G) + threadIdx.x / G;
+  const int lane = threadIdx.x % G;
+  if (a >= L.natoms) return;  // uniform over the lane group
   double f[3];
-  gather_slots(part, L.csr_idx, L.csr_off[a], L.csr_off[a + 1], lane, f);
+  gather_slots<G>(part, L.csr_idx, L.csr_off[a], L.csr_off[a + 1], lane, f);
   if (lane < 3) F[3 * a + lane] = lane == 0 ? f[0] : (lane == 1 ? f[1] : f[2]);
 }
 
 // Diagonal core-Hamiltonian blocks: U on the diagonal plus the electron-core attractions
 // collected from every tile the atom appears in (hcore.py:127-150), fixed summation order.
+template <int G>
 __global__ __launch_bounds__(256) void k_hcore_diag(pyseqm_layout_t L, const double* __restrict__ par,
                                                     const double* __restrict__ part, double* __restrict__ H) {
-  const int a = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (a >= L.natoms) return;  // uniform over the wave
+  const int a = blockIdx.x * (256 / G) + threadIdx.x / G;
+  const int lane = threadIdx.x % G;
+  if (a >= L.natoms) return;  // uniform over the lane group
   const double* p = par + (size_t)a * PYSEQM_NPAR;
   const int m = L.atom_mol[a];
   const int norb = L.mol_norb[m];
@@ -761,7 +763,7 @@ __global__ __launch_bounds__(256) void k_hcore_diag(pyseqm_layout_t L, const dou
   const int e0 = L.csr_off[a], e1 = L.csr_off[a + 1];
   if (L.atom_Z[a] > 1) {
     double acc[10];
-    gather_slots(part, L.csr_idx, e0, e1, lane, acc);
+    gather_slots<G>(part, L.csr_idx, e0, e1, lane, acc);
     acc[0] += p[PAR_USS];
     acc[2] += p[PAR_UPP];
     acc[5] += p[PAR_UPP];
@@ -776,27 +778,38 @@ __global__ __launch_bounds__(256) void k_hcore_diag(pyseqm_layout_t L, const dou
     }
   } else {
     double acc[1];
-    gather_slots(part, L.csr_idx, e0, e1, lane, acc);
+    gather_slots<G>(part, L.csr_idx, e0, e1, lane, acc);
     if (lane == 0) Hm[(size_t)ao * norb + ao] = acc[0] + p[PAR_USS];
   }
 }
 
-// Per-molecule sum of per-tile values (and optionally per-atom values): one wave per molecule,
-// lanes stride over the molecule's tiles / atoms, fixed lane assignment and shuffle tree.
-__global__ void k_mol_sum(pyseqm_layout_t L, const double* __restrict__ tile_v, const double* __restrict__ atom_v,
-                          const unsigned char* __restrict__ active, double* __restrict__ out) {
+// Per-molecule sum of per-tile values (and optionally per-atom values): one block per molecule,
+// threads stride over the molecule's tiles / atoms, fixed thread assignment, wave shuffle trees and
+// a fixed-order sum of the wave totals.
+template <int NT>
+__global__ __launch_bounds__(NT) void k_mol_sum(pyseqm_layout_t L, const double* __restrict__ tile_v,
+                                                const double* __restrict__ atom_v,
+                                                const unsigned char* __restrict__ active, double* __restrict__ out) {
+  __shared__ double wsum[NT / 64];
   const int m = blockIdx.x;
   if (active && !active[m]) return;
-  const int lane = threadIdx.x;
+  const int tid = threadIdx.x;
   double s = 0.0;
   for (int kind = 0; kind < 3; ++kind) {
     const int* off = L.mol_tile_off + kind * (L.nmol + 1);
-    for (int t = off[m] + lane; t < off[m + 1]; t += 64) s += tile_v[t];
+    for (int t = off[m] + tid; t < off[m + 1]; t += NT) s += tile_v[t];
   }
   if (atom_v)
-    for (int a = L.mol_atom_off[m] + lane; a < L.mol_atom_off[m + 1]; a += 64) s += atom_v[a];
+    for (int a = L.mol_atom_off[m] + tid; a < L.mol_atom_off[m + 1]; a += NT) s += atom_v[a];
   s = wave_sum(s);
-  if (lane == 0) out[m] = s;
+  if ((tid & 63) == 0) wsum[tid >> 6] = s;
+  __syncthreads();
+  if (tid == 0) {
+    double r = 0.0;
+#pragma unroll
+    for (int w = 0; w < NT / 64; ++w) r += wsum[w];
+    out[m] = r;
+  }
 }
 
 }  // namespace
@@ -804,7 +817,12 @@ __global__ void k_mol_sum(pyseqm_layout_t L, const double* __restrict__ tile_v, 
 namespace pyseqm {
 void launch_mol_sum(const pyseqm_layout_t* L, const double* tile_v, const double* atom_v,
                     const unsigned char* active, double* out, hipStream_t s) {
-  if (L->nmol > 0) k_mol_sum<<<L->nmol, 64, 0, s>>>(*L, tile_v, atom_v, active, out);
+  if (L->nmol <= 0) return;
+  const long long nt = (long long)L->ntile[0] + L->ntile[1] + L->ntile[2];
+  if (nt > 2048LL * L->nmol)
+    k_mol_sum<1024><<<L->nmol, 1024, 0, s>>>(*L, tile_v, atom_v, active, out);
+  else
+    k_mol_sum<64><<<L->nmol, 64, 0, s>>>(*L, tile_v, atom_v, active, out);
 }
 }  // namespace pyseqm
 
@@ -847,7 +865,12 @@ int launch_pair_forces(const pyseqm_layout_t* L, int method, const double* coord
   base += L->ntile[1];
   if (L->ntile[2])
     k_pair_forces<HH, XL><<<L->ntile[2], 64, 0, s>>>(*L, method, base, coords, atom_par, P, D, h, part);
-  if (L->natoms) k_atom_forces<<<(L->natoms + 3) / 4, 256, 0, s>>>(*L, part, forces);
+  if (L->natoms) {
+    if (lanes_per_atom(L) == 64)
+      k_atom_forces<64><<<(L->natoms + 3) / 4, 256, 0, s>>>(*L, part, forces);
+    else
+      k_atom_forces<16><<<(L->natoms + 15) / 16, 256, 0, s>>>(*L, part, forces);
+  }
   return hipGetLastError() == hipSuccess ? PYSEQM_OK : PYSEQM_ELAUNCH;
 }
 }  // namespace
@@ -892,7 +915,12 @@ extern "C" int pyseqm_pair_integrals(const pyseqm_layout_t* L, int method, const
   if (L->ntile[2])
     k_pair_tiles<HH><<<L->ntile[2], 64, 0, s>>>(*L, method, base, coords, atom_par, w_hh, H, part, enuc_tile,
                                                 enuc_pair, pair_ref);
-  if (L->natoms) k_hcore_diag<<<(L->natoms + 3) / 4, 256, 0, s>>>(*L, atom_par, part, H);
+  if (L->natoms) {
+    if (lanes_per_atom(L) == 64)
+      k_hcore_diag<64><<<(L->natoms + 3) / 4, 256, 0, s>>>(*L, atom_par, part, H);
+    else
+      k_hcore_diag<16><<<(L->natoms + 15) / 16, 256, 0, s>>>(*L, atom_par, part, H);
+  }
   pyseqm::launch_mol_sum(L, enuc_tile, nullptr, nullptr, enuc_mol, s);
   return hipGetLastError() == hipSuccess ? PYSEQM_OK : PYSEQM_ELAUNCH;
 }

@@ -47,6 +47,27 @@ def _region(mol, a0, r_start, nr, c_start, nc, tri):
     return t[keep]
 
 
+def _regions(a0, r_start, nr, c_start, nc, tri):
+    """_region for every molecule at once (same tiles, same molecule-major / row-major order).
+    Arrays are per molecule; returns (tiles (N, 6), tiles per molecule)."""
+    nmol = len(nr)
+    nbr, nbc = -(-nr // TILE), -(-nc // TILE)
+    cnt = nbr * nbc
+    mol = np.repeat(np.arange(nmol), cnt)
+    k = np.arange(int(cnt.sum())) - np.repeat(np.cumsum(cnt) - cnt, cnt)
+    bi = k // np.maximum(nbc[mol], 1)
+    bj = k % np.maximum(nbc[mol], 1)
+    keep = (bj >= bi) if tri else np.ones(len(k), bool)
+    mol, bi, bj = mol[keep], bi[keep], bj[keep]
+    nrow = np.minimum(TILE, nr[mol] - TILE * bi)
+    ncol = np.minimum(TILE, nc[mol] - TILE * bj)
+    diag = (bi == bj) & tri
+    keep = ~(diag & (nrow < 2))
+    t = np.stack([mol, a0[mol] + r_start[mol] + TILE * bi, a0[mol] + c_start[mol] + TILE * bj, nrow, ncol,
+                  diag.astype(np.int64)], 1)[keep]
+    return t.astype(np.int64), np.bincount(mol[keep], minlength=nmol)
+
+
 class MatrixLayout:
     """Layout of nmol dense n x n blocks stored back to back (a (nmol, n, n) tensor), for the
     matrix-only kernels (density steps, energies) on inputs given in the reference's dense
@@ -124,18 +145,16 @@ class PackedBatch:
     # ------------------------------------------------------------------ tiles
     def _build_tiles(self):
         kinds = []
+        a0 = self.mol_atom_off[:-1].astype(np.int64)
+        nh, nH = self.nheavy.astype(np.int64), self.nhydro.astype(np.int64)
+        zero = np.zeros_like(nh)
         for kind in range(3):
-            per_mol = []
-            for m in range(self.nmol):
-                a0, nh, nH = int(self.mol_atom_off[m]), int(self.nheavy[m]), int(self.nhydro[m])
-                if kind == 0:
-                    per_mol.append(_region(m, a0, 0, nh, 0, nh, True))
-                elif kind == 1:
-                    per_mol.append(_region(m, a0, 0, nh, nh, nH, False))
-                else:
-                    per_mol.append(_region(m, a0, nh, nH, nh, nH, True))
-            counts = np.array([len(t) for t in per_mol], dtype=np.int64)
-            tiles = np.concatenate(per_mol) if len(per_mol) else np.zeros((0, 6), np.int64)
+            if kind == 0:
+                tiles, counts = _regions(a0, zero, nh, zero, nh, True)
+            elif kind == 1:
+                tiles, counts = _regions(a0, zero, nh, nh, nH, False)
+            else:
+                tiles, counts = _regions(a0, nh, nH, nh, nH, True)
             kinds.append((tiles, np.concatenate([[0], np.cumsum(counts)])))
         self.ntile = [len(k[0]) for k in kinds]
         base = np.cumsum([0] + self.ntile)

@@ -111,3 +111,18 @@ def test_pack_unpack_match_orbital_selection():
         mask = np.ones((n, n), bool)
         mask[np.ix_(idx, idx)] = False
         assert not Xu[m][mask].any()
+
+
+def test_vectorised_tile_regions_match_per_molecule_construction():
+    from pyseqm_amd.layout import _region, _regions
+    rng = np.random.default_rng(4)
+    nmol = 40
+    nh = rng.integers(0, 30, nmol)
+    nH = rng.integers(0, 40, nmol)
+    a0 = np.concatenate([[0], np.cumsum(nh + nH)])[:-1]
+    zero = np.zeros(nmol, np.int64)
+    for args, tri in (((zero, nh, zero, nh), True), ((zero, nh, nh, nH), False), ((nh, nH, nh, nH), True)):
+        t, cnt = _regions(a0, *args, tri)
+        ref = [_region(m, a0[m], args[0][m], args[1][m], args[2][m], args[3][m], tri) for m in range(nmol)]
+        np.testing.assert_array_equal(t, np.concatenate(ref))
+        np.testing.assert_array_equal(cnt, [len(r) for r in ref])
