@@ -153,11 +153,18 @@ int pyseqm_gemm(const pyseqm_layout_t* L, const int* idx, int nact, int nmax, co
 int pyseqm_dgemm_sym(const pyseqm_layout_t* L, const int* idx, int nact, int nmax, const long long* off,
                      const int* lds, const double* A, const double* B, double* C, int mode, void* stream);
 
+/* C = A B for the listed molecules' packed blocks (B symmetric) on bf16x3 MFMA with FP32
+ * accumulation; FP64 in and out.  No reference counterpart of its own: it is the reduced-precision
+ * product of the DIIS residual below for the mixed-precision density step. */
+int pyseqm_gemm_bf16x3(const pyseqm_layout_t* L, const int* idx, int nact, int nmax, const double* A,
+                       const double* B, double* C, void* stream);
+
 /* Pulay DIIS (scf_loop.py:997-1035): residual C = FP - PF into a history slot and max|C|;
- * B-matrix row B[m][slot][j] = <C_slot, C_j> (packed upper triangle); F = sum_j c[m][j] F_j. */
+ * B-matrix row B[m][slot][j] = <C_slot, C_j> (packed upper triangle); F = sum_j c[m][j] F_j.
+ * mixed != 0 forms F P on bf16x3 MFMA (mixed-precision density step), else on FP64 MFMA. */
 int pyseqm_diis_residual(const pyseqm_layout_t* L, const int* idx, int nact, int nmax, const double* F,
                          const double* P, double* X, double* Cout, double* diis_err, double* red, int nb,
-                         void* stream);
+                         int mixed, void* stream);
 int pyseqm_diis_dots(const pyseqm_layout_t* L, const int* idx, int nact, const double* ring,
                      long long ring_stride, int slot, int count, int nfock, double* Bmat, double* red, int nb,
                      void* stream);

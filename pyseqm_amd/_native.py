@@ -44,7 +44,8 @@ _SIGS = {
                          c_void_p, c_void_p, c_void_p, c_int, c_void_p],
     "pyseqm_gemm": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p],
     "pyseqm_diis_residual": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
-                             c_int, c_void_p],
+                             c_int, c_int, c_void_p],
+    "pyseqm_gemm_bf16x3": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p],
     "pyseqm_diis_dots": [c_void_p, c_int, c_void_p, c_longlong, c_int, c_int, c_int, c_void_p, c_void_p, c_int,
                          c_void_p],
     "pyseqm_diis_combine": [c_void_p, c_int, c_int, c_void_p, c_longlong, c_int, c_void_p, c_int, c_void_p,

@@ -115,18 +115,23 @@ __global__ void k_check_partial(pyseqm_layout_t L, const int* __restrict__ idx, 
   }
 }
 
-__global__ void k_check_final(pyseqm_layout_t L, const int* __restrict__ idx, int nact, int nb,
-                              const double* __restrict__ red, const double* __restrict__ e_new,
-                              double* __restrict__ e_old, const double* __restrict__ diis_err, double eps,
-                              double diis_tol, unsigned char* __restrict__ notconv, double* __restrict__ metrics) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= nact) return;
+__global__ __launch_bounds__(64) void k_check_final(pyseqm_layout_t L, const int* __restrict__ idx, int nact, int nb,
+                                                   const double* __restrict__ red, const double* __restrict__ e_new,
+                                                   double* __restrict__ e_old, const double* __restrict__ diis_err,
+                                                   double eps, double diis_tol, unsigned char* __restrict__ notconv,
+                                                   double* __restrict__ metrics) {
+  // one wave per molecule: lanes take block partials b = lane, lane + 64, ... (fixed order)
+  const int k = blockIdx.x;
+  const int lane = threadIdx.x;
   const int m = idx[k];
   double s2 = 0.0, m2 = 0.0;
-  for (int b = 0; b < nb; ++b) {
+  for (int b = lane; b < nb; b += 64) {
     s2 += red[((size_t)k * nb + b) * 2];
     m2 = fmax(m2, red[((size_t)k * nb + b) * 2 + 1]);
   }
+  s2 = wave_sum(s2);
+  m2 = wave_max(m2);
+  if (lane != 0) return;
   const int nat = L.mol_atom_off[m + 1] - L.mol_atom_off[m];
   const double dm = sqrt(s2) / (4.0 * nat);
   const double de = e_new[m] - e_old[m];
@@ -211,13 +216,13 @@ __global__ void k_antisym(pyseqm_layout_t L, const int* __restrict__ idx, int nb
   if (threadIdx.x == 0) red[(size_t)k * nb + blockIdx.x] = r;
 }
 
-__global__ void k_antisym_final(const int* __restrict__ idx, int nact, int nb, const double* __restrict__ red,
-                                double* __restrict__ diis_err) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= nact) return;
+__global__ __launch_bounds__(64) void k_antisym_final(const int* __restrict__ idx, int nact, int nb,
+                                                     const double* __restrict__ red, double* __restrict__ diis_err) {
+  const int k = blockIdx.x;
   double mx = 0.0;
-  for (int b = 0; b < nb; ++b) mx = fmax(mx, red[(size_t)k * nb + b]);
-  diis_err[idx[k]] = mx;
+  for (int b = threadIdx.x; b < nb; b += 64) mx = fmax(mx, red[(size_t)k * nb + b]);
+  mx = wave_max(mx);
+  if (threadIdx.x == 0) diis_err[idx[k]] = mx;
 }
 
 // B[m][slot][j] = <C_slot, C_j> over the packed upper triangle = 0.5 * full Frobenius product
@@ -246,14 +251,15 @@ __global__ void k_diis_dots(pyseqm_layout_t L, const int* __restrict__ idx, int 
   }
 }
 
-__global__ void k_diis_dots_final(const int* __restrict__ idx, int nact, int nb, int slot, int count, int nfock,
-                                  const double* __restrict__ red, double* __restrict__ Bmat) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nact * count) return;
-  const int k = t / count, j = t % count;
+__global__ __launch_bounds__(64) void k_diis_dots_final(const int* __restrict__ idx, int nact, int nb, int slot,
+                                                       int count, int nfock, const double* __restrict__ red,
+                                                       double* __restrict__ Bmat) {
+  // one wave per (molecule, history entry): lanes take block partials b = lane, lane + 64, ...
+  const int k = blockIdx.x / count, j = blockIdx.x % count;
   double s = 0.0;
-  for (int b = 0; b < nb; ++b) s += red[((size_t)k * nb + b) * 16 + j];
-  Bmat[((size_t)idx[k] * (nfock + 1) + slot) * (nfock + 1) + j] = 0.5 * s;
+  for (int b = threadIdx.x; b < nb; b += 64) s += red[((size_t)k * nb + b) * 16 + j];
+  s = wave_sum(s);
+  if (threadIdx.x == 0) Bmat[((size_t)idx[k] * (nfock + 1) + slot) * (nfock + 1) + j] = 0.5 * s;
 }
 
 // F = sum_j c[m][j] F_j over the stored Fock matrices (scf_loop.py:1033-1035).
@@ -326,7 +332,7 @@ extern "C" int pyseqm_scf_check(const pyseqm_layout_t* L, const int* idx, int na
   if (nact == 0) return PYSEQM_OK;
   hipStream_t s = (hipStream_t)stream;
   k_check_partial<<<dim3(nb, nact), kEl, 0, s>>>(*L, idx, nb, P, Pold, red);
-  k_check_final<<<ceil_div(nact, 64), 64, 0, s>>>(*L, idx, nact, nb, red, e_new, e_old, diis_err, eps, diis_tol,
+  k_check_final<<<nact, 64, 0, s>>>(*L, idx, nact, nb, red, e_new, e_old, diis_err, eps, diis_tol,
                                                   notconv, metrics);
   return status();
 }
@@ -344,16 +350,22 @@ extern "C" int pyseqm_dgemm_sym(const pyseqm_layout_t* L, const int* idx, int na
                                 const long long* off, const int* lds, const double* A, const double* B, double* C,
                                 int mode, void* stream);
 
-// X = F P on the FP64 MFMA GEMM (P symmetric), then C = X - X^T and max|C|.
+extern "C" int pyseqm_gemm_bf16x3(const pyseqm_layout_t* L, const int* idx, int nact, int nmax, const double* A,
+                                  const double* B, double* C, void* stream);
+
+// X = F P on the FP64 MFMA GEMM (P symmetric), or on the bf16x3 MFMA GEMM when mixed != 0 (the
+// mixed-precision density step, whose own error floor is far above FP32 rounding), then
+// C = X - X^T and max|C|.
 extern "C" int pyseqm_diis_residual(const pyseqm_layout_t* L, const int* idx, int nact, int nmax, const double* F,
                                     const double* P, double* X, double* Cout, double* diis_err, double* red, int nb,
-                                    void* stream) {
+                                    int mixed, void* stream) {
   if (nb < 1 || nb > PYSEQM_MAX_RED_BLOCKS) return PYSEQM_EINVAL;
-  int rc = pyseqm_dgemm_sym(L, idx, nact, nmax, L->mol_mat_off, nullptr, F, P, X, 0, stream);
+  int rc = mixed ? pyseqm_gemm_bf16x3(L, idx, nact, nmax, F, P, X, stream)
+                 : pyseqm_dgemm_sym(L, idx, nact, nmax, L->mol_mat_off, nullptr, F, P, X, 0, stream);
   if (rc != PYSEQM_OK || nact == 0) return rc;
   hipStream_t s = (hipStream_t)stream;
   k_antisym<<<dim3(nb, nact), kEl, 0, s>>>(*L, idx, nb, X, Cout, red);
-  k_antisym_final<<<ceil_div(nact, 64), 64, 0, s>>>(idx, nact, nb, red, diis_err);
+  k_antisym_final<<<nact, 64, 0, s>>>(idx, nact, nb, red, diis_err);
   return status();
 }
 
@@ -366,7 +378,7 @@ extern "C" int pyseqm_diis_dots(const pyseqm_layout_t* L, const int* idx, int na
   if (nact == 0) return PYSEQM_OK;
   hipStream_t s = (hipStream_t)stream;
   k_diis_dots<<<dim3(nb, nact), kEl, 0, s>>>(*L, idx, nb, ring, (size_t)ring_stride, slot, count, red);
-  k_diis_dots_final<<<ceil_div((long long)nact * count, 64), 64, 0, s>>>(idx, nact, nb, slot, count, nfock, red,
+  k_diis_dots_final<<<nact * count, 64, 0, s>>>(idx, nact, nb, slot, count, nfock, red,
                                                                          Bmat);
   return status();
 }

@@ -189,13 +189,14 @@ __global__ __launch_bounds__(256) void k_sp2_bf16x3(const int* __restrict__ idx,
   __shared__ __attribute__((aligned(16))) __bf16 As[2][2][BT][LDSK];
   __shared__ __attribute__((aligned(16))) __bf16 Bs[2][2][BT][LDSK];
   double(*red)[4] = reinterpret_cast<double(*)[4]>(&As[0][0][0][0]);  // reused after the main loop
+  const int tile = blockIdx.x;
   const int m = idx[blockIdx.y];
   const double* s = st + (size_t)m * S_NST;
   if (s[S_DONE] != 0.0) return;
   const int n = norb[m];
   const int ld = xld[m];
   int ti, tj;
-  tri_decode(blockIdx.x, ti, tj);
+  tri_decode(tile, ti, tj);
   if (tj * BT >= n) return;
   const bool sq = s[S_BRANCH] != 0.0;
   const float* Xm = Xin + xoff[m];
@@ -309,35 +310,152 @@ __global__ __launch_bounds__(256) void k_sp2_bf16x3(const int* __restrict__ idx,
   }
   __syncthreads();
   if (tid == 0) {
-    double* p = part + (size_t)m * part_stride + 2 * blockIdx.x;
+    double* p = part + (size_t)m * part_stride + 2 * tile;
     p[0] = red[0][0] + red[0][1] + red[0][2] + red[0][3];
     p[1] = red[1][0] + red[1][1] + red[1][2] + red[1][3];
   }
 }
 
+// C = A B for FP64 packed square blocks with B symmetric (B is read by rows: B[k][j] = B[j][k]),
+// on bf16x3 MFMA with FP32 accumulation (relative error ~1e-6, enough for a DIIS error vector):
+// the operands are rounded to FP32 and split into bf16 hi + lo while they are staged through LDS.
+// All 128 x 128 output tiles of each active molecule, same pipeline as k_sp2_bf16x3.
+__global__ __launch_bounds__(256) void k_gemm_bf16x3_f64(const int* __restrict__ idx, int ntmax,
+                                                         const int* __restrict__ norb,
+                                                         const long long* __restrict__ off,
+                                                         const double* __restrict__ A, const double* __restrict__ B,
+                                                         double* __restrict__ C) {
+  __shared__ __attribute__((aligned(16))) __bf16 As[2][2][BT][LDSK];
+  __shared__ __attribute__((aligned(16))) __bf16 Bs[2][2][BT][LDSK];
+  const int m = idx[blockIdx.y];
+  const int n = norb[m];
+  const int nt = (n + BT - 1) / BT;
+  const int ti = blockIdx.x / ntmax, tj = blockIdx.x % ntmax;
+  if (ti >= nt || tj >= nt) return;
+  const double* Am = A + off[m];
+  const double* Bm = B + off[m];
+  double* Cm = C + off[m];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = wave >> 1, wc = wave & 1;
+  const int i0 = ti * BT, j0 = tj * BT;
+  floatx16 acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) acc[a][b][v] = 0.0f;
+  float4 ra[4], rb[4];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int f = tid + 256 * q;
+      const int row = f >> 3, k = k0 + 4 * (f & 7);
+      float x[4], y[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        x[u] = (i0 + row < n && k + u < n) ? (float)Am[(size_t)(i0 + row) * n + k + u] : 0.0f;
+        y[u] = (j0 + row < n && k + u < n) ? (float)Bm[(size_t)(j0 + row) * n + k + u] : 0.0f;
+      }
+      ra[q] = make_float4(x[0], x[1], x[2], x[3]);
+      rb[q] = make_float4(y[0], y[1], y[2], y[3]);
+    }
+  };
+  auto split_store = [&](const float4& v, __bf16 (*hi)[LDSK], __bf16 (*lo)[LDSK], int row, int col) {
+    union {
+      __bf16 h[4];
+      uint2 u;
+    } ph, pl;
+    const float x[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      ph.h[u] = (__bf16)x[u];
+      pl.h[u] = (__bf16)(x[u] - (float)ph.h[u]);
+    }
+    *reinterpret_cast<uint2*>(&hi[row][col]) = ph.u;
+    *reinterpret_cast<uint2*>(&lo[row][col]) = pl.u;
+  };
+  auto stage = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int f = tid + 256 * q;
+      const int row = f >> 3, kq = f & 7;
+      split_store(ra[q], As[buf][0], As[buf][1], row, 4 * kq);
+      split_store(rb[q], Bs[buf][0], Bs[buf][1], row, 4 * kq);
+    }
+  };
+  const int kend = (n + BK - 1) / BK * BK;
+  gload(0);
+  stage(0);
+  __syncthreads();
+  int cur = 0;
+  for (int k0 = 0; k0 < kend; k0 += BK) {
+    const bool more = k0 + BK < kend;
+    if (more) gload(k0 + BK);
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 16) {
+      bf16x8 ah[2], al[2], bh[2], bl[2];
+      const int kofs = kk + 8 * (lane >> 5);
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+        const int r = wr * 64 + a * 32 + (lane & 31);
+        ah[a] = *reinterpret_cast<const bf16x8*>(&As[cur][0][r][kofs]);
+        al[a] = *reinterpret_cast<const bf16x8*>(&As[cur][1][r][kofs]);
+        const int c = wc * 64 + a * 32 + (lane & 31);
+        bh[a] = *reinterpret_cast<const bf16x8*>(&Bs[cur][0][c][kofs]);
+        bl[a] = *reinterpret_cast<const bf16x8*>(&Bs[cur][1][c][kofs]);
+      }
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[a], bh[b], acc[a][b], 0, 0, 0);
+          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a], bl[b], acc[a][b], 0, 0, 0);
+          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a], bh[b], acc[a][b], 0, 0, 0);
+        }
+    }
+    if (more) stage(cur ^ 1);
+    __syncthreads();
+    cur ^= 1;
+  }
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const int r = i0 + wr * 64 + a * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+        const int c = j0 + wc * 64 + b * 32 + (lane & 31);
+        if (r < n && c < n) Cm[(size_t)r * n + c] = (double)acc[a][b][v];
+      }
+}
+
 // Sum the tile partials (fixed order), update the trace errors, decide the next branch and
 // whether the molecule stops: err < eps and no improvement over two iterations ago (the
 // reference's reduced-precision criterion, SP2.py:75-79), or iteration cap.
-__global__ void k_sp2_decide(const int* __restrict__ idx, int nact, const int* __restrict__ norb,
-                             const int* __restrict__ nocc, double* __restrict__ st, const double* __restrict__ part,
-                             int part_stride, double eps, int max_iter, int* __restrict__ ndone) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= nact) return;
-  const int m = idx[k];
+__global__ __launch_bounds__(64) void k_sp2_decide(const int* __restrict__ idx, int nact,
+                                                   const int* __restrict__ norb, const int* __restrict__ nocc,
+                                                   double* __restrict__ st, const double* __restrict__ part,
+                                                   int part_stride, double eps, int max_iter,
+                                                   int* __restrict__ ndone) {
+  // one wave per molecule: lanes take tile partials t = lane, lane + 64, ... and a fixed shuffle
+  // tree sums them (deterministic)
+  const int m = idx[blockIdx.x];
+  const int lane = threadIdx.x;
   double* s = st + (size_t)m * S_NST;
-  if (s[S_DONE] != 0.0) return;
+  if (s[S_DONE] != 0.0) return;  // uniform over the wave
   const int n = norb[m];
   const int nt = (n + BT - 1) / BT;
   const int ntri = nt * (nt + 1) / 2;
   double tr = 0.0, fro = 0.0;
   const double* p = part + (size_t)m * part_stride;
-  for (int t = 0; t < ntri; ++t) {
-    int ti, tj;
-    tri_decode(t, ti, tj);
-    if (tj * BT >= n) continue;
+  for (int t = lane; t < ntri; t += 64) {
     tr += p[2 * t];
     fro += p[2 * t + 1];
   }
+  tr = wsum(tr);
+  fro = wsum(fro);
+  if (lane != 0) return;
   const double N = nocc[m];
   s[S_TR] = tr;
   s[S_FRO] = fro;
@@ -513,8 +631,18 @@ extern "C" int pyseqm_sp2_iterate(const pyseqm_layout_t* L, const int* idx, int 
   hipStream_t s = (hipStream_t)stream;
   k_sp2_bf16x3<<<dim3(nt * (nt + 1) / 2, nact), 256, 0, s>>>(idx, L->mol_norb, xoff, xld, Xin, Xout, state, part,
                                                                part_stride);
-  k_sp2_decide<<<(nact + 63) / 64, 64, 0, s>>>(idx, nact, L->mol_norb, L->mol_nocc, state, part, part_stride, eps,
-                                               max_iter, ndone);
+  k_sp2_decide<<<nact, 64, 0, s>>>(idx, nact, L->mol_norb, L->mol_nocc, state, part, part_stride, eps, max_iter,
+                                   ndone);
+  return st_ok();
+}
+
+extern "C" int pyseqm_gemm_bf16x3(const pyseqm_layout_t* L, const int* idx, int nact, int nmax, const double* A,
+                                  const double* B, double* C, void* stream) {
+  if (!L || nact < 0 || nmax <= 0) return PYSEQM_EINVAL;
+  if (nact == 0) return PYSEQM_OK;
+  const int nt = (nmax + BT - 1) / BT;
+  k_gemm_bf16x3_f64<<<dim3(nt * nt, nact), 256, 0, (hipStream_t)stream>>>(idx, nt, L->mol_norb, L->mol_mat_off, A,
+                                                                           B, C);
   return st_ok();
 }
 

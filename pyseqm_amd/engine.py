@@ -295,7 +295,8 @@ class SCFEngine:
             nat.call("pyseqm_copy_active", b.L, nat.ptr(idx), nact, nmax, nat.ptr(F), nat.ptr(Fslot), self.stream)
             nb = nat.red_blocks(nmax)
             nat.call("pyseqm_diis_residual", b.L, nat.ptr(idx), nact, nmax, nat.ptr(F), nat.ptr(P), nat.ptr(X),
-                     nat.ptr(Cring[counter]), nat.ptr(diis_err), nat.ptr(self.red), nb, self.stream)
+                     nat.ptr(Cring[counter]), nat.ptr(diis_err), nat.ptr(self.red), nb,
+                     int(self.density_kind == "sp2"), self.stream)
             nat.call("pyseqm_diis_dots", b.L, nat.ptr(idx), nact, nat.ptr(Cring), total, counter, cF, NFOCK,
                      nat.ptr(EMAT), nat.ptr(self.red), nb, self.stream)
             reset_t = None

@@ -81,3 +81,26 @@ def test_sp2_scf_energy_per_atom():
     assert not r.notconverged.any()
     nat_ = e2.b.nat
     assert np.all(np.abs(h_sp2 - h_eig) / nat_ < 1e-4), (h_sp2 - h_eig) / nat_
+
+
+def test_gemm_bf16x3_matches_fp64():
+    """The bf16x3 product used for the mixed-precision DIIS residual: F P to ~1e-6 relative."""
+    from pyseqm_amd import _native as nat
+    from pyseqm_amd.engine import SCFEngine
+    sp, c = W.water_clusters(40, 3, seed=2)
+    eng = SCFEngine(sp, c, "PM3", "cuda")
+    r = eng.scf(eps=1e-6, density="sp2")
+    b = eng.b
+    F = torch.empty_like(r.P)
+    eng.fock(r.P, F)
+    idx = torch.arange(b.nmol, dtype=torch.int32, device="cuda")
+    X = torch.zeros_like(F)
+    nat.call("pyseqm_gemm_bf16x3", b.L, nat.ptr(idx), b.nmol, int(b.nmax), nat.ptr(F), nat.ptr(r.P), nat.ptr(X),
+             eng.stream)
+    for m in range(b.nmol):
+        n, o = int(b.norb[m]), int(b.mol_mat_off[m])
+        Fm = F[o:o + n * n].view(n, n)
+        Pm = r.P[o:o + n * n].view(n, n)
+        ref = Fm @ Pm
+        err = (X[o:o + n * n].view(n, n) - ref).abs().max().item()
+        assert err < 1e-5 * ref.abs().max().item(), (m, err)
