@@ -73,9 +73,15 @@ def cpu_baseline(method, sp, xyz, nsample, eps, conv, target_s=15.0):
                 converged=conv_ok)
 
 
-def serve(args, dev):
+def serve(args, dev, rate=None, xl_trajectories=None):
     """Open-loop Poisson SCF stream (half AM1 alkanes, half PM3 water clusters) through the
     continuous-batching server, beside 256 graph-stepped XL-BOMD trajectories."""
+    import copy
+    args = copy.copy(args)
+    if rate is not None:
+        args.serve_rate = rate
+    if xl_trajectories is not None:
+        args.xl_trajectories = xl_trajectories
     import threading
 
     import torch
@@ -238,6 +244,8 @@ def main():
     ap.add_argument("--sp2-roofline-config", default="C3", help="workload of the SP2 GEMM roofline ('' = skip)")
     ap.add_argument("--serve-window", type=float, default=10.0, help="seconds of Poisson arrivals (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=2000.0)
+    ap.add_argument("--serve-rate-light", type=float, default=300.0,
+                    help="second, lighter Poisson rate whose latencies show the loop below saturation (0 = skip)")
     ap.add_argument("--serve-water-max", type=int, default=256)
     ap.add_argument("--serve-lanes", type=int, default=3)
     ap.add_argument("--serve-eps", type=float, default=1e-5)
@@ -358,6 +366,8 @@ def main():
         line["serving"] = sv
         line["p50_ms"] = sv["latency_ms"]["p50"]
         line["p99_ms"] = sv["latency_ms"]["p99"]
+        if args.serve_rate_light > 0:
+            line["serving_light"] = serve(args, dev, rate=args.serve_rate_light)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(method, sp, xyz, args.cpu_sample, args.eps, conv, args.cpu_seconds)
     print(json.dumps(line), flush=True)

@@ -27,6 +27,9 @@ class SCFResult:
 # (measured on MI355X with the ROCm 7 solver: 1024 blocks of 64 take 417 ms, of 68 take 8 ms), so
 # a group of many small blocks is padded up to this size.
 EIGH_MIN_N = 68
+# The DIIS product F P goes to the bf16x3 GEMM (128-wide tiles) only for blocks larger than this;
+# smaller blocks stay on the FP64 GEMM with 64-wide tiles, which is faster at that size.
+MIXED_DIIS_MIN_N = 256
 EIGH_MIN_N_BATCH = 16
 
 
@@ -296,7 +299,7 @@ class SCFEngine:
             nb = nat.red_blocks(nmax)
             nat.call("pyseqm_diis_residual", b.L, nat.ptr(idx), nact, nmax, nat.ptr(F), nat.ptr(P), nat.ptr(X),
                      nat.ptr(Cring[counter]), nat.ptr(diis_err), nat.ptr(self.red), nb,
-                     int(self.density_kind == "sp2"), self.stream)
+                     int(self.density_kind == "sp2" and nmax > MIXED_DIIS_MIN_N), self.stream)
             nat.call("pyseqm_diis_dots", b.L, nat.ptr(idx), nact, nat.ptr(Cring), total, counter, cF, NFOCK,
                      nat.ptr(EMAT), nat.ptr(self.red), nb, self.stream)
             reset_t = None

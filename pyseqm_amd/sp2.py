@@ -36,6 +36,8 @@ class SP2Workspace:
         self.part_stride = max(nt * (nt + 1), 2)
         self.part = torch.zeros(b.nmol, self.part_stride, dtype=torch.float64, device=dev)
         self.ndone = torch.zeros(1, dtype=torch.int32, device=dev)
+        # two pinned slots: the stop counter of chunk c is copied while chunk c+1 is launched
+        self.ndone_host = torch.zeros(2, dtype=torch.int32, pin_memory=True)
         self.last_iters = None
         self.timing = None  # list of (start event, end event, idx, nact) of the SP2 GEMM loops
 
@@ -59,12 +61,35 @@ def density_sp2(eng, F, idx, nact, Pout, eps=SP2_EPS, max_iter=SP2_MAX_ITER, fix
         ev0 = torch.cuda.Event(enable_timing=True)
         ev1 = torch.cuda.Event(enable_timing=True)
         ev0.record()
-    for it in range(max_iter if fixed_iters is None else int(fixed_iters)):
+    def launch(it):
         nat.call("pyseqm_sp2_iterate", L, nat.ptr(idx), nact, nmax, nat.ptr(ws.xoff), nat.ptr(ws.xld),
                  nat.ptr(bufs[it % 2]), nat.ptr(bufs[(it + 1) % 2]), nat.ptr(ws.state), nat.ptr(ws.part),
                  ws.part_stride, float(eps), int(max_iter), nat.ptr(ws.ndone), s)
-        if fixed_iters is None and (it + 1) % POLL == 0 and int(ws.ndone.item()) >= nact:
-            break
+
+    if fixed_iters is not None:
+        for it in range(int(fixed_iters)):
+            launch(it)
+    else:
+        # pipelined stop test: after each chunk of POLL iterations the counter is copied to pinned
+        # host memory behind an event; the host reads the previous chunk's copy while the current
+        # chunk runs, so the stream never drains for the test (molecules that stopped make the
+        # extra chunk's launches no-ops)
+        it, chunk, pending = 0, 0, None
+        while it < max_iter:
+            for _ in range(min(POLL, max_iter - it)):
+                launch(it)
+                it += 1
+            slot = chunk % 2
+            ws.ndone_host[slot:slot + 1].copy_(ws.ndone, non_blocking=True)
+            ev = torch.cuda.Event()
+            ev.record()
+            if pending is not None:
+                pev, pslot = pending
+                pev.synchronize()
+                if int(ws.ndone_host[pslot]) >= nact:
+                    break
+            pending = (ev, slot)
+            chunk += 1
     if ws.timing is not None:
         ev1.record()
         ws.timing.append((ev0, ev1, idx.clone(), nact))
