@@ -168,6 +168,12 @@ int pyseqm_diis_residual(const pyseqm_layout_t* L, const int* idx, int nact, int
 int pyseqm_diis_dots(const pyseqm_layout_t* L, const int* idx, int nact, const double* ring,
                      long long ring_stride, int slot, int count, int nfock, double* Bmat, double* red, int nb,
                      void* stream);
+/* DIIS coefficients per listed molecule from the bordered B matrix Bmat[m][(nfock+1)^2] (lower
+ * triangle; the count x count block normalised by B[slot][slot]), count + 1 <= 16: FP64 Jacobi
+ * eigendecomposition and the eigenvalue pseudo-inverse of scf_loop.py:1015-1031; coef[m][0..count)
+ * (stride nfock) and reset[m] = 1 when max|lambda| / min|lambda| > 1e7. */
+int pyseqm_diis_solve(const pyseqm_layout_t* L, const int* idx, int nact, int nfock, int count, int slot,
+                      const double* Bmat, double* coef, unsigned char* reset, void* stream);
 int pyseqm_diis_combine(const pyseqm_layout_t* L, const int* idx, int nact, int nmax, const double* ring,
                         long long ring_stride, int count, const double* coef, int cstride, double* F,
                         void* stream);
@@ -185,6 +191,11 @@ int pyseqm_sp2_init(const pyseqm_layout_t* L, const double* F, const int* idx, i
 int pyseqm_sp2_iterate(const pyseqm_layout_t* L, const int* idx, int nact, int nmax, const long long* xoff,
                        const int* xld, const float* Xin, float* Xout, double* state, double* part,
                        int part_stride, double eps, int max_iter, int* ndone, void* stream);
+/* `count` iterations it0 .. it0+count-1 in one call; iteration it reads Xa when it is even (else Xb)
+ * and writes the other buffer. */
+int pyseqm_sp2_iterate_many(const pyseqm_layout_t* L, const int* idx, int nact, int nmax, const long long* xoff,
+                            const int* xld, float* Xa, float* Xb, double* state, double* part, int part_stride,
+                            double eps, int max_iter, int* ndone, int it0, int count, void* stream);
 /* Final X (buffer chosen by each molecule's iteration parity) -> FP64 padded copy. */
 int pyseqm_sp2_x_to_f64(const pyseqm_layout_t* L, const int* idx, int nact, int nmax, const long long* xoff,
                         const int* xld, const double* state, const float* Xa, const float* Xb, double* Y,

@@ -262,6 +262,84 @@ __global__ __launch_bounds__(64) void k_diis_dots_final(const int* __restrict__ 
   if (threadIdx.x == 0) Bmat[((size_t)idx[k] * (nfock + 1) + slot) * (nfock + 1) + j] = 0.5 * s;
 }
 
+// DIIS coefficients from the bordered B matrix (scf_loop.py:1015-1031): the lower triangle of
+// EMAT[m][0..cF][0..cF] (mirrored), its cF x cF block divided by B[slot][slot], eigendecomposed by
+// cyclic Jacobi in FP64 (one 64-lane block per molecule, matrix and vectors in LDS, lane i owns
+// row / column i), then c_k = -sum_i Q[k][i] q_i Q[cF][i] with q_i = 1/lambda_i for |lambda_i| >
+// 1e-13 (else 0), and reset[m] = (max|lambda| / min|lambda| > 1e7).
+constexpr int kDiisMax = 16;
+
+__global__ __launch_bounds__(64) void k_diis_solve(const int* __restrict__ idx, int nfock, int cF, int slot,
+                                                   const double* __restrict__ EMAT, double* __restrict__ coef,
+                                                   unsigned char* __restrict__ reset) {
+  __shared__ double A[kDiisMax][kDiisMax + 1], V[kDiisMax][kDiisMax + 1];
+  const int m = idx[blockIdx.x];
+  const int lane = threadIdx.x;
+  const int n = cF + 1, nf1 = nfock + 1;
+  const double* E = EMAT + (size_t)m * nf1 * nf1;
+  const double den = fmax(E[slot * nf1 + slot], 1.0e-15);
+  for (int e = lane; e < n * n; e += 64) {
+    const int i = e / n, j = e % n;
+    double v = i >= j ? E[i * nf1 + j] : E[j * nf1 + i];
+    if (i < cF && j < cF) v /= den;
+    A[i][j] = v;
+    V[i][j] = i == j ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  for (int sweep = 0; sweep < 30; ++sweep) {
+    double off = 0.0, tot = 0.0;
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) {
+        const double a = A[i][j];
+        tot += a * a;
+        if (j > i) off += a * a;
+      }
+    if (off <= 1.0e-34 * tot) break;  // the same decision in every lane
+    for (int p = 0; p < n - 1; ++p)
+      for (int q = p + 1; q < n; ++q) {
+        const double apq = A[p][q];
+        if (fabs(apq) < 1.0e-300) continue;
+        const double tau = (A[q][q] - A[p][p]) / (2.0 * apq);
+        const double t = fabs(tau) < 1.0e150 ? (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau))
+                                             : 0.5 / tau;
+        const double c = 1.0 / sqrt(1.0 + t * t), sn = t * c;
+        __syncthreads();
+        if (lane < n) {
+          const double ap = A[lane][p], aq = A[lane][q];
+          A[lane][p] = c * ap - sn * aq;
+          A[lane][q] = sn * ap + c * aq;
+        }
+        __syncthreads();
+        if (lane < n) {
+          const double ap = A[p][lane], aq = A[q][lane];
+          A[p][lane] = c * ap - sn * aq;
+          A[q][lane] = sn * ap + c * aq;
+          const double vp = V[lane][p], vq = V[lane][q];
+          V[lane][p] = c * vp - sn * vq;
+          V[lane][q] = sn * vp + c * vq;
+        }
+        __syncthreads();
+      }
+  }
+  if (lane < cF) {
+    double ck = 0.0;
+    for (int i = 0; i < n; ++i) {
+      const double li = A[i][i];
+      const double qi = fabs(li) > 1.0e-13 ? 1.0 / li : 0.0;
+      ck -= V[lane][i] * qi * V[cF][i];
+    }
+    coef[(size_t)m * nfock + lane] = ck;
+  }
+  if (lane == 0) {
+    double lo = 1.0e308, hi = 0.0;
+    for (int i = 0; i < n; ++i) {
+      lo = fmin(lo, fabs(A[i][i]));
+      hi = fmax(hi, fabs(A[i][i]));
+    }
+    reset[m] = hi > 1.0e7 * lo ? 1 : 0;
+  }
+}
+
 // F = sum_j c[m][j] F_j over the stored Fock matrices (scf_loop.py:1033-1035).
 __global__ void k_diis_combine(pyseqm_layout_t L, const int* __restrict__ idx, const double* __restrict__ ring,
                                size_t ring_stride, int count, const double* __restrict__ coef, int cstride,
@@ -380,6 +458,15 @@ extern "C" int pyseqm_diis_dots(const pyseqm_layout_t* L, const int* idx, int na
   k_diis_dots<<<dim3(nb, nact), kEl, 0, s>>>(*L, idx, nb, ring, (size_t)ring_stride, slot, count, red);
   k_diis_dots_final<<<nact * count, 64, 0, s>>>(idx, nact, nb, slot, count, nfock, red,
                                                                          Bmat);
+  return status();
+}
+
+extern "C" int pyseqm_diis_solve(const pyseqm_layout_t* L, const int* idx, int nact, int nfock, int count, int slot,
+                                 const double* Bmat, double* coef, unsigned char* reset, void* stream) {
+  if (!L || nact < 0 || count < 1 || count + 1 > kDiisMax || count > nfock || slot < 0 || slot >= nfock)
+    return PYSEQM_EINVAL;
+  if (nact == 0) return PYSEQM_OK;
+  k_diis_solve<<<nact, 64, 0, (hipStream_t)stream>>>(idx, nfock, count, slot, Bmat, coef, reset);
   return status();
 }
 

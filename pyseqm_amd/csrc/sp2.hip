@@ -636,6 +636,22 @@ extern "C" int pyseqm_sp2_iterate(const pyseqm_layout_t* L, const int* idx, int 
   return st_ok();
 }
 
+// `count` SP2 iterations it0 .. it0+count-1 in one call: iteration it reads buffer (it even ? Xa : Xb)
+// and writes the other one (the ping-pong of pyseqm_sp2_iterate), with no host work in between.
+extern "C" int pyseqm_sp2_iterate_many(const pyseqm_layout_t* L, const int* idx, int nact, int nmax,
+                                       const long long* xoff, const int* xld, float* Xa, float* Xb, double* state,
+                                       double* part, int part_stride, double eps, int max_iter, int* ndone, int it0,
+                                       int count, void* stream) {
+  if (count < 0 || it0 < 0) return PYSEQM_EINVAL;
+  for (int k = 0; k < count; ++k) {
+    const bool even = ((it0 + k) & 1) == 0;
+    const int rc = pyseqm_sp2_iterate(L, idx, nact, nmax, xoff, xld, even ? Xa : Xb, even ? Xb : Xa, state, part,
+                                      part_stride, eps, max_iter, ndone, stream);
+    if (rc != PYSEQM_OK) return rc;
+  }
+  return PYSEQM_OK;
+}
+
 extern "C" int pyseqm_gemm_bf16x3(const pyseqm_layout_t* L, const int* idx, int nact, int nmax, const double* A,
                                   const double* B, double* C, void* stream) {
   if (!L || nact < 0 || nmax <= 0) return PYSEQM_EINVAL;

@@ -283,6 +283,7 @@ class SCFEngine:
         emat0 = torch.tril(torch.eye(NFOCK + 1, **f64) - 1.0)
         EMAT = emat0.expand(b.nmol, -1, -1).contiguous()
         coef = torch.zeros(b.nmol, NFOCK, **f64)
+        reset_flags = torch.zeros(b.nmol, dtype=torch.uint8, device=dev)
         nc_host = np.ones(b.nmol, dtype=bool)
         counter, cF = -1, 0
         it = 0
@@ -304,21 +305,12 @@ class SCFEngine:
                      nat.ptr(EMAT), nat.ptr(self.red), nb, self.stream)
             reset_t = None
             if cF >= 2:
-                il = idx.long()
-                EV = EMAT[il, :cF + 1, :cF + 1].clone()
-                den = EV[:, counter, counter].clamp(min=1e-15)
-                EV[:, :cF, :cF] /= den.view(-1, 1, 1)
-                # the reference's eigh reads only the lower triangle (stale upper entries included);
-                # mirror it explicitly so the result does not depend on the solver's UPLO handling
-                EV = torch.tril(EV) + torch.tril(EV, -1).transpose(1, 2)
-                Lw, Q = torch.linalg.eigh(EV)
-                absL = Lw.abs()
-                cond = absL.amax(-1) / absL.amin(-1)
-                inv = torch.where(absL > 1e-13, 1.0 / torch.where(absL > 1e-13, Lw, torch.ones_like(Lw)),
-                                  torch.zeros_like(Lw))
-                c = -torch.einsum("bki,bi,bi->bk", Q[:, :cF, :], inv, Q[:, -1, :])
-                coef[il, :cF] = c
-                reset_t = (cond > 1e7).any()  # read with the convergence flags (one host sync)
+                # the reference's eigh reads only the lower triangle (stale upper entries included):
+                # the kernel mirrors it, normalises by B[slot][slot] and solves by FP64 Jacobi
+                reset_flags.zero_()
+                nat.call("pyseqm_diis_solve", b.L, nat.ptr(idx), nact, NFOCK, cF, counter, nat.ptr(EMAT),
+                         nat.ptr(coef), nat.ptr(reset_flags), self.stream)
+                reset_t = reset_flags.any()  # read with the convergence flags (one host sync)
                 nat.call("pyseqm_diis_combine", b.L, nat.ptr(idx), nact, nmax, nat.ptr(Fring), total, cF,
                          nat.ptr(coef), NFOCK, nat.ptr(F), self.stream)
             self.density(F, idx, nact, Pnew)

@@ -56,19 +56,18 @@ def density_sp2(eng, F, idx, nact, Pout, eps=SP2_EPS, max_iter=SP2_MAX_ITER, fix
     nat.call("pyseqm_sp2_init", L, nat.ptr(F), nat.ptr(idx), nact, nat.ptr(ws.xoff), nat.ptr(ws.xld),
              nat.ptr(ws.Xa), nat.ptr(ws.state), nat.ptr(eng.red), nat.red_blocks(nmax), s)
     ws.ndone.zero_()
-    bufs = (ws.Xa, ws.Xb)
     if ws.timing is not None:
         ev0 = torch.cuda.Event(enable_timing=True)
         ev1 = torch.cuda.Event(enable_timing=True)
         ev0.record()
-    def launch(it):
-        nat.call("pyseqm_sp2_iterate", L, nat.ptr(idx), nact, nmax, nat.ptr(ws.xoff), nat.ptr(ws.xld),
-                 nat.ptr(bufs[it % 2]), nat.ptr(bufs[(it + 1) % 2]), nat.ptr(ws.state), nat.ptr(ws.part),
-                 ws.part_stride, float(eps), int(max_iter), nat.ptr(ws.ndone), s)
+    args = (nat.ptr(idx), nact, nmax, nat.ptr(ws.xoff), nat.ptr(ws.xld), nat.ptr(ws.Xa), nat.ptr(ws.Xb),
+            nat.ptr(ws.state), nat.ptr(ws.part), ws.part_stride, float(eps), int(max_iter), nat.ptr(ws.ndone))
+
+    def launch_many(it0, count):
+        nat.call("pyseqm_sp2_iterate_many", L, *args, it0, count, s)
 
     if fixed_iters is not None:
-        for it in range(int(fixed_iters)):
-            launch(it)
+        launch_many(0, int(fixed_iters))
     else:
         # pipelined stop test: after each chunk of POLL iterations the counter is copied to pinned
         # host memory behind an event; the host reads the previous chunk's copy while the current
@@ -76,9 +75,9 @@ def density_sp2(eng, F, idx, nact, Pout, eps=SP2_EPS, max_iter=SP2_MAX_ITER, fix
         # extra chunk's launches no-ops)
         it, chunk, pending = 0, 0, None
         while it < max_iter:
-            for _ in range(min(POLL, max_iter - it)):
-                launch(it)
-                it += 1
+            k = min(POLL, max_iter - it)
+            launch_many(it, k)
+            it += k
             slot = chunk % 2
             ws.ndone_host[slot:slot + 1].copy_(ws.ndone, non_blocking=True)
             ev = torch.cuda.Event()

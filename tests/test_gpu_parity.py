@@ -182,3 +182,45 @@ def test_forces_match_oracle(name, method):
     Po = eng.b.to_padded(r.P).cpu().numpy()
     fo = R.forces_fd(sp, c, method, Po)
     np.testing.assert_allclose(f, fo, atol=1e-6, rtol=0)
+
+
+def test_diis_solve_matches_eigh_pseudo_inverse():
+    """The Jacobi DIIS solve against the reference's eigh-based pseudo-inverse (scf_loop.py:1015-1031)
+    on bordered B matrices of nearly dependent residuals: coefficients to 1e-8 relative where the
+    system is kept (cond <= 1e7), identical reset decision."""
+    from pyseqm_amd import _native as nat
+    sp, c = _inputs("alkanes")
+    eng = SCFEngine(sp, c, "AM1", DEV)
+    rng = np.random.default_rng(9)
+    nf = 10
+    nmol = eng.b.nmol
+    for cF in (2, 5, 10):
+        E = np.zeros((nmol, nf + 1, nf + 1))
+        for m in range(nmol):
+            X = rng.normal(size=(cF, 40))
+            X[1:] = X[0] + 10.0 ** rng.uniform(-4, 0) * X[1:]
+            B = X @ X.T
+            Em = -np.ones((cF + 1, cF + 1))
+            Em[:cF, :cF] = B
+            Em[cF, cF] = 0.0
+            E[m, :cF + 1, :cF + 1] = np.tril(Em)
+            E[m] += np.triu(rng.normal(size=(nf + 1, nf + 1)), 1)  # stale upper entries are ignored
+        slot = cF - 1
+        Et = torch.as_tensor(E, device=DEV)
+        coef = torch.zeros(nmol, nf, dtype=torch.float64, device=DEV)
+        reset = torch.zeros(nmol, dtype=torch.uint8, device=DEV)
+        idx = torch.arange(nmol, dtype=torch.int32, device=DEV)
+        nat.call("pyseqm_diis_solve", eng.b.L, nat.ptr(idx), nmol, nf, cF, slot, nat.ptr(Et), nat.ptr(coef),
+                 nat.ptr(reset), eng.stream)
+        for m in range(nmol):
+            EV = np.tril(E[m, :cF + 1, :cF + 1])
+            EV = EV + np.tril(EV, -1).T
+            EV[:cF, :cF] /= max(EV[slot, slot], 1e-15)
+            w, Q = np.linalg.eigh(EV)
+            inv = np.where(np.abs(w) > 1e-13, 1.0 / np.where(np.abs(w) > 1e-13, w, 1.0), 0.0)
+            cref = -(Q[:cF] * inv[None] * Q[cF][None]).sum(1)
+            cond = np.abs(w).max() / np.abs(w).min()
+            assert bool(reset[m].item()) == (cond > 1e7), (m, cond)
+            if cond <= 1e7:
+                np.testing.assert_allclose(coef[m, :cF].cpu().numpy(), cref, rtol=0,
+                                           atol=1e-8 * max(1.0, np.abs(cref).max()))
