@@ -59,6 +59,9 @@ typedef struct {
   const int* tiles;          /* [4*ntiles_total]: mol, row0, col0, nrow | ncol<<8 | diag<<16 */
   const int* csr_off;        /* [natoms+1] per-atom list of tile partial slots */
   const int* csr_idx;        /* slot = tile * 16 + s (s < 8: row r = s, else column s - 8) */
+  const long long* tile_woff; /* [ntiles_total] offset (doubles) of the tile's integrals in its kind's
+                                buffer: NW rows (XX 100, XH 10, HH 1) of round_up(#valid pairs, 16)
+                                doubles, valid lanes packed in lane order */
 } pyseqm_layout_t;
 
 /* Derived per-atom quantities: dd, qq (charge separations), rho0/1/2 (additive terms),
@@ -71,9 +74,12 @@ int pyseqm_atom_setup(const pyseqm_layout_t* L, double* atom_par, void* stream);
  * two_elec_two_center_int (two_elec_two_center_int.py:98-283), diatom_overlap_matrix_PM6_SP
  * (diat_overlap_PM6_SP.py:6-444) and pair_nuclear_energy (energy.py:91-134).
  *   coords   [natoms*3] Angstrom
- *   w_xx     [ntile[0]*6400]  tile-major: w[(tile*100 + 10*k + l)*64 + lane]
- *   w_xh     [ntile[1]*640]
- *   w_hh     [ntile[2]*64]
+ *   w_xx     integral k*10+l of the pair on lane `lane` of tile t at
+ *            w[tile_woff[t] + (k*10 + l)*S_t + rank(lane)], S_t = round_up(#valid lanes, 16),
+ *            rank = position of the lane among the tile's valid lanes in lane order;
+ *            size sum over XX tiles of 100*S_t doubles
+ *   w_xh     same with 10 integrals per pair (sum 10*S_t)
+ *   w_hh     same with 1 integral per pair (sum S_t)
  *   H        packed core Hamiltonian (full symmetric blocks)
  *   part     [ntiles_total*16*10] scratch: per-tile row/column partial sums
  *   enuc_tile[ntiles_total] per-tile core-core repulsion sums

@@ -23,7 +23,7 @@ class Layout(ctypes.Structure):
         ("mol_atom_off", c_void_p), ("mol_norb", c_void_p), ("mol_nocc", c_void_p),
         ("mol_mat_off", c_void_p), ("mol_tile_off", c_void_p), ("atom_mol", c_void_p),
         ("atom_ao", c_void_p), ("atom_Z", c_void_p), ("tiles", c_void_p), ("csr_off", c_void_p),
-        ("csr_idx", c_void_p),
+        ("csr_idx", c_void_p), ("tile_woff", c_void_p),
     ]
 
 

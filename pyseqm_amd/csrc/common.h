@@ -43,6 +43,18 @@ __device__ inline bool lane_pair(const Tile& t, int lane, int& i, int& j) {
   return r < t.nrow && c < t.ncol && (!t.diag || c > r);
 }
 
+// Compact integral storage: a tile keeps one row of S = round_up(#valid pairs, 16) doubles per
+// integral, the valid lanes packed in lane order (row-major over the tile, upper triangle for a
+// diagonal tile), starting at the tile's offset tile_woff[tile] in its kind's buffer.
+__device__ inline int lane_rank(const Tile& t, int lane) {
+  const int r = lane >> 3, c = lane & 7;
+  return t.diag ? r * t.nrow - r * (r + 1) / 2 + (c - r - 1) : r * t.ncol + c;
+}
+__device__ inline int tile_wstride(const Tile& t) {
+  const int nv = t.diag ? t.nrow * (t.nrow - 1) / 2 : t.nrow * t.ncol;
+  return (nv + 15) & ~15;
+}
+
 // Sum over the 8 lanes sharing a row r (lanes 8r..8r+7) / a column c (lanes c, c+8, ...).
 __device__ inline double row_sum(double v) {
   v += __shfl_xor(v, 1);

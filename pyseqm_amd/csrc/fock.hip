@@ -52,7 +52,8 @@ __global__ __launch_bounds__(64) void k_fock_tiles(pyseqm_layout_t L, int tile_b
     const double* Hm = H + mo;
     double* Fm = F + mo;
     if constexpr (KIND == XX) {
-      const double* wt = w + (size_t)tl * 6400 + lane;
+      const int S = tile_wstride(t);
+      const double* wt = w + L.tile_woff[tile] + lane_rank(t, lane);
       double pa[10], pb[10], pab[4][4];
 #pragma unroll
       for (int k = 0; k < 10; ++k) {
@@ -76,7 +77,7 @@ __global__ __launch_bounds__(64) void k_fock_tiles(pyseqm_layout_t L, int tile_b
 #pragma unroll
         for (int l = 0; l < 10; ++l) {
           const int la = kTriI[l], si = kTriJ[l];
-          const double v = wt[(size_t)(k * 10 + l) * 64];
+          const double v = wt[(size_t)(k * 10 + l) * S];
           colv[l] += v * pa[k];
           rowv[k] += v * pb[l];
           K[mu][la] += v * pab[nu][si];
@@ -96,7 +97,8 @@ __global__ __launch_bounds__(64) void k_fock_tiles(pyseqm_layout_t L, int tile_b
           e += pab[a][b] * (h + f);
         }
     } else if constexpr (KIND == XH) {
-      const double* wt = w + (size_t)tl * 640 + lane;
+      const int S = tile_wstride(t);
+      const double* wt = w + L.tile_woff[tile] + lane_rank(t, lane);
       double pa[10], pab[4];
 #pragma unroll
       for (int k = 0; k < 10; ++k) {
@@ -110,7 +112,7 @@ __global__ __launch_bounds__(64) void k_fock_tiles(pyseqm_layout_t L, int tile_b
 #pragma unroll
       for (int k = 0; k < 10; ++k) {
         const int mu = kTriI[k], nu = kTriJ[k];
-        const double v = wt[(size_t)k * 64];
+        const double v = wt[(size_t)k * S];
         colv[0] += v * pa[k];
         rowv[k] = v * pb;
         K[mu] += v * pab[nu];
@@ -125,7 +127,7 @@ __global__ __launch_bounds__(64) void k_fock_tiles(pyseqm_layout_t L, int tile_b
         e += pab[a] * (h + f);
       }
     } else {
-      const double v = w[(size_t)tl * 64 + lane];
+      const double v = w[L.tile_woff[tile] + lane_rank(t, lane)];
       const double pij = Pm[(size_t)ai * n + aj];
       rowv[0] = v * Pm[(size_t)aj * n + aj];
       colv[0] = v * Pm[(size_t)ai * n + ai];
@@ -143,7 +145,7 @@ __global__ __launch_bounds__(64) void k_fock_tiles(pyseqm_layout_t L, int tile_b
 
 // The integral contraction of k_fock_xx2 for the (mu nu) pairs k = K0 .. K0+4 (static indices).
 template <int K0>
-__device__ inline void xx_half(const double* __restrict__ wt, const double* Pm, int n, int ai, int aj,
+__device__ inline void xx_half(const double* __restrict__ wt, int S, const double* Pm, int n, int ai, int aj,
                                const double (&pab)[4][4], double (&rowv)[5], double (&colv)[10], double (&K)[4][4]) {
   double pa[5], pb[10];
 #pragma unroll
@@ -161,7 +163,7 @@ __device__ inline void xx_half(const double* __restrict__ wt, const double* Pm, 
 #pragma unroll
     for (int l = 0; l < 10; ++l) {
       const int la = kTriI[l], si = kTriJ[l];
-      const double v = wt[(size_t)(k * 10 + l) * 64];
+      const double v = wt[(size_t)(k * 10 + l) * S];
       colv[l] += v * pa[kk];
       rowv[kk] += v * pb[l];
       K[mu][la] += v * pab[nu][si];
@@ -207,15 +209,16 @@ __global__ __launch_bounds__(128) void k_fock_xx2(pyseqm_layout_t L, const doubl
     ai = L.atom_ao[i];
     aj = L.atom_ao[j];
     const double* Pm = P + mo;
-    const double* wt = w + (size_t)tile * 6400 + lane;
+    const int S = tile_wstride(t);
+    const double* wt = w + L.tile_woff[tile] + lane_rank(t, lane);
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
       for (int b = 0; b < 4; ++b) pab[a][b] = Pm[(size_t)(ai + a) * n + aj + b];
     if (q == 0)
-      xx_half<0>(wt, Pm, n, ai, aj, pab, rowv, colv, K);
+      xx_half<0>(wt, S, Pm, n, ai, aj, pab, rowv, colv, K);
     else
-      xx_half<5>(wt, Pm, n, ai, aj, pab, rowv, colv, K);
+      xx_half<5>(wt, S, Pm, n, ai, aj, pab, rowv, colv, K);
   }
   if (q == 1) {
 #pragma unroll

@@ -541,7 +541,8 @@ __global__ __launch_bounds__(64) void k_pair_tiles(pyseqm_layout_t L, int method
   for (int k = 0; k < NC; ++k) colv[k] = 0.0;
   double en = 0.0;
   constexpr int NW = KIND == XX ? 100 : (KIND == XH ? 10 : 1);
-  double* wt = wout + (size_t)tl * NW * 64 + lane;
+  const int S = tile_wstride(t);
+  double* wt = wout + L.tile_woff[tile] + (valid ? lane_rank(t, lane) : 0);
 
   if (valid) {
     const double* pi = par + (size_t)i * PYSEQM_NPAR;
@@ -551,14 +552,14 @@ __global__ __launch_bounds__(64) void k_pair_tiles(pyseqm_layout_t L, int method
     double gam = 0.0;
     if constexpr (KIND == XX) {
       xx_integrals(g, pi, pj, [&](int k, int l, double w) {
-        wt[(size_t)(k * 10 + l) * 64] = w;
+        wt[(size_t)(k * 10 + l) * S] = w;
         if (l == 0) rowv[k] = -tj * w;  // e1b: electrons of i, core of j
         if (k == 0) colv[l] = -ti * w;  // e2a: electrons of j, core of i
         if (k == 0 && l == 0) gam = w;
       });
     } else if constexpr (KIND == XH) {
       xh_integrals(g, pi, pj, [&](int k, double w) {
-        wt[(size_t)k * 64] = w;
+        wt[(size_t)k * S] = w;
         rowv[k] = -tj * w;
         if (k == 0) gam = w;
       });
@@ -584,9 +585,6 @@ __global__ __launch_bounds__(64) void k_pair_tiles(pyseqm_layout_t L, int method
       }
     en = core_repulsion(method, (int)pi[PAR_Z], (int)pj[PAR_Z], g.r, gam, pi, pj);
     if (enuc_pair) enuc_pair[pair_ref[(size_t)tile * 64 + lane]] = en;
-  } else {
-#pragma unroll
-    for (int k = 0; k < NW; ++k) wt[(size_t)k * 64] = 0.0;
   }
   write_partials(part, tile, lane, rowv, colv);
   const double es = wave_sum(en);

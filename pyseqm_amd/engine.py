@@ -75,9 +75,9 @@ class SCFEngine:
         self.stream = nat.stream_ptr(dev)
         nat.call("pyseqm_atom_setup", b.L, nat.ptr(self.atom_par), self.stream)
         nt = b.ntiles
-        self.w_xx = torch.empty(max(b.ntile[0], 1) * 6400, **f64)
-        self.w_xh = torch.empty(max(b.ntile[1], 1) * 640, **f64)
-        self.w_hh = torch.empty(max(b.ntile[2], 1) * 64, **f64)
+        self.w_xx = torch.empty(max(b.w_size[0], 1), **f64)
+        self.w_xh = torch.empty(max(b.w_size[1], 1), **f64)
+        self.w_hh = torch.empty(max(b.w_size[2], 1), **f64)
         self.H = torch.empty(max(b.total, 1), **f64)
         self.part = torch.empty(max(nt, 1) * 160, **f64)
         self.tile_v = torch.empty(max(nt, 1), **f64)
@@ -372,21 +372,15 @@ class SCFEngine:
         b = self.b
         dev = self.device
         w = torch.zeros(max(b.npairs, 1), 10, 10, dtype=torch.float64, device=dev)
-        ref = torch.as_tensor(b.pair_ref_host, device=dev).long()
-        valid = torch.as_tensor(b.lane_valid, device=dev)
-        nt = b.ntile
-        lo = 0
-        for kind, buf, nw in ((0, self.w_xx, 100), (1, self.w_xh, 10), (2, self.w_hh, 1)):
-            if nt[kind] == 0:
+        for kind, buf in enumerate((self.w_xx, self.w_xh, self.w_hh)):
+            ref, idx = b.w_gather_index(kind, dev)
+            if ref is None:
                 continue
-            rows = buf[:nt[kind] * nw * 64].view(nt[kind], nw, 64).permute(0, 2, 1).reshape(-1, nw)
-            sel = slice(lo * 64, (lo + nt[kind]) * 64)
-            v, r = valid[sel], ref[sel]
+            rows = buf[idx]
             if kind == 0:
-                w[r[v]] = rows[v].view(-1, 10, 10)
+                w[ref] = rows.view(-1, 10, 10)
             elif kind == 1:
-                w[r[v], :, 0] = rows[v]
+                w[ref, :, 0] = rows
             else:
-                w[r[v], 0, 0] = rows[v][:, 0]
-            lo += nt[kind]
+                w[ref, 0, 0] = rows[:, 0]
         return w[:b.npairs]

@@ -15,6 +15,7 @@ from .seqm_functions.constants import QN, TORE, MAX_Z
 from .seqm_functions.parameters import supported_elements
 
 TILE = 8
+W_PER_LANE = (100, 10, 1)  # integrals stored per atom pair: heavy-heavy, heavy-H, H-H
 
 
 def check_species(species):
@@ -175,6 +176,23 @@ class PackedBatch:
         ref = self.pair_base[mol] + li * nat - li * (li + 1) // 2 + (lj - li - 1)
         self.pair_ref_host = np.where(valid, ref, -1).reshape(-1)
         self.lane_valid = valid.reshape(-1)
+        # compact integral storage: per tile NW rows of S = round_up(#valid lanes, 16) doubles,
+        # valid lanes packed in lane order; offsets are local to each kind's buffer
+        nvalid = valid.sum(1).astype(np.int64)
+        S = (nvalid + 15) // 16 * 16
+        kind = np.concatenate([np.full(n, k) for k, n in enumerate(self.ntile)]).astype(np.int64)
+        nw = np.asarray(W_PER_LANE, dtype=np.int64)[kind] if nt else np.zeros(0, np.int64)
+        woff = np.zeros(nt, dtype=np.int64)
+        self.w_size = [0, 0, 0]
+        for k in range(3):
+            sel = slice(base[k], base[k + 1])
+            blk = S[sel] * nw[sel]
+            woff[sel] = np.concatenate([[0], np.cumsum(blk)[:-1]]) if len(blk) else blk
+            self.w_size[k] = int(blk.sum())
+        self.tile_woff_host = woff
+        self.tile_wstride = S
+        rank = np.cumsum(valid, axis=1) - 1
+        self.lane_wpos = np.where(valid, woff[:, None] + rank, -1).reshape(-1)
         # partial slots: row r of tile t <-> atom row0 + r, column c <-> atom col0 + c
         tid = np.repeat(np.arange(nt), 16)
         s = np.tile(np.arange(16), nt)
@@ -196,6 +214,8 @@ class PackedBatch:
             "mol_tile_off": i32(self.mol_tile_off), "atom_mol": i32(self.atom_mol), "atom_ao": i32(self.atom_ao),
             "atom_Z": i32(self.Z), "csr_off": i32(self.csr_off_host), "csr_idx": i32(self.csr_idx_host),
             "pair_ref": i32(self.pair_ref_host),
+            "tile_woff": torch.as_tensor(self.tile_woff_host if len(self.tile_woff_host) else np.zeros(1, np.int64),
+                                         dtype=torch.int64, device=dev),
         }
         T = self.tiles_host
         packed = np.zeros((len(T), 4), dtype=np.int64)
@@ -210,7 +230,7 @@ class PackedBatch:
         for k in range(3):
             L.ntile[k] = self.ntile[k]
         for name in ("mol_atom_off", "mol_norb", "mol_nocc", "mol_mat_off", "mol_tile_off", "atom_mol",
-                     "atom_ao", "atom_Z", "tiles", "csr_off", "csr_idx"):
+                     "atom_ao", "atom_Z", "tiles", "csr_off", "csr_idx", "tile_woff"):
             setattr(L, name, self.d[name].data_ptr())
         self.L = L
 
@@ -236,6 +256,20 @@ class PackedBatch:
         self.L.mol_nocc = self.d["mol_nocc"].data_ptr()
 
     # ------------------------------------------------------------- helpers
+    def w_gather_index(self, kind, device):
+        """(pair index, (nvalid, NW) positions in the kind's integral buffer) of every valid lane
+        of the tiles of one kind, or (None, None) when the kind has no tiles."""
+        lo, hi = int(self.tile_base[kind]), int(self.tile_base[kind]) + self.ntile[kind]
+        if hi == lo:
+            return None, None
+        sel = slice(lo * 64, hi * 64)
+        v = self.lane_valid[sel]
+        pos = self.lane_wpos[sel][v]
+        S = np.repeat(self.tile_wstride[lo:hi], 64)[v]
+        idx = pos[:, None] + np.arange(W_PER_LANE[kind])[None, :] * S[:, None]
+        return (torch.as_tensor(self.pair_ref_host[sel][v], dtype=torch.int64, device=device),
+                torch.as_tensor(idx, dtype=torch.int64, device=device))
+
     @property
     def ntiles(self):
         return sum(self.ntile)

@@ -45,26 +45,18 @@ def compact_w(b, w, idxi, idxj):
     full = torch.as_tensor(b.pair_base, device=dev)[mol] + li * nat_m - li * (li + 1) // 2 + (lj - li - 1)
     wf = torch.zeros(max(b.npairs, 1), 10, 10, dtype=torch.float64, device=dev)
     wf[full] = w.to(torch.float64)
-    ref = torch.as_tensor(b.pair_ref_host, device=dev).long()
-    valid = torch.as_tensor(b.lane_valid, device=dev)
     out = []
-    lo = 0
-    for kind, nw in ((0, 100), (1, 10), (2, 1)):
-        nt = b.ntile[kind]
-        buf = torch.zeros(max(nt, 1) * nw * 64, dtype=torch.float64, device=dev)
-        if nt:
-            sel = slice(lo * 64, (lo + nt) * 64)
-            v, r = valid[sel], ref[sel]
-            rows = torch.zeros(nt * 64, nw, dtype=torch.float64, device=dev)
+    for kind in range(3):
+        buf = torch.zeros(max(b.w_size[kind], 1), dtype=torch.float64, device=dev)
+        ref, idx = b.w_gather_index(kind, dev)
+        if ref is not None:
             if kind == 0:
-                rows[v] = wf[r[v]].reshape(-1, 100)
+                buf[idx] = wf[ref].reshape(-1, 100)
             elif kind == 1:
-                rows[v] = wf[r[v], :, 0]
+                buf[idx] = wf[ref, :, 0]
             else:
-                rows[v] = wf[r[v], 0, 0].unsqueeze(1)
-            buf[:nt * nw * 64] = rows.view(nt, 64, nw).permute(0, 2, 1).reshape(-1)
+                buf[idx] = wf[ref, 0, 0].unsqueeze(1)
         out.append(buf)
-        lo += nt
     return out
 
 
