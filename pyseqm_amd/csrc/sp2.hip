@@ -180,15 +180,21 @@ __device__ inline void tri_decode(int t, int& ti, int& tj) {
   ti = t - tj * (tj + 1) / 2;
 }
 
-__global__ __launch_bounds__(256) void k_sp2_bf16x3(const int* __restrict__ idx, const int* __restrict__ norb,
-                                                    const long long* __restrict__ xoff,
-                                                    const int* __restrict__ xld, const float* __restrict__ Xin,
-                                                    float* __restrict__ Xout, double* __restrict__ st,
-                                                    double* __restrict__ part, int part_stride) {
-  // double-buffered slabs: [buffer][hi/lo][row][k]; 160 KiB for two blocks per CU
-  __shared__ __attribute__((aligned(16))) __bf16 As[2][2][BT][LDSK];
-  __shared__ __attribute__((aligned(16))) __bf16 Bs[2][2][BT][LDSK];
-  double(*red)[4] = reinterpret_cast<double(*)[4]>(&As[0][0][0][0]);  // reused after the main loop
+__global__ __launch_bounds__(256, 2) void k_sp2_bf16x3(const int* __restrict__ idx, const int* __restrict__ norb,
+                                                       const long long* __restrict__ xoff,
+                                                       const int* __restrict__ xld, const float* __restrict__ Xin,
+                                                       float* __restrict__ Xout, double* __restrict__ st,
+                                                       double* __restrict__ part, int part_stride) {
+  // double-buffered slabs: [buffer][hi/lo][row][k], 80 KiB so that two blocks share a CU; after the
+  // main loop the same LDS holds the output tile (128 x 129 floats) for the mirrored write and
+  // the per-wave trace / Frobenius partials
+  __shared__ __attribute__((aligned(16))) __bf16 smem[2 * 2 * 2 * BT * LDSK];
+  __bf16(*As)[2][BT][LDSK] = reinterpret_cast<__bf16(*)[2][BT][LDSK]>(smem);
+  __bf16(*Bs)[2][BT][LDSK] = reinterpret_cast<__bf16(*)[2][BT][LDSK]>(smem + 2 * 2 * BT * LDSK);
+  float* T = reinterpret_cast<float*>(smem);
+  double(*red)[4] = reinterpret_cast<double(*)[4]>(T + BT * (BT + 1));
+  static_assert(BT * (BT + 1) * sizeof(float) + 8 * sizeof(double) <= sizeof(__bf16) * 2 * 2 * 2 * BT * LDSK,
+                "epilogue buffers exceed the slab LDS");
   const int tile = blockIdx.x;
   const int m = idx[blockIdx.y];
   const double* s = st + (size_t)m * S_NST;
@@ -282,26 +288,49 @@ __global__ __launch_bounds__(256) void k_sp2_bf16x3(const int* __restrict__ idx,
     __syncthreads();
     cur ^= 1;
   }
-  // epilogue: Y = X^2 or 2X - X^2, mirror write, FP64 partial trace / Frobenius norm
+  // epilogue: Y = X^2 or 2X - X^2, FP64 partial trace / Frobenius norm; the tile is written
+  // directly and kept in LDS for the mirrored (transposed) write
   double tr = 0.0, fro = 0.0;
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b)
+    for (int b = 0; b < 2; ++b) {
+      // X of the 2X - X^2 branch for this 32 x 32 block: all 16 loads issue before the first use
+      // (clamped addresses keep the masked-off elements in bounds)
+      float xv[16];
+      if (!sq) {
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+          const int r = min(i0 + wr * 64 + a * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3), n - 1);
+          const int c = min(j0 + wc * 64 + b * 32 + (lane & 31), n - 1);
+          xv[v] = Xm[(size_t)r * ld + c];
+        }
+      }
 #pragma unroll
       for (int v = 0; v < 16; ++v) {
         const int r = i0 + wr * 64 + a * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
         const int c = j0 + wc * 64 + b * 32 + (lane & 31);
         if (r < n && c < n && (ti != tj || r <= c)) {
           const float x2 = acc[a][b][v];
-          const float y = sq ? x2 : 2.0f * Xm[(size_t)r * ld + c] - x2;
+          const float y = sq ? x2 : 2.0f * xv[v] - x2;
           Ym[(size_t)r * ld + c] = y;
-          Ym[(size_t)c * ld + r] = y;
+          T[(r - i0) * (BT + 1) + (c - j0)] = y;
           const double w = (r == c) ? 1.0 : 2.0;
           fro += w * (double)y * (double)y;
           if (r == c) tr += y;
         }
       }
+    }
+  __syncthreads();
+  {
+    // mirror: Y[j0 + cc][i0 + rr] = tile[rr][cc], consecutive lanes along rr (coalesced rows of Y,
+    // conflict-free LDS reads with the 129-float row pitch)
+    const int rmax = min(BT, n - i0), cmax = min(BT, n - j0);
+    for (int e = tid; e < BT * BT; e += 256) {
+      const int cc = e / BT, rr = e % BT;
+      if (rr < rmax && cc < cmax && (ti != tj || rr < cc)) Ym[(size_t)(j0 + cc) * ld + i0 + rr] = T[rr * (BT + 1) + cc];
+    }
+  }
   tr = wsum(tr);
   fro = wsum(fro);
   if (lane == 0) {
@@ -351,11 +380,16 @@ __global__ __launch_bounds__(256) void k_gemm_bf16x3_f64(const int* __restrict__
     for (int q = 0; q < 4; ++q) {
       const int f = tid + 256 * q;
       const int row = f >> 3, k = k0 + 4 * (f & 7);
+      // unconditional loads from clamped addresses (all of them in flight at once), then masked
       float x[4], y[4];
+      const int ra_ = min(i0 + row, n - 1), rb_ = min(j0 + row, n - 1);
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        x[u] = (i0 + row < n && k + u < n) ? (float)Am[(size_t)(i0 + row) * n + k + u] : 0.0f;
-        y[u] = (j0 + row < n && k + u < n) ? (float)Bm[(size_t)(j0 + row) * n + k + u] : 0.0f;
+        const int kc = min(k + u, n - 1);
+        const double va = Am[(size_t)ra_ * n + kc];
+        const double vb = Bm[(size_t)rb_ * n + kc];
+        x[u] = (i0 + row < n && k + u < n) ? (float)va : 0.0f;
+        y[u] = (j0 + row < n && k + u < n) ? (float)vb : 0.0f;
       }
       ra[q] = make_float4(x[0], x[1], x[2], x[3]);
       rb[q] = make_float4(y[0], y[1], y[2], y[3]);
@@ -479,7 +513,7 @@ __global__ __launch_bounds__(64) void k_sp2_decide(const int* __restrict__ idx, 
 // mode bit 0: McWeeny epilogue C <- 2 (3 A - 2 A B) (A = X^2, B = X); bit 1: the product is known
 // to be symmetric (X^2, X^2 X): only tiles ti <= tj are computed and mirrored.
 template <int DT>
-__global__ __launch_bounds__(256) void k_dgemm_sym(const int* __restrict__ idx, const int* __restrict__ norb,
+__global__ __launch_bounds__(256, 2) void k_dgemm_sym(const int* __restrict__ idx, const int* __restrict__ norb,
                                                    const long long* __restrict__ off, const int* __restrict__ lds,
                                                    const double* __restrict__ A, const double* __restrict__ B,
                                                    double* __restrict__ C, int mode, int ntile_side) {
@@ -517,8 +551,11 @@ __global__ __launch_bounds__(256) void k_dgemm_sym(const int* __restrict__ idx, 
     for (int q = 0; q < NQ; ++q) {
       const int e = tid + 256 * q;
       const int row = e >> 4, gk = k0 + (e & 15);
-      ra[q] = (i0 + row < n && gk < n) ? Am[(size_t)(i0 + row) * ld + gk] : 0.0;
-      rb[q] = (j0 + row < n && gk < n) ? Bm[(size_t)(j0 + row) * ld + gk] : 0.0;
+      const int kc = min(gk, n - 1);
+      const double va = Am[(size_t)min(i0 + row, n - 1) * ld + kc];
+      const double vb = Bm[(size_t)min(j0 + row, n - 1) * ld + kc];
+      ra[q] = (i0 + row < n && gk < n) ? va : 0.0;
+      rb[q] = (j0 + row < n && gk < n) ? vb : 0.0;
     }
   };
   gload(0);
@@ -550,18 +587,29 @@ __global__ __launch_bounds__(256) void k_dgemm_sym(const int* __restrict__ idx, 
 #pragma unroll
   for (int a = 0; a < NB; ++a)
 #pragma unroll
-    for (int b = 0; b < NB; ++b)
+    for (int b = 0; b < NB; ++b) {
+      // A of the McWeeny form for this 16 x 16 block, loaded before the first use
+      double av[4];
+      if (mode & 1) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int r = min(i0 + wr * WT + a * 16 + 4 * v + (lane >> 4), n - 1);
+          const int c = min(j0 + wc * WT + b * 16 + (lane & 15), n - 1);
+          av[v] = Am[(size_t)r * ld + c];
+        }
+      }
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int r = i0 + wr * WT + a * 16 + 4 * v + (lane >> 4);
         const int c = j0 + wc * WT + b * 16 + (lane & 15);
         if (r < n && c < n) {
           const double x = acc[a][b][v];
-          const double y = (mode & 1) ? 2.0 * (3.0 * Am[(size_t)r * ld + c] - 2.0 * x) : x;
+          const double y = (mode & 1) ? 2.0 * (3.0 * av[v] - 2.0 * x) : x;
           Cm[(size_t)r * ld + c] = y;
           if (symout && ti != tj) Cm[(size_t)c * ld + r] = y;
         }
       }
+    }
 }
 
 // final FP32 X of each molecule (in buffer Xa or Xb by the parity of its iteration count) ->
