@@ -193,7 +193,10 @@ int pyseqm_sp2_init(const pyseqm_layout_t* L, const double* F, const int* idx, i
                     const long long* xoff, const int* xld, float* X, double* state, double* red, int nb,
                     void* stream);
 /* One SP2 iteration for every active, not yet stopped molecule: Xout = Xin^2 or 2 Xin - Xin^2,
- * then the device-side branch / stop decision; *ndone counts molecules that stopped. */
+ * then the device-side branch / stop decision; *ndone counts molecules that stopped.
+ * The product runs over upper-triangle output tiles of edge tb = 64 when nmax <= 256, else 128;
+ * part holds 2 doubles per tile and molecule: part_stride >= nt (nt + 1), nt = ceil(nmax / tb)
+ * (returns PYSEQM_EINVAL otherwise). */
 int pyseqm_sp2_iterate(const pyseqm_layout_t* L, const int* idx, int nact, int nmax, const long long* xoff,
                        const int* xld, const float* Xin, float* Xout, double* state, double* part,
                        int part_stride, double eps, int max_iter, int* ndone, void* stream);

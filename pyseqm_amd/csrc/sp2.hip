@@ -171,6 +171,7 @@ __global__ void k_pad_fill(pyseqm_layout_t L, const double* __restrict__ F, cons
 constexpr int BT = 128;       // output tile edge
 constexpr int BK = 32;        // K slab
 constexpr int LDSK = BK + 8;  // padded LDS row (bf16): 80 bytes, 16-byte aligned
+constexpr int kSp2SmallTileMaxN = 256;  // largest active block that still uses 64 x 64 SP2 tiles
 
 // upper-triangle tile enumeration t = tj*(tj+1)/2 + ti, ti <= tj
 __device__ inline void tri_decode(int t, int& ti, int& tj) {
@@ -180,50 +181,59 @@ __device__ inline void tri_decode(int t, int& ti, int& tj) {
   ti = t - tj * (tj + 1) / 2;
 }
 
-__global__ __launch_bounds__(256, 2) void k_sp2_bf16x3(const int* __restrict__ idx, const int* __restrict__ norb,
-                                                       const long long* __restrict__ xoff,
-                                                       const int* __restrict__ xld, const float* __restrict__ Xin,
-                                                       float* __restrict__ Xout, double* __restrict__ st,
-                                                       double* __restrict__ part, int part_stride) {
-  // double-buffered slabs: [buffer][hi/lo][row][k], 80 KiB so that two blocks share a CU; after the
-  // main loop the same LDS holds the output tile (128 x 129 floats) for the mirrored write and
-  // the per-wave trace / Frobenius partials
-  __shared__ __attribute__((aligned(16))) __bf16 smem[2 * 2 * 2 * BT * LDSK];
-  __bf16(*As)[2][BT][LDSK] = reinterpret_cast<__bf16(*)[2][BT][LDSK]>(smem);
-  __bf16(*Bs)[2][BT][LDSK] = reinterpret_cast<__bf16(*)[2][BT][LDSK]>(smem + 2 * 2 * BT * LDSK);
+// One SP2 iteration Y = X^2 (or 2X - X^2) over the upper-triangle TB x TB output tiles of every
+// active molecule: 4 waves in 2 x 2, each TB/2 x TB/2 of 32 x 32 MFMA blocks.  Consecutive entries
+// of the molecule-major tile list are given to one XCD (block id mod 8), so the tiles that share a
+// molecule's row panels share that XCD's L2.
+template <int TB>
+__global__ __launch_bounds__(256, TB == 128 ? 2 : 4) void k_sp2_bf16x3(
+    const int* __restrict__ idx, int nact, int ntri, const int* __restrict__ norb,
+    const long long* __restrict__ xoff, const int* __restrict__ xld, const float* __restrict__ Xin,
+    float* __restrict__ Xout, double* __restrict__ st, double* __restrict__ part, int part_stride) {
+  constexpr int NW = TB / 64;              // 32 x 32 blocks per wave side
+  constexpr int NQ = TB * (BK / 4) / 256;  // float4 slab loads per thread and operand
+  // double-buffered slabs: [buffer][hi/lo][row][k] (80 KiB at TB = 128, so two blocks share a CU);
+  // after the main loop the same LDS holds the output tile (TB x (TB+1) floats) for the mirrored
+  // write and the per-wave trace / Frobenius partials
+  __shared__ __attribute__((aligned(16))) __bf16 smem[2 * 2 * 2 * TB * LDSK];
+  __bf16(*As)[2][TB][LDSK] = reinterpret_cast<__bf16(*)[2][TB][LDSK]>(smem);
+  __bf16(*Bs)[2][TB][LDSK] = reinterpret_cast<__bf16(*)[2][TB][LDSK]>(smem + 2 * 2 * TB * LDSK);
   float* T = reinterpret_cast<float*>(smem);
-  double(*red)[4] = reinterpret_cast<double(*)[4]>(T + BT * (BT + 1));
-  static_assert(BT * (BT + 1) * sizeof(float) + 8 * sizeof(double) <= sizeof(__bf16) * 2 * 2 * 2 * BT * LDSK,
+  double(*red)[4] = reinterpret_cast<double(*)[4]>(T + TB * (TB + 1));
+  static_assert(TB * (TB + 1) * sizeof(float) + 8 * sizeof(double) <= sizeof(__bf16) * 2 * 2 * 2 * TB * LDSK,
                 "epilogue buffers exceed the slab LDS");
-  const int tile = blockIdx.x;
-  const int m = idx[blockIdx.y];
+  const int total = nact * ntri, per = (total + 7) / 8;
+  const int w = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+  if (w >= total) return;
+  const int tile = w % ntri;
+  const int m = idx[w / ntri];
   const double* s = st + (size_t)m * S_NST;
   if (s[S_DONE] != 0.0) return;
   const int n = norb[m];
   const int ld = xld[m];
   int ti, tj;
   tri_decode(tile, ti, tj);
-  if (tj * BT >= n) return;
+  if (tj * TB >= n) return;
   const bool sq = s[S_BRANCH] != 0.0;
   const float* Xm = Xin + xoff[m];
   float* Ym = Xout + xoff[m];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wr = wave >> 1, wc = wave & 1;  // 2x2 waves, 64x64 each
-  const int i0 = ti * BT, j0 = tj * BT;
-  floatx16 acc[2][2];
+  const int wr = wave >> 1, wc = wave & 1;
+  const int i0 = ti * TB, j0 = tj * TB;
+  floatx16 acc[NW][NW];
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < NW; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b)
+    for (int b = 0; b < NW; ++b)
 #pragma unroll
       for (int v = 0; v < 16; ++v) acc[a][b][v] = 0.0f;
 
   // global -> registers (next slab, in flight while the MFMAs of this slab run) -> LDS as bf16 hi/lo
-  float4 ra[4], rb[4];
+  float4 ra[NQ], rb[NQ];
   auto gload = [&](int k0) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int f = tid + 256 * q;  // float4 index in the 128 x 8 slab
+    for (int q = 0; q < NQ; ++q) {
+      const int f = tid + 256 * q;  // float4 index in the TB x 8 slab
       const int row = f >> 3, kq = f & 7;
       ra[q] = make_float4(0.f, 0.f, 0.f, 0.f);
       rb[q] = ra[q];
@@ -247,7 +257,7 @@ __global__ __launch_bounds__(256, 2) void k_sp2_bf16x3(const int* __restrict__ i
   };
   auto stage = [&](int buf) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < NQ; ++q) {
       const int f = tid + 256 * q;
       const int row = f >> 3, kq = f & 7;
       split_store(ra[q], As[buf][0], As[buf][1], row, 4 * kq);
@@ -264,21 +274,21 @@ __global__ __launch_bounds__(256, 2) void k_sp2_bf16x3(const int* __restrict__ i
     if (more) gload(k0 + BK);
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 16) {
-      bf16x8 ah[2], al[2], bh[2], bl[2];
+      bf16x8 ah[NW], al[NW], bh[NW], bl[NW];
       const int kofs = kk + 8 * (lane >> 5);
 #pragma unroll
-      for (int a = 0; a < 2; ++a) {
-        const int r = wr * 64 + a * 32 + (lane & 31);
+      for (int a = 0; a < NW; ++a) {
+        const int r = wr * (TB / 2) + a * 32 + (lane & 31);
         ah[a] = *reinterpret_cast<const bf16x8*>(&As[cur][0][r][kofs]);
         al[a] = *reinterpret_cast<const bf16x8*>(&As[cur][1][r][kofs]);
-        const int c = wc * 64 + a * 32 + (lane & 31);
+        const int c = wc * (TB / 2) + a * 32 + (lane & 31);
         bh[a] = *reinterpret_cast<const bf16x8*>(&Bs[cur][0][c][kofs]);
         bl[a] = *reinterpret_cast<const bf16x8*>(&Bs[cur][1][c][kofs]);
       }
 #pragma unroll
-      for (int a = 0; a < 2; ++a)
+      for (int a = 0; a < NW; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b) {
+        for (int b = 0; b < NW; ++b) {
           acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[a], bh[b], acc[a][b], 0, 0, 0);
           acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a], bl[b], acc[a][b], 0, 0, 0);
           acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a], bh[b], acc[a][b], 0, 0, 0);
@@ -292,31 +302,31 @@ __global__ __launch_bounds__(256, 2) void k_sp2_bf16x3(const int* __restrict__ i
   // directly and kept in LDS for the mirrored (transposed) write
   double tr = 0.0, fro = 0.0;
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < NW; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b) {
+    for (int b = 0; b < NW; ++b) {
       // X of the 2X - X^2 branch for this 32 x 32 block: all 16 loads issue before the first use
       // (clamped addresses keep the masked-off elements in bounds)
       float xv[16];
       if (!sq) {
 #pragma unroll
         for (int v = 0; v < 16; ++v) {
-          const int r = min(i0 + wr * 64 + a * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3), n - 1);
-          const int c = min(j0 + wc * 64 + b * 32 + (lane & 31), n - 1);
+          const int r = min(i0 + wr * (TB / 2) + a * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3), n - 1);
+          const int c = min(j0 + wc * (TB / 2) + b * 32 + (lane & 31), n - 1);
           xv[v] = Xm[(size_t)r * ld + c];
         }
       }
 #pragma unroll
       for (int v = 0; v < 16; ++v) {
-        const int r = i0 + wr * 64 + a * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
-        const int c = j0 + wc * 64 + b * 32 + (lane & 31);
+        const int r = i0 + wr * (TB / 2) + a * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+        const int c = j0 + wc * (TB / 2) + b * 32 + (lane & 31);
         if (r < n && c < n && (ti != tj || r <= c)) {
           const float x2 = acc[a][b][v];
           const float y = sq ? x2 : 2.0f * xv[v] - x2;
           Ym[(size_t)r * ld + c] = y;
-          T[(r - i0) * (BT + 1) + (c - j0)] = y;
-          const double w = (r == c) ? 1.0 : 2.0;
-          fro += w * (double)y * (double)y;
+          T[(r - i0) * (TB + 1) + (c - j0)] = y;
+          const double wgt = (r == c) ? 1.0 : 2.0;
+          fro += wgt * (double)y * (double)y;
           if (r == c) tr += y;
         }
       }
@@ -324,11 +334,11 @@ __global__ __launch_bounds__(256, 2) void k_sp2_bf16x3(const int* __restrict__ i
   __syncthreads();
   {
     // mirror: Y[j0 + cc][i0 + rr] = tile[rr][cc], consecutive lanes along rr (coalesced rows of Y,
-    // conflict-free LDS reads with the 129-float row pitch)
-    const int rmax = min(BT, n - i0), cmax = min(BT, n - j0);
-    for (int e = tid; e < BT * BT; e += 256) {
-      const int cc = e / BT, rr = e % BT;
-      if (rr < rmax && cc < cmax && (ti != tj || rr < cc)) Ym[(size_t)(j0 + cc) * ld + i0 + rr] = T[rr * (BT + 1) + cc];
+    // conflict-free LDS reads with the TB+1 float row pitch)
+    const int rmax = min(TB, n - i0), cmax = min(TB, n - j0);
+    for (int e = tid; e < TB * TB; e += 256) {
+      const int cc = e / TB, rr = e % TB;
+      if (rr < rmax && cc < cmax && (ti != tj || rr < cc)) Ym[(size_t)(j0 + cc) * ld + i0 + rr] = T[rr * (TB + 1) + cc];
     }
   }
   tr = wsum(tr);
@@ -470,7 +480,7 @@ __global__ __launch_bounds__(256) void k_gemm_bf16x3_f64(const int* __restrict__
 __global__ __launch_bounds__(64) void k_sp2_decide(const int* __restrict__ idx, int nact,
                                                    const int* __restrict__ norb, const int* __restrict__ nocc,
                                                    double* __restrict__ st, const double* __restrict__ part,
-                                                   int part_stride, double eps, int max_iter,
+                                                   int part_stride, int tb, double eps, int max_iter,
                                                    int* __restrict__ ndone) {
   // one wave per molecule: lanes take tile partials t = lane, lane + 64, ... and a fixed shuffle
   // tree sums them (deterministic)
@@ -479,7 +489,7 @@ __global__ __launch_bounds__(64) void k_sp2_decide(const int* __restrict__ idx, 
   double* s = st + (size_t)m * S_NST;
   if (s[S_DONE] != 0.0) return;  // uniform over the wave
   const int n = norb[m];
-  const int nt = (n + BT - 1) / BT;
+  const int nt = (n + tb - 1) / tb;
   const int ntri = nt * (nt + 1) / 2;
   double tr = 0.0, fro = 0.0;
   const double* p = part + (size_t)m * part_stride;
@@ -674,13 +684,22 @@ extern "C" int pyseqm_sp2_iterate(const pyseqm_layout_t* L, const int* idx, int 
                                   int* ndone, void* stream) {
   if (!L || nact < 0 || nmax <= 0) return PYSEQM_EINVAL;
   if (nact == 0) return PYSEQM_OK;
-  const int nt = (nmax + BT - 1) / BT;
+  // 64 x 64 output tiles while the largest active molecule is small (less zero padding in the
+  // MFMA work of the many small blocks), 128 x 128 tiles above that
+  const int tb = nmax <= kSp2SmallTileMaxN ? 64 : 128;
+  const int nt = (nmax + tb - 1) / tb;
+  const int ntri = nt * (nt + 1) / 2;
   if (part_stride < nt * (nt + 1)) return PYSEQM_EINVAL;
   hipStream_t s = (hipStream_t)stream;
-  k_sp2_bf16x3<<<dim3(nt * (nt + 1) / 2, nact), 256, 0, s>>>(idx, L->mol_norb, xoff, xld, Xin, Xout, state, part,
-                                                               part_stride);
-  k_sp2_decide<<<nact, 64, 0, s>>>(idx, nact, L->mol_norb, L->mol_nocc, state, part, part_stride, eps, max_iter,
-                                   ndone);
+  const int grid = 8 * ((nact * ntri + 7) / 8);
+  if (tb == 64)
+    k_sp2_bf16x3<64><<<grid, 256, 0, s>>>(idx, nact, ntri, L->mol_norb, xoff, xld, Xin, Xout, state, part,
+                                          part_stride);
+  else
+    k_sp2_bf16x3<128><<<grid, 256, 0, s>>>(idx, nact, ntri, L->mol_norb, xoff, xld, Xin, Xout, state, part,
+                                           part_stride);
+  k_sp2_decide<<<nact, 64, 0, s>>>(idx, nact, L->mol_norb, L->mol_nocc, state, part, part_stride, tb, eps,
+                                   max_iter, ndone);
   return st_ok();
 }
 

@@ -32,7 +32,9 @@ class SP2Workspace:
         self.Y2 = torch.zeros_like(self.Y)
         self.Y3 = torch.zeros_like(self.Y)
         self.state = torch.zeros(b.nmol, NSTATE, dtype=torch.float64, device=dev)
-        nt = (b.nmax + 127) // 128
+        # per-tile (trace, Frobenius) partials; sized for the 64 x 64 tiles (the 128 x 128 tiles
+        # used for large blocks need fewer)
+        nt = (b.nmax + 63) // 64
         self.part_stride = max(nt * (nt + 1), 2)
         self.part = torch.zeros(b.nmol, self.part_stride, dtype=torch.float64, device=dev)
         self.ndone = torch.zeros(1, dtype=torch.int32, device=dev)
