@@ -198,19 +198,54 @@ __global__ __launch_bounds__(256) void k_gemm64(pyseqm_layout_t L, const int* __
 }
 
 // C = X - X^T (X = F P, so C = FP - PF) into the ring slot, with per-block partial max|C|.
-__global__ void k_antisym(pyseqm_layout_t L, const int* __restrict__ idx, int nb, const double* __restrict__ X,
-                          double* __restrict__ Cout, double* __restrict__ red) {
+// C = X - X^T and per-block max|C| over the active blocks.  Work unit: a pair of mirrored 32 x 32
+// tiles (I, J), I <= J, of one molecule; both are read as coalesced rows into LDS and both output
+// tiles are written as rows (C_JI = -C_IJ^T), so no element of X is read down a column of HBM.
+__global__ __launch_bounds__(kEl) void k_antisym(pyseqm_layout_t L, const int* __restrict__ idx, int nb,
+                                                 const double* __restrict__ X, double* __restrict__ Cout,
+                                                 double* __restrict__ red) {
+  __shared__ double A[32][33], B[32][33];
   __shared__ double sh[kEl / 64];
   const int k = blockIdx.y;
   const int m = idx[k];
   const int n = L.mol_norb[m];
-  const size_t off = L.mol_mat_off[m];
+  const double* Xm = X + L.mol_mat_off[m];
+  double* Cm = Cout + L.mol_mat_off[m];
+  const int T = (n + 31) / 32;
+  const int ntri = T * (T + 1) / 2;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 8 rows of 32 per pass
   double mx = 0.0;
-  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < (size_t)n * n; e += (size_t)nb * blockDim.x) {
-    const int r = (int)(e / n), c = (int)(e % n);
-    const double v = X[off + e] - X[off + (size_t)c * n + r];
-    Cout[off + e] = v;
-    mx = fmax(mx, fabs(v));
+  for (int t = blockIdx.x; t < ntri; t += nb) {
+    int tj = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+    while ((tj + 1) * (tj + 2) / 2 <= t) ++tj;
+    while (tj * (tj + 1) / 2 > t) --tj;
+    const int ti = t - tj * (tj + 1) / 2;
+    const int i0 = ti * 32, j0 = tj * 32;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int rr = ty + 8 * q;
+      // unconditional loads from clamped addresses, masked afterwards
+      const double va = Xm[(size_t)min(i0 + rr, n - 1) * n + min(j0 + tx, n - 1)];
+      const double vb = Xm[(size_t)min(j0 + rr, n - 1) * n + min(i0 + tx, n - 1)];
+      A[rr][tx] = (i0 + rr < n && j0 + tx < n) ? va : 0.0;
+      B[rr][tx] = (j0 + rr < n && i0 + tx < n) ? vb : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int rr = ty + 8 * q;
+      if (i0 + rr < n && j0 + tx < n) {
+        const double v = A[rr][tx] - B[tx][rr];  // X[r][c] - X[c][r]
+        Cm[(size_t)(i0 + rr) * n + j0 + tx] = v;
+        mx = fmax(mx, fabs(v));
+      }
+      if (ti != tj && j0 + rr < n && i0 + tx < n) {
+        const double v = B[rr][tx] - A[tx][rr];
+        Cm[(size_t)(j0 + rr) * n + i0 + tx] = v;
+        mx = fmax(mx, fabs(v));
+      }
+    }
+    __syncthreads();
   }
   const double r = block_max(mx, sh);
   if (threadIdx.x == 0) red[(size_t)k * nb + blockIdx.x] = r;

@@ -100,8 +100,10 @@ __global__ void k_sp2_x0(pyseqm_layout_t L, const double* __restrict__ F, const 
   gersh_bounds(red, k, nb, sh2);
   const double hN = sh2[0], inv = 1.0 / (sh2[0] - sh2[1]);
   double tr = 0.0, fro = 0.0;
-  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < (size_t)ld * ld; e += (size_t)nb * blockDim.x) {
-    const int r = (int)(e / ld), c = (int)(e % ld);
+  // 32-bit element index (blocks are far below 2^32 elements): cheap division into row, column
+  const unsigned n2 = (unsigned)ld * (unsigned)ld;
+  for (unsigned e = blockIdx.x * blockDim.x + threadIdx.x; e < n2; e += (unsigned)nb * blockDim.x) {
+    const int r = (int)(e / (unsigned)ld), c = (int)(e % (unsigned)ld);
     float v = 0.0f;
     if (r < n && c < n) v = (float)(((r == c ? hN : 0.0) - Fm[(size_t)r * n + c]) * inv);
     Xm[e] = v;
@@ -155,9 +157,9 @@ __global__ void k_pad_fill(pyseqm_layout_t L, const double* __restrict__ F, cons
   gersh_bounds(red, k, nb, sh2);
   const double top = sh2[0], dE = sh2[0] - sh2[1];
   double* O = out + (size_t)k * nmax * nmax;
-  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < (size_t)nmax * nmax;
-       e += (size_t)nb * blockDim.x) {
-    const int r = (int)(e / nmax), c = (int)(e % nmax);
+  const unsigned n2 = (unsigned)nmax * (unsigned)nmax;
+  for (unsigned e = blockIdx.x * blockDim.x + threadIdx.x; e < n2; e += (unsigned)nb * blockDim.x) {
+    const int r = (int)(e / (unsigned)nmax), c = (int)(e % (unsigned)nmax);
     double v = 0.0;
     if (r < n && c < n)
       v = Fm[(size_t)r * n + c];
@@ -641,10 +643,10 @@ __global__ void k_unpad(const int* __restrict__ idx, const int* __restrict__ nor
                         double* __restrict__ P) {
   const int m = idx[blockIdx.y];
   const int n = norb[m], ld = xld[m];
-  const size_t n2 = (size_t)n * n;
-  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n2; e += (size_t)gridDim.x * blockDim.x) {
-    const size_t r = e / n, c = e % n;
-    P[poff[m] + e] = Y[xoff[m] + r * ld + c];
+  const unsigned n2 = (unsigned)n * (unsigned)n;
+  for (unsigned e = blockIdx.x * blockDim.x + threadIdx.x; e < n2; e += gridDim.x * blockDim.x) {
+    const unsigned r = e / (unsigned)n, c = e % (unsigned)n;
+    P[poff[m] + e] = Y[xoff[m] + (size_t)r * ld + c];
   }
 }
 

@@ -104,3 +104,39 @@ def test_gemm_bf16x3_matches_fp64():
         ref = Fm @ Pm
         err = (X[o:o + n * n].view(n, n) - ref).abs().max().item()
         assert err < 1e-5 * ref.abs().max().item(), (m, err)
+
+
+def test_diis_residual_is_commutator():
+    """pyseqm_diis_residual: C = F P - P F (= X - X^T with X = F P) and max|C| per molecule, FP64 path, on
+    a ragged batch whose block sizes are not multiples of the 32 x 32 transpose tiles."""
+    from pyseqm_amd import _native as nat
+    from pyseqm_amd.engine import SCFEngine
+    sp, c = W.alkanes(nmol=24, seed=5, nmin=2, nmax=30)
+    eng = SCFEngine(sp, c, "AM1", "cuda")
+    b = eng.b
+    g = torch.Generator(device="cuda").manual_seed(7)
+    F = torch.zeros(b.total, dtype=torch.float64, device="cuda")
+    P = torch.zeros_like(F)
+    for m in range(b.nmol):
+        n, o = int(b.norb[m]), int(b.mol_mat_off[m])
+        a = torch.randn(n, n, generator=g, device="cuda", dtype=torch.float64)
+        q = torch.randn(n, n, generator=g, device="cuda", dtype=torch.float64)
+        F[o:o + n * n] = (a + a.T).reshape(-1)
+        P[o:o + n * n] = (q + q.T).reshape(-1)
+    idx = torch.arange(b.nmol, dtype=torch.int32, device="cuda")
+    X = torch.zeros_like(F)
+    C = torch.zeros_like(F)
+    err = torch.zeros(b.nmol, dtype=torch.float64, device="cuda")
+    nb = nat.red_blocks(int(b.nmax))
+    nat.call("pyseqm_diis_residual", b.L, nat.ptr(idx), b.nmol, int(b.nmax), nat.ptr(F), nat.ptr(P), nat.ptr(X),
+             nat.ptr(C), nat.ptr(err), nat.ptr(eng.red), nb, 0, eng.stream)
+    torch.cuda.synchronize()
+    for m in range(b.nmol):
+        n, o = int(b.norb[m]), int(b.mol_mat_off[m])
+        Fm = F[o:o + n * n].view(n, n)
+        Pm = P[o:o + n * n].view(n, n)
+        ref = Fm @ Pm - Pm @ Fm
+        got = C[o:o + n * n].view(n, n)
+        assert (got - ref).abs().max().item() < 1e-11 * max(1.0, ref.abs().max().item()), m
+        assert torch.equal(got, -got.T), m
+        assert abs(err[m].item() - got.abs().max().item()) == 0.0, m
