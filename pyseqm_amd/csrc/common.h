@@ -90,6 +90,52 @@ __device__ inline void write_partials(double* part, int tile, int lane, const do
   }
 }
 
+// Compact partial layout of the Fock and core-Hamiltonian reductions: a tile keeps 8 row slots
+// of NR values, then 8 column slots of NC values, with (NR, NC) = (10, 10) for heavy-heavy,
+// (10, 1) for heavy-H and (1, 1) for H-H tiles (an H atom carries one value); the tiles of a kind
+// are back to back, kinds in the order XX, XH, HH.  (The force reductions keep 10-wide slots.)
+__device__ inline size_t cpart_tile(const pyseqm_layout_t& L, int tile, int& nr, int& nc) {
+  const int t0 = L.ntile[0], t1 = L.ntile[1];
+  if (tile < t0) {
+    nr = 10;
+    nc = 10;
+    return (size_t)tile * 160;
+  }
+  if (tile < t0 + t1) {
+    nr = 10;
+    nc = 1;
+    return (size_t)t0 * 160 + (size_t)(tile - t0) * 88;
+  }
+  nr = 1;
+  nc = 1;
+  return (size_t)t0 * 160 + (size_t)t1 * 88 + (size_t)(tile - t0 - t1) * 16;
+}
+__device__ inline size_t cpart_slot(const pyseqm_layout_t& L, int slot) {
+  int nr, nc;
+  const size_t b = cpart_tile(L, slot >> 4, nr, nc);
+  const int s = slot & 15;
+  return s < 8 ? b + (size_t)s * nr : b + (size_t)8 * nr + (size_t)(s - 8) * nc;
+}
+template <int NR, int NC>
+__device__ inline void write_partials_c(double* part, const pyseqm_layout_t& L, int tile, int lane,
+                                        const double (&rowv)[NR], const double (&colv)[NC]) {
+  const int r = lane >> 3, c = lane & 7;
+  int nr, nc;
+  const size_t b = cpart_tile(L, tile, nr, nc);  // (nr, nc) == (NR, NC) for the tile's kind
+  double* prow = part + b + (size_t)r * NR;
+  double* pcol = part + b + (size_t)8 * NR + (size_t)c * NC;
+#pragma unroll
+  for (int k = 0; k < NR; ++k) {
+    const double rs = row_sum(rowv[k]);
+    if (c == 0) prow[k] = rs;
+  }
+#pragma unroll
+  for (int k = 0; k < NC; ++k) {
+    const double cs = col_sum(colv[k]);
+    if (r == 0) pcol[k] = cs;
+  }
+}
+
 // Sum over the G lanes of an aligned lane group (G = 16 or 64), every lane gets the total.
 template <int G>
 __device__ inline double group_sum(double v) {
@@ -108,6 +154,21 @@ __device__ inline void gather_slots(const double* part, const int* csr_idx, int 
   for (int k = 0; k < N; ++k) acc[k] = 0.0;
   for (int e = e0 + gl; e < e1; e += G) {
     const double* src = part + (size_t)csr_idx[e] * kPartStride;
+#pragma unroll
+    for (int k = 0; k < N; ++k) acc[k] += src[k];
+  }
+#pragma unroll
+  for (int k = 0; k < N; ++k) acc[k] = group_sum<G>(acc[k]);
+}
+
+// gather_slots over the compact partial layout (cpart_slot addressing).
+template <int G, int N>
+__device__ inline void gather_slots_c(const double* part, const pyseqm_layout_t& L, int e0, int e1, int gl,
+                                      double (&acc)[N]) {
+#pragma unroll
+  for (int k = 0; k < N; ++k) acc[k] = 0.0;
+  for (int e = e0 + gl; e < e1; e += G) {
+    const double* src = part + cpart_slot(L, L.csr_idx[e]);
 #pragma unroll
     for (int k = 0; k < N; ++k) acc[k] += src[k];
   }

@@ -138,7 +138,7 @@ __global__ __launch_bounds__(64) void k_fock_tiles(pyseqm_layout_t L, int tile_b
       e = pij * (h + f);
     }
   }
-  write_partials(part, tile, lane, rowv, colv);
+  write_partials_c(part, L, tile, lane, rowv, colv);
   const double es = wave_sum(e);
   if (lane == 0) etile[tile] = es;
 }
@@ -256,7 +256,7 @@ __global__ __launch_bounds__(128) void k_fock_xx2(pyseqm_layout_t L, const doubl
         e += pab[a][b] * (h + f);
       }
   }
-  write_partials(part, tile, lane, row10, colv);
+  write_partials_c(part, L, tile, lane, row10, colv);
   const double es = wave_sum(e);
   if (lane == 0) etile[tile] = es;
 }
@@ -284,7 +284,7 @@ __global__ __launch_bounds__(256) void k_fock_diag(pyseqm_layout_t L, const doub
   const int e0 = L.csr_off[a], e1 = L.csr_off[a + 1];
   if (L.atom_Z[a] > 1) {
     double acc[10], pd[10], hd[10];
-    gather_slots<G>(part, L.csr_idx, e0, e1, lane, acc);
+    gather_slots_c<G>(part, L, e0, e1, lane, acc);
 #pragma unroll
     for (int k = 0; k < 10; ++k) {
       pd[k] = Pm[(size_t)(ao + kTriI[k]) * n + ao + kTriJ[k]];
@@ -321,7 +321,7 @@ __global__ __launch_bounds__(256) void k_fock_diag(pyseqm_layout_t L, const doub
     }
   } else {
     double acc[1];
-    gather_slots<G>(part, L.csr_idx, e0, e1, lane, acc);
+    gather_slots_c<G>(part, L, e0, e1, lane, acc);
     if (lane == 0) {
       const double pd = Pm[(size_t)ao * n + ao], hd = Hm[(size_t)ao * n + ao];
       const double f = acc[0] + hd + 0.5 * pd * p[PAR_GSS];

@@ -586,7 +586,7 @@ __global__ __launch_bounds__(64) void k_pair_tiles(pyseqm_layout_t L, int method
     en = core_repulsion(method, (int)pi[PAR_Z], (int)pj[PAR_Z], g.r, gam, pi, pj);
     if (enuc_pair) enuc_pair[pair_ref[(size_t)tile * 64 + lane]] = en;
   }
-  write_partials(part, tile, lane, rowv, colv);
+  write_partials_c(part, L, tile, lane, rowv, colv);
   const double es = wave_sum(en);
   if (lane == 0) enuc_tile[tile] = es;
 }
@@ -761,7 +761,7 @@ __global__ __launch_bounds__(256) void k_hcore_diag(pyseqm_layout_t L, const dou
   const int e0 = L.csr_off[a], e1 = L.csr_off[a + 1];
   if (L.atom_Z[a] > 1) {
     double acc[10];
-    gather_slots<G>(part, L.csr_idx, e0, e1, lane, acc);
+    gather_slots_c<G>(part, L, e0, e1, lane, acc);
     acc[0] += p[PAR_USS];
     acc[2] += p[PAR_UPP];
     acc[5] += p[PAR_UPP];
@@ -776,7 +776,7 @@ __global__ __launch_bounds__(256) void k_hcore_diag(pyseqm_layout_t L, const dou
     }
   } else {
     double acc[1];
-    gather_slots<G>(part, L.csr_idx, e0, e1, lane, acc);
+    gather_slots_c<G>(part, L, e0, e1, lane, acc);
     if (lane == 0) Hm[(size_t)ao * norb + ao] = acc[0] + p[PAR_USS];
   }
 }
