@@ -90,6 +90,15 @@ __device__ inline void write_partials(double* part, int tile, int lane, const do
   }
 }
 
+// Blocks are dealt to the 8 XCDs round-robin (block b runs on XCD b % 8).  Give each XCD a
+// contiguous run of the work list instead, so that neighbouring tiles (which share row segments
+// of P, H and F and the atoms' diagonal blocks) share that XCD's L2.  A bijection of [0, nb).
+__device__ inline int xcd_run(int b, int nb) {
+  const int per = nb / 8, rem = nb % 8;
+  const int x = b % 8, i = b / 8;
+  return x * per + min(x, rem) + i;
+}
+
 // Compact partial layout of the Fock and core-Hamiltonian reductions: a tile keeps 8 row slots
 // of NR values, then 8 column slots of NC values, with (NR, NC) = (10, 10) for heavy-heavy,
 // (10, 1) for heavy-H and (1, 1) for H-H tiles (an H atom carries one value); the tiles of a kind

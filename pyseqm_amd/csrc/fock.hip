@@ -29,7 +29,7 @@ __global__ __launch_bounds__(64) void k_fock_tiles(pyseqm_layout_t L, int tile_b
                                                     double* __restrict__ F, double* __restrict__ part,
                                                     double* __restrict__ etile,
                                                     const unsigned char* __restrict__ active) {
-  const int tl = blockIdx.x;
+  const int tl = xcd_run(blockIdx.x, gridDim.x);
   const int tile = tile_base + tl;
   const int lane = threadIdx.x;
   const Tile t = load_tile(L.tiles, tile);
@@ -184,7 +184,7 @@ __global__ __launch_bounds__(128) void k_fock_xx2(pyseqm_layout_t L, const doubl
                                                   double* __restrict__ etile,
                                                   const unsigned char* __restrict__ active) {
   __shared__ double xch[31][64];
-  const int tile = blockIdx.x;
+  const int tile = xcd_run(blockIdx.x, gridDim.x);
   const int q = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
   const Tile t = load_tile(L.tiles, tile);

@@ -524,8 +524,7 @@ __global__ __launch_bounds__(64) void k_pair_tiles(pyseqm_layout_t L, int method
                                                     double* __restrict__ enuc_tile,
                                                     double* __restrict__ enuc_pair,
                                                     const int* __restrict__ pair_ref) {
-  const int tl = blockIdx.x;
-  const int tile = tile_base + tl;
+  const int tile = tile_base + xcd_run(blockIdx.x, gridDim.x);
   const int lane = threadIdx.x;
   const Tile t = load_tile(L.tiles, tile);
   int i, j;
@@ -689,7 +688,7 @@ __global__ __launch_bounds__(64) void k_pair_forces(pyseqm_layout_t L, int metho
                                                      const double* __restrict__ par, const double* __restrict__ P,
                                                      const double* __restrict__ Dm, double h,
                                                      double* __restrict__ part) {
-  const int tile = tile_base + blockIdx.x;
+  const int tile = tile_base + xcd_run(blockIdx.x, gridDim.x);
   const int lane = threadIdx.x;
   const Tile t = load_tile(L.tiles, tile);
   int i, j;
