@@ -205,13 +205,13 @@ int pyseqm_sp2_iterate(const pyseqm_layout_t* L, const int* idx, int nact, int n
 int pyseqm_sp2_iterate_many(const pyseqm_layout_t* L, const int* idx, int nact, int nmax, const long long* xoff,
                             const int* xld, float* Xa, float* Xb, double* state, double* part, int part_stride,
                             double eps, int max_iter, int* ndone, int it0, int count, void* stream);
-/* Final X (buffer chosen by each molecule's iteration parity) -> FP64 padded copy. */
-int pyseqm_sp2_x_to_f64(const pyseqm_layout_t* L, const int* idx, int nact, int nmax, const long long* xoff,
-                        const int* xld, const double* state, const float* Xa, const float* Xb, double* Y,
-                        void* stream);
-/* FP64 padded block -> packed density. */
-int pyseqm_sp2_unpad(const pyseqm_layout_t* L, const int* idx, int nact, int nmax, const long long* xoff,
-                     const int* xld, const double* Y, double* P, void* stream);
+/* FP64 McWeeny purification of the final X (SP2.py:9-85 followed by the FP64 correction this
+ * path adds): Y2 = X X, then P = 2 (3 Y2 - 2 Y2 X) written to the packed density P (offsets
+ * mol_mat_off).  X is each molecule's FP32 block in Xa or Xb by the parity of its iteration count,
+ * widened exactly to FP64 as it is loaded; Y2 is an FP64 scratch with the X layout. */
+int pyseqm_sp2_mcweeny(const pyseqm_layout_t* L, const int* idx, int nact, int nmax, const long long* xoff,
+                       const int* xld, const double* state, const float* Xa, const float* Xb, double* Y2, double* P,
+                       void* stream);
 
 /* ------------------------------------------------------------------ XL-BOMD / MD
  * Density propagation of XL_BOMD._propagate_P (seqm/MolecularDynamics.py:1418-1427) plus the

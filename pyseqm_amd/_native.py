@@ -59,9 +59,8 @@ _SIGS = {
                            c_int, c_double, c_int, c_void_p, c_void_p],
     "pyseqm_dgemm_sym": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
                          c_void_p],
-    "pyseqm_sp2_x_to_f64": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
-                            c_void_p],
-    "pyseqm_sp2_unpad": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
+    "pyseqm_sp2_mcweeny": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                           c_void_p, c_void_p],
     "pyseqm_pair_forces_xl": [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_void_p, c_void_p,
                               c_void_p],
     "pyseqm_xl_energy": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p],

@@ -15,6 +15,8 @@
 //     idempotency: P = 2 (3 X^2 - 2 X^3).
 #include "common.h"
 
+#include <type_traits>
+
 using namespace pyseqm;
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -524,11 +526,20 @@ __global__ __launch_bounds__(64) void k_sp2_decide(const int* __restrict__ idx, 
 // through registers (prefetched during the MFMAs of the previous slab) and LDS.
 // mode bit 0: McWeeny epilogue C <- 2 (3 A - 2 A B) (A = X^2, B = X); bit 1: the product is known
 // to be symmetric (X^2, X^2 X): only tiles ti <= tj are computed and mirrored.
-template <int DT>
+// AX / BX: the A / B operand is the final FP32 SP2 matrix X of each molecule (buffer Xa or Xb by
+// the parity of its iteration count), widened to FP64 as it is loaded (exact); CPACK: C is written
+// to the packed layout (offset coff[m], leading dimension n) instead of the operands' layout.
+template <typename T>
+__device__ inline double ld64(const T* p) {
+  return (double)*p;
+}
+template <int DT, bool AX, bool BX, bool CPACK>
 __global__ __launch_bounds__(256, 2) void k_dgemm_sym(const int* __restrict__ idx, const int* __restrict__ norb,
                                                    const long long* __restrict__ off, const int* __restrict__ lds,
                                                    const double* __restrict__ A, const double* __restrict__ B,
-                                                   double* __restrict__ C, int mode, int ntile_side) {
+                                                   double* __restrict__ C, int mode, int ntile_side,
+                                                   const float* __restrict__ Xa, const float* __restrict__ Xb,
+                                                   const double* __restrict__ st, const long long* __restrict__ coff) {
   constexpr int WT = DT / 2;    // per-wave tile edge
   constexpr int NB = WT / 16;   // 16x16 MFMA blocks per wave side
   constexpr int NQ = DT * 16 / 256;
@@ -546,9 +557,16 @@ __global__ __launch_bounds__(256, 2) void k_dgemm_sym(const int* __restrict__ id
   if (ti * DT >= n || tj * DT >= n) return;
   const int ld = lds ? lds[m] : n;
   const size_t o = off[m];
-  const double* Am = A + o;
-  const double* Bm = B + o;
-  double* Cm = C + o;
+  const float* Xm = nullptr;
+  if (AX || BX) Xm = ((((long long)st[(size_t)m * S_NST + S_IT]) & 1) ? Xb : Xa) + o;
+  using TA = typename std::conditional<AX, float, double>::type;
+  using TB = typename std::conditional<BX, float, double>::type;
+  const TA* Am;
+  const TB* Bm;
+  if constexpr (AX) Am = Xm; else Am = A + o;
+  if constexpr (BX) Bm = Xm; else Bm = B + o;
+  double* Cm = CPACK ? C + coff[m] : C + o;
+  const int ldc = CPACK ? n : ld;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = wave >> 1, wc = wave & 1;
   const int i0 = ti * DT, j0 = tj * DT;
@@ -564,8 +582,8 @@ __global__ __launch_bounds__(256, 2) void k_dgemm_sym(const int* __restrict__ id
       const int e = tid + 256 * q;
       const int row = e >> 4, gk = k0 + (e & 15);
       const int kc = min(gk, n - 1);
-      const double va = Am[(size_t)min(i0 + row, n - 1) * ld + kc];
-      const double vb = Bm[(size_t)min(j0 + row, n - 1) * ld + kc];
+      const double va = ld64(Am + (size_t)min(i0 + row, n - 1) * ld + kc);
+      const double vb = ld64(Bm + (size_t)min(j0 + row, n - 1) * ld + kc);
       ra[q] = (i0 + row < n && gk < n) ? va : 0.0;
       rb[q] = (j0 + row < n && gk < n) ? vb : 0.0;
     }
@@ -607,7 +625,7 @@ __global__ __launch_bounds__(256, 2) void k_dgemm_sym(const int* __restrict__ id
         for (int v = 0; v < 4; ++v) {
           const int r = min(i0 + wr * WT + a * 16 + 4 * v + (lane >> 4), n - 1);
           const int c = min(j0 + wc * WT + b * 16 + (lane & 15), n - 1);
-          av[v] = Am[(size_t)r * ld + c];
+          av[v] = ld64(Am + (size_t)r * ld + c);
         }
       }
 #pragma unroll
@@ -617,37 +635,11 @@ __global__ __launch_bounds__(256, 2) void k_dgemm_sym(const int* __restrict__ id
         if (r < n && c < n) {
           const double x = acc[a][b][v];
           const double y = (mode & 1) ? 2.0 * (3.0 * av[v] - 2.0 * x) : x;
-          Cm[(size_t)r * ld + c] = y;
-          if (symout && ti != tj) Cm[(size_t)c * ld + r] = y;
+          Cm[(size_t)r * ldc + c] = y;
+          if (symout && ti != tj) Cm[(size_t)c * ldc + r] = y;
         }
       }
     }
-}
-
-// final FP32 X of each molecule (in buffer Xa or Xb by the parity of its iteration count) ->
-// FP64 padded copy
-__global__ void k_x_to_f64(const int* __restrict__ idx, const long long* __restrict__ xoff,
-                           const int* __restrict__ xld, const double* __restrict__ st, const float* __restrict__ Xa,
-                           const float* __restrict__ Xb, double* __restrict__ Y) {
-  const int m = idx[blockIdx.y];
-  const size_t n2 = (size_t)xld[m] * xld[m];
-  const size_t o = xoff[m];
-  const float* X = (((long long)st[(size_t)m * S_NST + S_IT]) & 1) ? Xb : Xa;
-  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n2; e += (size_t)gridDim.x * blockDim.x)
-    Y[o + e] = X[o + e];
-}
-
-// padded (ld) FP64 block -> packed (n) FP64 block
-__global__ void k_unpad(const int* __restrict__ idx, const int* __restrict__ norb, const long long* __restrict__ xoff,
-                        const int* __restrict__ xld, const long long* __restrict__ poff, const double* __restrict__ Y,
-                        double* __restrict__ P) {
-  const int m = idx[blockIdx.y];
-  const int n = norb[m], ld = xld[m];
-  const unsigned n2 = (unsigned)n * (unsigned)n;
-  for (unsigned e = blockIdx.x * blockDim.x + threadIdx.x; e < n2; e += gridDim.x * blockDim.x) {
-    const unsigned r = e / (unsigned)n, c = e % (unsigned)n;
-    P[poff[m] + e] = Y[xoff[m] + (size_t)r * ld + c];
-  }
 }
 
 inline int st_ok() { return hipGetLastError() == hipSuccess ? PYSEQM_OK : PYSEQM_ELAUNCH; }
@@ -740,32 +732,40 @@ extern "C" int pyseqm_dgemm_sym(const pyseqm_layout_t* L, const int* idx, int na
   const bool symout = (mode & 2) != 0;
   if (nmax <= 256) {
     const int nt = (nmax + 63) / 64;
-    k_dgemm_sym<64><<<dim3(symout ? nt * (nt + 1) / 2 : nt * nt, nact), 256, 0, s>>>(idx, L->mol_norb, off, lds, A,
-                                                                                      B, C, mode, nt);
+    k_dgemm_sym<64, false, false, false><<<dim3(symout ? nt * (nt + 1) / 2 : nt * nt, nact), 256, 0, s>>>(
+        idx, L->mol_norb, off, lds, A, B, C, mode, nt, nullptr, nullptr, nullptr, nullptr);
   } else {
     const int nt = (nmax + 127) / 128;
-    k_dgemm_sym<128><<<dim3(symout ? nt * (nt + 1) / 2 : nt * nt, nact), 256, 0, s>>>(idx, L->mol_norb, off, lds,
-                                                                                       A, B, C, mode, nt);
+    k_dgemm_sym<128, false, false, false><<<dim3(symout ? nt * (nt + 1) / 2 : nt * nt, nact), 256, 0, s>>>(
+        idx, L->mol_norb, off, lds, A, B, C, mode, nt, nullptr, nullptr, nullptr, nullptr);
   }
   return st_ok();
 }
 
-extern "C" int pyseqm_sp2_x_to_f64(const pyseqm_layout_t* L, const int* idx, int nact, int nmax,
-                                   const long long* xoff, const int* xld, const double* state, const float* Xa,
-                                   const float* Xb, double* Y, void* stream) {
-  if (!L || nact < 0) return PYSEQM_EINVAL;
+// McWeeny purification of the final SP2 matrix, FP64: Y2 = X X, then P = 2 (3 Y2 - 2 Y2 X) into the
+// packed layout.  X (FP32, buffer by iteration parity) is widened as it is loaded.
+template <int DT>
+static void launch_mcweeny(const pyseqm_layout_t* L, const int* idx, int nact, int nmax, const long long* xoff,
+                           const int* xld, const double* state, const float* Xa, const float* Xb, double* Y2,
+                           double* P, hipStream_t s) {
+  const int nt = (nmax + DT - 1) / DT;
+  const dim3 grid(nt * (nt + 1) / 2, nact);
+  k_dgemm_sym<DT, true, true, false><<<grid, 256, 0, s>>>(idx, L->mol_norb, xoff, xld, nullptr, nullptr, Y2, 2, nt,
+                                                          Xa, Xb, state, nullptr);
+  k_dgemm_sym<DT, false, true, true><<<grid, 256, 0, s>>>(idx, L->mol_norb, xoff, xld, Y2, nullptr, P, 3, nt, Xa,
+                                                          Xb, state, L->mol_mat_off);
+}
+
+extern "C" int pyseqm_sp2_mcweeny(const pyseqm_layout_t* L, const int* idx, int nact, int nmax,
+                                  const long long* xoff, const int* xld, const double* state, const float* Xa,
+                                  const float* Xb, double* Y2, double* P, void* stream) {
+  if (!L || nact < 0 || nmax <= 0) return PYSEQM_EINVAL;
   if (nact == 0) return PYSEQM_OK;
-  const int ld = (nmax + 31) / 32 * 32;
-  k_x_to_f64<<<dim3(eblocks((long long)ld * ld), nact), 256, 0, (hipStream_t)stream>>>(idx, xoff, xld, state, Xa,
-                                                                                       Xb, Y);
+  hipStream_t s = (hipStream_t)stream;
+  if (nmax <= 256)
+    launch_mcweeny<64>(L, idx, nact, nmax, xoff, xld, state, Xa, Xb, Y2, P, s);
+  else
+    launch_mcweeny<128>(L, idx, nact, nmax, xoff, xld, state, Xa, Xb, Y2, P, s);
   return st_ok();
 }
 
-extern "C" int pyseqm_sp2_unpad(const pyseqm_layout_t* L, const int* idx, int nact, int nmax,
-                                const long long* xoff, const int* xld, const double* Y, double* P, void* stream) {
-  if (!L || nact < 0) return PYSEQM_EINVAL;
-  if (nact == 0) return PYSEQM_OK;
-  k_unpad<<<dim3(eblocks((long long)nmax * nmax), nact), 256, 0, (hipStream_t)stream>>>(
-      idx, L->mol_norb, xoff, xld, L->mol_mat_off, Y, P);
-  return st_ok();
-}

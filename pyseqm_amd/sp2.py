@@ -28,9 +28,7 @@ class SP2Workspace:
         self.xld = torch.as_tensor(ld.astype(np.int32), device=dev)
         self.Xa = torch.zeros(max(self.xtotal, 1), dtype=torch.float32, device=dev)
         self.Xb = torch.zeros_like(self.Xa)
-        self.Y = torch.zeros(max(self.xtotal, 1), dtype=torch.float64, device=dev)
-        self.Y2 = torch.zeros_like(self.Y)
-        self.Y3 = torch.zeros_like(self.Y)
+        self.Y2 = torch.zeros(max(self.xtotal, 1), dtype=torch.float64, device=dev)  # McWeeny X^2
         self.state = torch.zeros(b.nmol, NSTATE, dtype=torch.float64, device=dev)
         # per-tile (trace, Frobenius) partials; sized for the 64 x 64 tiles (the 128 x 128 tiles
         # used for large blocks need fewer)
@@ -95,14 +93,8 @@ def density_sp2(eng, F, idx, nact, Pout, eps=SP2_EPS, max_iter=SP2_MAX_ITER, fix
         ev1.record()
         ws.timing.append((ev0, ev1, idx.clone(), nact))
     # FP64 McWeeny correction: P = 2 (3 X^2 - 2 X^3)
-    nat.call("pyseqm_sp2_x_to_f64", L, nat.ptr(idx), nact, nmax, nat.ptr(ws.xoff), nat.ptr(ws.xld),
-             nat.ptr(ws.state), nat.ptr(ws.Xa), nat.ptr(ws.Xb), nat.ptr(ws.Y), s)
-    nat.call("pyseqm_dgemm_sym", L, nat.ptr(idx), nact, nmax, nat.ptr(ws.xoff), nat.ptr(ws.xld), nat.ptr(ws.Y),
-             nat.ptr(ws.Y), nat.ptr(ws.Y2), 2, s)
-    nat.call("pyseqm_dgemm_sym", L, nat.ptr(idx), nact, nmax, nat.ptr(ws.xoff), nat.ptr(ws.xld), nat.ptr(ws.Y2),
-             nat.ptr(ws.Y), nat.ptr(ws.Y3), 3, s)
-    nat.call("pyseqm_sp2_unpad", L, nat.ptr(idx), nact, nmax, nat.ptr(ws.xoff), nat.ptr(ws.xld), nat.ptr(ws.Y3),
-             nat.ptr(Pout), s)
+    nat.call("pyseqm_sp2_mcweeny", L, nat.ptr(idx), nact, nmax, nat.ptr(ws.xoff), nat.ptr(ws.xld), nat.ptr(ws.state),
+             nat.ptr(ws.Xa), nat.ptr(ws.Xb), nat.ptr(ws.Y2), nat.ptr(Pout), s)
     ws.last_iters = ws.state[:, 7]
     if ws.timing is not None:
         ws.timing[-1] = ws.timing[-1] + (ws.state[:, 7].clone(),)
