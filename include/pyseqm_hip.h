@@ -139,6 +139,8 @@ int pyseqm_density_from_evecs(const pyseqm_layout_t* L, const double* V, int nma
 /* Density mixing: Pold = P; P = alpha P + (1 - alpha) Pnew (scf_loop.py:266-267, 1052-1056). */
 int pyseqm_mix(const pyseqm_layout_t* L, const int* idx, int nact, int nmax, double alpha, double* P,
                const double* Pnew, double* Pold, void* stream);
+/* dst = src on the upper triangle (c >= r) of the active blocks: the DIIS Fock ring keeps only
+ * the upper triangles current (the matrices are symmetric; pyseqm_diis_combine reads only them). */
 int pyseqm_copy_active(const pyseqm_layout_t* L, const int* idx, int nact, int nmax, const double* src,
                        double* dst, void* stream);
 
@@ -180,6 +182,8 @@ int pyseqm_diis_dots(const pyseqm_layout_t* L, const int* idx, int nact, const d
  * (stride nfock) and reset[m] = 1 when max|lambda| / min|lambda| > 1e7. */
 int pyseqm_diis_solve(const pyseqm_layout_t* L, const int* idx, int nact, int nfock, int count, int slot,
                       const double* Bmat, double* coef, unsigned char* reset, void* stream);
+/* F = sum_j coef[j] ring[j] (scf_loop.py:1033-1035), full symmetric F formed from the ring
+ * entries' upper triangles. */
 int pyseqm_diis_combine(const pyseqm_layout_t* L, const int* idx, int nact, int nmax, const double* ring,
                         long long ring_stride, int count, const double* coef, int cstride, double* F,
                         void* stream);

@@ -260,25 +260,32 @@ __global__ __launch_bounds__(64) void k_antisym_final(const int* __restrict__ id
   if (threadIdx.x == 0) diis_err[idx[k]] = mx;
 }
 
-// B[m][slot][j] = <C_slot, C_j> over the packed upper triangle = 0.5 * full Frobenius product
-// (C is antisymmetric), j = 0..count-1 (scf_loop.py:1003-1013).  Each block reads its chunk of
-// C_slot once and forms all `count` products; per-block partials, then a fixed-order final sum.
-__global__ void k_diis_dots(pyseqm_layout_t L, const int* __restrict__ idx, int nb, const double* __restrict__ ring,
-                            size_t ring_stride, int slot, int count, double* __restrict__ red) {
+// B[m][slot][j] = <C_slot, C_j> over the strict upper triangle (= 0.5 * full Frobenius product: C is
+// antisymmetric with a zero diagonal), j = 0..count-1 (scf_loop.py:1003-1013).  One wave per row r
+// (lanes over the contiguous segment c > r of the row), so only half of each ring entry is read;
+// each wave forms all `count` products from one read of C_slot; per-block partials, then a
+// fixed-order final sum.
+__global__ __launch_bounds__(kEl) void k_diis_dots(pyseqm_layout_t L, const int* __restrict__ idx, int nb,
+                                                   const double* __restrict__ ring, size_t ring_stride, int slot,
+                                                   int count, double* __restrict__ red) {
   __shared__ double sh[kEl / 64];
   const int k = blockIdx.y;
   const int m = idx[k];
-  const size_t n2 = (size_t)L.mol_norb[m] * L.mol_norb[m];
+  const int n = L.mol_norb[m];
   const size_t off = L.mol_mat_off[m];
   const double* a = ring + (size_t)slot * ring_stride + off;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   double acc[16];
 #pragma unroll
   for (int j = 0; j < 16; ++j) acc[j] = 0.0;
-  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n2; e += (size_t)nb * blockDim.x) {
-    const double av = a[e];
+  for (int r = blockIdx.x * (kEl / 64) + wave; r < n - 1; r += nb * (kEl / 64)) {
+    const size_t row = (size_t)r * n;
+    for (int c = r + 1 + lane; c < n; c += 64) {
+      const double av = a[row + c];
 #pragma unroll
-    for (int j = 0; j < 16; ++j)
-      if (j < count) acc[j] += av * ring[(size_t)j * ring_stride + off + e];
+      for (int j = 0; j < 16; ++j)
+        if (j < count) acc[j] += av * ring[(size_t)j * ring_stride + off + row + c];
+    }
   }
   for (int j = 0; j < count; ++j) {
     const double r = block_sum(acc[j], sh);
@@ -294,7 +301,7 @@ __global__ __launch_bounds__(64) void k_diis_dots_final(const int* __restrict__ 
   double s = 0.0;
   for (int b = threadIdx.x; b < nb; b += 64) s += red[((size_t)k * nb + b) * 16 + j];
   s = wave_sum(s);
-  if (threadIdx.x == 0) Bmat[((size_t)idx[k] * (nfock + 1) + slot) * (nfock + 1) + j] = 0.5 * s;
+  if (threadIdx.x == 0) Bmat[((size_t)idx[k] * (nfock + 1) + slot) * (nfock + 1) + j] = s;
 }
 
 // DIIS coefficients from the bordered B matrix (scf_loop.py:1015-1031): the lower triangle of
@@ -376,28 +383,65 @@ __global__ __launch_bounds__(64) void k_diis_solve(const int* __restrict__ idx, 
 }
 
 // F = sum_j c[m][j] F_j over the stored Fock matrices (scf_loop.py:1033-1035).
-__global__ void k_diis_combine(pyseqm_layout_t L, const int* __restrict__ idx, const double* __restrict__ ring,
-                               size_t ring_stride, int count, const double* __restrict__ coef, int cstride,
-                               double* __restrict__ F) {
+// F = sum_j coef[j] Fring[j] on the active blocks (scf_loop.py:1033-1035).  The ring entries are
+// symmetric and only their upper triangles are kept current (pyseqm_copy_active copies the upper
+// triangle): each block takes 32 x 32 tile pairs (I <= J), forms the upper tile from coalesced row
+// reads, and writes it and its transpose (through LDS) as rows of F.
+__global__ __launch_bounds__(kEl) void k_diis_combine(pyseqm_layout_t L, const int* __restrict__ idx, int nb,
+                                                      const double* __restrict__ ring, size_t ring_stride, int count,
+                                                      const double* __restrict__ coef, int cstride,
+                                                      double* __restrict__ F) {
+  __shared__ double S[32][33];
   const int m = idx[blockIdx.y];
-  const size_t n2 = (size_t)L.mol_norb[m] * L.mol_norb[m];
+  const int n = L.mol_norb[m];
   const size_t off = L.mol_mat_off[m];
   const double* c = coef + (size_t)m * cstride;
-  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n2; e += (size_t)gridDim.x * blockDim.x) {
-    double s = 0.0;
-    for (int j = 0; j < count; ++j) s += c[j] * ring[(size_t)j * ring_stride + off + e];
-    F[off + e] = s;
+  const int T = (n + 31) / 32;
+  const int ntri = T * (T + 1) / 2;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int t = blockIdx.x; t < ntri; t += nb) {
+    int tj = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+    while ((tj + 1) * (tj + 2) / 2 <= t) ++tj;
+    while (tj * (tj + 1) / 2 > t) --tj;
+    const int ti = t - tj * (tj + 1) / 2;
+    const int i0 = ti * 32, j0 = tj * 32;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int rr = ty + 8 * q;
+      const int r = i0 + rr, cc = j0 + tx;
+      // element (r, cc) of the symmetric matrix, read from the upper triangle
+      const int ur = min(r, cc), uc = max(r, cc);
+      double v = 0.0;
+      if (r < n && cc < n) {
+        const size_t e = off + (size_t)ur * n + uc;
+        for (int j = 0; j < count; ++j) v += c[j] * ring[(size_t)j * ring_stride + e];
+        F[off + (size_t)r * n + cc] = v;
+      }
+      S[rr][tx] = v;
+    }
+    __syncthreads();
+    if (ti != tj) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int rr = ty + 8 * q;
+        if (j0 + rr < n && i0 + tx < n) F[off + (size_t)(j0 + rr) * n + i0 + tx] = S[tx][rr];
+      }
+    }
+    __syncthreads();
   }
 }
 
-// dst = src on the active molecules' blocks.
-__global__ void k_copy_active(pyseqm_layout_t L, const int* __restrict__ idx, const double* __restrict__ src,
-                              double* __restrict__ dst) {
+// dst = src on the upper triangle (c >= r) of the active molecules' blocks: one wave per row.
+__global__ __launch_bounds__(kEl) void k_copy_active(pyseqm_layout_t L, const int* __restrict__ idx, int nb,
+                                                     const double* __restrict__ src, double* __restrict__ dst) {
   const int m = idx[blockIdx.y];
-  const size_t n2 = (size_t)L.mol_norb[m] * L.mol_norb[m];
+  const int n = L.mol_norb[m];
   const size_t off = L.mol_mat_off[m];
-  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n2; e += (size_t)gridDim.x * blockDim.x)
-    dst[off + e] = src[off + e];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int r = blockIdx.x * (kEl / 64) + wave; r < n; r += nb * (kEl / 64)) {
+    const size_t row = off + (size_t)r * n;
+    for (int c = r + lane; c < n; c += 64) dst[row + c] = src[row + c];
+  }
 }
 
 inline int ceil_div(long long a, long long b) { return (int)((a + b - 1) / b); }
@@ -409,6 +453,18 @@ inline int elem_blocks(int nmax) {
 }
 
 inline int status() { return hipGetLastError() == hipSuccess ? PYSEQM_OK : PYSEQM_ELAUNCH; }
+
+// blocks per molecule of the one-wave-per-row kernels (4 rows per block) and of the 32 x 32
+// tile-pair kernels, capped (the kernels loop)
+inline int row_blocks(int nmax) {
+  const int b = ceil_div(nmax, kEl / 64);
+  return b < 1 ? 1 : (b > 1024 ? 1024 : b);
+}
+inline int tile_pair_blocks(int nmax) {
+  const long long t = ceil_div(nmax, 32);
+  const long long b = t * (t + 1) / 2;
+  return (int)(b < 1 ? 1 : (b > 1024 ? 1024 : b));
+}
 
 }  // namespace
 
@@ -433,7 +489,8 @@ extern "C" int pyseqm_copy_active(const pyseqm_layout_t* L, const int* idx, int 
                                   double* dst, void* stream) {
   if (!L || nact < 0) return PYSEQM_EINVAL;
   if (nact == 0) return PYSEQM_OK;
-  k_copy_active<<<dim3(elem_blocks(nmax), nact), kEl, 0, (hipStream_t)stream>>>(*L, idx, src, dst);
+  const int nb = row_blocks(nmax);
+  k_copy_active<<<dim3(nb, nact), kEl, 0, (hipStream_t)stream>>>(*L, idx, nb, src, dst);
   return status();
 }
 
@@ -510,7 +567,8 @@ extern "C" int pyseqm_diis_combine(const pyseqm_layout_t* L, const int* idx, int
                                    void* stream) {
   if (!L || nact < 0 || count <= 0) return PYSEQM_EINVAL;
   if (nact == 0) return PYSEQM_OK;
-  k_diis_combine<<<dim3(elem_blocks(nmax), nact), kEl, 0, (hipStream_t)stream>>>(*L, idx, ring, (size_t)ring_stride,
-                                                                                  count, coef, cstride, F);
+  const int nb = tile_pair_blocks(nmax);
+  k_diis_combine<<<dim3(nb, nact), kEl, 0, (hipStream_t)stream>>>(*L, idx, nb, ring, (size_t)ring_stride, count, coef,
+                                                                   cstride, F);
   return status();
 }
