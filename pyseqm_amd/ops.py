@@ -124,5 +124,83 @@ def core_repulsion_pairs(method: int, idxi: torch.Tensor, idxj: torch.Tensor, ri
                  _p(atom_par), _p(out), _s())
 
 
+@torch.library.custom_op("pyseqm::density_eigh_fp64", mutates_args=("P", "red"))
+def density_eigh_fp64(layout: int, F: torch.Tensor, idx: torch.Tensor, nmax: int, P: torch.Tensor,
+                      red: torch.Tensor) -> tuple[torch.Tensor, torch.Tensor]:
+    """sym_eig_trunc (diag.py:110-241) on the listed blocks: Gershgorin-padded nmax x nmax stack, batched
+    FP64 eigensolve, P = 2 C_occ C_occ^T into the packed density.  Returns the padded eigenvalues
+    (nact, nmax) and eigenvectors (nact, nmax, nmax, columns)."""
+    nact = int(idx.numel())
+    Fp = torch.empty(nact, nmax, nmax, dtype=torch.float64, device=F.device)
+    nat.call("pyseqm_pad_for_eigh", _L(layout), _p(F), _p(idx), nact, nmax, _p(Fp), _p(red), nat.red_blocks(nmax),
+             _s())
+    e, V = torch.linalg.eigh(Fp)
+    V = V.contiguous()
+    nat.call("pyseqm_density_from_evecs", _L(layout), _p(V), nmax, _p(idx), nact, _p(P), _s())
+    return e, V
+
+
+@torch.library.custom_op("pyseqm::density_sp2", mutates_args=("Xa", "Xb", "state", "part", "Y2", "red", "ndone", "P"))
+def density_sp2(layout: int, F: torch.Tensor, idx: torch.Tensor, nmax: int, xoff: torch.Tensor, xld: torch.Tensor,
+                Xa: torch.Tensor, Xb: torch.Tensor, state: torch.Tensor, part: torch.Tensor, Y2: torch.Tensor,
+                red: torch.Tensor, ndone: torch.Tensor, P: torch.Tensor, eps: float, iters: int) -> None:
+    """SP2 (SP2.py:9-85) in mixed precision on the listed blocks, then the FP64 McWeeny step into the packed
+    density P.  `iters` iterations are launched (molecules that stop earlier make the rest no-ops), so the
+    call makes no host synchronisation.  Buffers as pyseqm_amd.sp2.SP2Workspace lays them out: X blocks
+    (FP32, Xa / Xb) at xoff with leading dimension xld, state (nmol, 8), part (nmol, part_stride), Y2 (FP64,
+    X layout), ndone (1 int)."""
+    L = _L(layout)
+    nact = int(idx.numel())
+    nat.call("pyseqm_sp2_init", L, _p(F), _p(idx), nact, _p(xoff), _p(xld), _p(Xa), _p(state), _p(red),
+             nat.red_blocks(nmax), _s())
+    ndone.zero_()
+    nat.call("pyseqm_sp2_iterate_many", L, _p(idx), nact, nmax, _p(xoff), _p(xld), _p(Xa), _p(Xb), _p(state),
+             _p(part), int(part.shape[-1]), float(eps), int(iters), _p(ndone), 0, int(iters), _s())
+    nat.call("pyseqm_sp2_mcweeny", L, _p(idx), nact, nmax, _p(xoff), _p(xld), _p(state), _p(Xa), _p(Xb), _p(Y2),
+             _p(P), _s())
+
+
+@torch.library.custom_op("pyseqm::mix", mutates_args=("P", "Pold"))
+def mix(layout: int, idx: torch.Tensor, nmax: int, alpha: float, P: torch.Tensor, Pnew: torch.Tensor,
+        Pold: torch.Tensor) -> None:
+    """Pold = P; P = alpha P + (1 - alpha) Pnew on the listed blocks (scf_loop.py:266-267, 1052-1056)."""
+    nat.call("pyseqm_mix", _L(layout), _p(idx), int(idx.numel()), nmax, float(alpha), _p(P), _p(Pnew), _p(Pold),
+             _s())
+
+
+@torch.library.custom_op("pyseqm::scf_check", mutates_args=("e_old", "notconv", "metrics", "red"))
+def scf_check(layout: int, idx: torch.Tensor, nmax: int, P: torch.Tensor, Pold: torch.Tensor, e_new: torch.Tensor,
+              e_old: torch.Tensor, diis_err: torch.Tensor, eps: float, diis_tol: float, notconv: torch.Tensor,
+              metrics: torch.Tensor, red: torch.Tensor) -> None:
+    """The 4-way convergence test get_error (scf_loop.py:106-147) per listed molecule."""
+    nat.call("pyseqm_scf_check", _L(layout), _p(idx), int(idx.numel()), _p(P), _p(Pold), _p(e_new), _p(e_old),
+             _p(diis_err), float(eps), float(diis_tol), _p(notconv), _p(metrics), _p(red), nat.red_blocks(nmax), _s())
+
+
+@torch.library.custom_op("pyseqm::diis_step", mutates_args=("Fring", "Cring", "X", "diis_err", "Bmat", "coef",
+                                                             "reset", "F", "red"))
+def diis_step(layout: int, idx: torch.Tensor, nmax: int, F: torch.Tensor, P: torch.Tensor, Fring: torch.Tensor,
+              Cring: torch.Tensor, X: torch.Tensor, diis_err: torch.Tensor, Bmat: torch.Tensor, coef: torch.Tensor,
+              reset: torch.Tensor, red: torch.Tensor, slot: int, count: int, mixed: bool) -> None:
+    """One Pulay/DIIS update (scf_loop.py:990-1035) on the listed blocks: F into ring slot `slot`, the
+    residual FP - PF into the error ring with its max-norm, the new B-matrix row against the `count`
+    stored entries and, when count >= 2, the coefficients (FP64 Jacobi solve with the reference's
+    pseudo-inverse and reset flag) and the extrapolated F.  Fring / Cring are (nfock, total) rings."""
+    L = _L(layout)
+    nact = int(idx.numel())
+    nfock = int(Fring.shape[0])
+    total = int(Fring.shape[1])
+    nb = nat.red_blocks(nmax)
+    nat.call("pyseqm_copy_active", L, _p(idx), nact, nmax, _p(F), _p(Fring[slot]), _s())
+    nat.call("pyseqm_diis_residual", L, _p(idx), nact, nmax, _p(F), _p(P), _p(X), _p(Cring[slot]), _p(diis_err),
+             _p(red), nb, int(bool(mixed)), _s())
+    nat.call("pyseqm_diis_dots", L, _p(idx), nact, _p(Cring), total, slot, count, nfock, _p(Bmat), _p(red), nb, _s())
+    if count >= 2:
+        reset.zero_()
+        nat.call("pyseqm_diis_solve", L, _p(idx), nact, nfock, count, slot, _p(Bmat), _p(coef), _p(reset), _s())
+        nat.call("pyseqm_diis_combine", L, _p(idx), nact, nmax, _p(Fring), total, count, _p(coef), nfock, _p(F), _s())
+
+
 OPS = ("atom_setup", "pair_integrals", "fock_build", "pad_for_eigh", "density_from_evecs", "pair_forces",
-       "pair_forces_xl", "xl_energy", "xl_propagate", "core_repulsion_pairs")
+       "pair_forces_xl", "xl_energy", "xl_propagate", "core_repulsion_pairs", "density_eigh_fp64", "density_sp2",
+       "mix", "scf_check", "diis_step")

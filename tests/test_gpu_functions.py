@@ -15,6 +15,7 @@ from pyseqm_amd.seqm_functions.diag import sym_eig_trunc
 from pyseqm_amd.seqm_functions.energy import elec_energy, pair_nuclear_energy
 from pyseqm_amd.seqm_functions.fock import fock
 from pyseqm_amd.seqm_functions.hcore import hcore
+from pyseqm_amd import workloads as W
 from pyseqm_amd.seqm_functions.pack import pack, unpack
 from pyseqm_amd.seqm_functions.two_elec_two_center_int import two_elec_two_center_int
 from tests.test_gpu_parity import _inputs
@@ -131,3 +132,92 @@ def test_custom_ops_match_engine():
     f2 = torch.zeros(b.natoms, 3, dtype=torch.float64, device=DEV)
     torch.ops.pyseqm.pair_forces(h, nat.METHODS["PM3"], eng.coords, eng.atom_par, r.P, 1e-3, eng.part, f2)
     assert torch.equal(f1, f2)
+
+
+def test_density_ops_match_engine():
+    """torch.ops.pyseqm.density_eigh_fp64 / density_sp2 give the engine's densities bit for bit."""
+    import pyseqm_amd.ops  # noqa: F401
+    from pyseqm_amd import sp2 as S
+    from pyseqm_amd.engine import SCFEngine
+    from pyseqm_amd.ops import layout_handle
+    sp, c = W.water_clusters(20, 3, seed=4)
+    eng = SCFEngine(sp, c, "PM3", DEV)
+    r = eng.scf(eps=1e-6, converger=(2,), density="eigh")
+    b = eng.b
+    F = eng.energies(r.P)["F"]
+    idx = torch.arange(b.nmol, dtype=torch.int32, device=DEV)
+    eng._idx_host = np.arange(b.nmol)
+    nmax = int(b.nmax)
+    h = layout_handle(b)
+    P1 = torch.zeros_like(F)
+    e1, V1, _ = eng.density_eigh(F, idx, b.nmol, P1, want_evecs=True)
+    P2 = torch.zeros_like(F)
+    e2, V2 = torch.ops.pyseqm.density_eigh_fp64(h, F, idx, e1.shape[1], P2, eng.red)
+    assert torch.equal(P1, P2) and torch.equal(e1, e2)
+    P3 = torch.zeros_like(F)
+    S.density_sp2(eng, F, idx, b.nmol, P3, eps=1e-4, fixed_iters=40)
+    ws = eng._sp2ws
+    P4 = torch.zeros_like(F)
+    torch.ops.pyseqm.density_sp2(h, F, idx, nmax, ws.xoff, ws.xld, ws.Xa, ws.Xb, ws.state, ws.part, ws.Y2, eng.red,
+                                 ws.ndone, P4, 1e-4, 40)
+    assert torch.equal(P3, P4)
+
+
+def test_diis_step_op_extrapolates():
+    """torch.ops.pyseqm.diis_step: after three updates F is sum_j c_j F_j with c from the bordered system
+    [B 1; 1 0][c; l] = [0; 1], B_ij = <C_i, C_j> / 2 over the commutators C_j = F_j P_j - P_j F_j (the
+    reference's Pulay extrapolation, scf_loop.py:990-1035), here solved by torch."""
+    import pyseqm_amd.ops  # noqa: F401
+    from pyseqm_amd.engine import SCFEngine
+    from pyseqm_amd.ops import layout_handle
+    sp, c = W.water_clusters(8, 2, seed=6)
+    eng = SCFEngine(sp, c, "PM3", DEV)
+    b = eng.b
+    h = layout_handle(b)
+    g = torch.Generator(device=DEV).manual_seed(11)
+    nf, total = 10, b.total
+    Fring = torch.zeros(nf, total, dtype=torch.float64, device=DEV)
+    Cring = torch.zeros_like(Fring)
+    X = torch.zeros(total, dtype=torch.float64, device=DEV)
+    err = torch.zeros(b.nmol, dtype=torch.float64, device=DEV)
+    # the bordered matrix as the engine keeps it: lower triangle, border -1, corner 0
+    emat0 = torch.tril(torch.eye(nf + 1, dtype=torch.float64, device=DEV) - 1.0)
+    Bmat = emat0.expand(b.nmol, -1, -1).contiguous()
+    coef = torch.zeros(b.nmol, nf, dtype=torch.float64, device=DEV)
+    reset = torch.zeros(b.nmol, dtype=torch.uint8, device=DEV)
+    idx = torch.arange(b.nmol, dtype=torch.int32, device=DEV)
+    nmax = int(b.nmax)
+    Fs, Ps = [], []
+    for j in range(3):
+        F = torch.zeros(total, dtype=torch.float64, device=DEV)
+        P = torch.zeros_like(F)
+        for m in range(b.nmol):
+            n, o = int(b.norb[m]), int(b.mol_mat_off[m])
+            a = torch.randn(n, n, generator=g, device=DEV, dtype=torch.float64)
+            q = torch.randn(n, n, generator=g, device=DEV, dtype=torch.float64)
+            F[o:o + n * n] = (a + a.T).reshape(-1)
+            P[o:o + n * n] = (q + q.T).reshape(-1)
+        Fs.append(F.clone())
+        Ps.append(P)
+        torch.ops.pyseqm.diis_step(h, idx, nmax, F, P, Fring, Cring, X, err, Bmat, coef, reset, eng.red, j, j + 1,
+                                   False)
+    torch.cuda.synchronize()
+    assert not bool(reset.any())
+    for m in range(b.nmol):
+        n, o = int(b.norb[m]), int(b.mol_mat_off[m])
+        Cs = []
+        for j in range(3):
+            Fm = Fs[j][o:o + n * n].view(n, n)
+            Pm = Ps[j][o:o + n * n].view(n, n)
+            Cs.append(Fm @ Pm - Pm @ Fm)
+        B = torch.tensor([[0.5 * (Cs[i] * Cs[k]).sum().item() for k in range(3)] for i in range(3)],
+                         dtype=torch.float64)
+        A = torch.zeros(4, 4, dtype=torch.float64)
+        A[:3, :3] = B / B[2, 2]
+        A[3, :3] = 1.0
+        A[:3, 3] = 1.0
+        rhs = torch.tensor([0.0, 0.0, 0.0, 1.0], dtype=torch.float64)
+        cref = torch.linalg.solve(A, rhs)[:3]
+        Fref = sum(cref[j].item() * Fs[j][o:o + n * n] for j in range(3))
+        got = F[o:o + n * n]
+        assert (got - Fref).abs().max().item() < 1e-8 * Fref.abs().max().item(), m
