@@ -245,6 +245,8 @@ def main():
     ap.add_argument("--fock-reps", type=int, default=20)
     ap.add_argument("--no-compare", action="store_true", help="skip the SP2-mixed run on the same inputs")
     ap.add_argument("--sp2-roofline-config", default="C3", help="workload of the SP2 GEMM roofline ('' = skip)")
+    ap.add_argument("--scale-config", default="C4",
+                    help="single large system for the Fock / SP2 rooflines at scale ('' = skip)")
     ap.add_argument("--serve-window", type=float, default=10.0, help="seconds of Poisson arrivals (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=2000.0)
     ap.add_argument("--serve-rate-light", type=float, default=300.0,
@@ -364,6 +366,28 @@ def main():
         roof["workload"] = f"{args.sp2_roofline_config}: {e3.b.nmol} x {int(e3.b.nmax)} orbitals, SCF {t3 * 1e3:.1f} ms, " \
                            f"{r3.iterations} SCF iterations"
         line["roofline_sp2"] = roof
+    if args.scale_config:
+        # one large system: Fock HBM rate and SP2 MFMA use against peak, SP2 SCF time
+        m4, sp4, xyz4 = make_workload(args.scale_config, args.seed)
+        e4 = SCFEngine(sp4, None, m4, dev)
+        c4 = torch.as_tensor(xyz4, dtype=torch.float64, device=dev)
+
+        def step4():
+            e4.set_coordinates(c4)
+            return e4.scf(eps=args.sp2_eps, converger=conv, density="sp2")
+        step4()
+        torch.cuda.synchronize()
+        t4 = time.perf_counter()
+        r4 = step4()
+        torch.cuda.synchronize()
+        t4 = time.perf_counter() - t4
+        line["roofline_scale"] = {
+            "workload": f"{args.scale_config}: {e4.b.nmol} x {int(e4.b.nmax)} orbitals, {e4.b.natoms} atoms, "
+                        f"mixed SP2 SCF {t4 * 1e3:.1f} ms, {r4.iterations} SCF iterations, converged "
+                        f"{int((~r4.notconverged).sum())}/{e4.b.nmol}",
+            "fock": fock_roofline(e4, r4.P, args.fock_reps),
+            "sp2": sp2_roofline(e4, step4)}
+        del e4
     if args.serve_window > 0:
         sv = serve(args, dev)
         line["serving"] = sv
