@@ -190,22 +190,28 @@ __device__ inline void tri_decode(int t, int& ti, int& tj) {
 // of the molecule-major tile list are given to one XCD (block id mod 8), so the tiles that share a
 // molecule's row panels share that XCD's L2.
 template <int TB>
-__global__ __launch_bounds__(256, TB == 128 ? 2 : 4) void k_sp2_bf16x3(
+__global__ __launch_bounds__(256, TB == 128 ? 3 : 4) void k_sp2_bf16x3(
     const int* __restrict__ idx, int nact, int ntri, const int* __restrict__ norb,
     const long long* __restrict__ xoff, const int* __restrict__ xld, const float* __restrict__ Xin,
     float* __restrict__ Xout, double* __restrict__ st, double* __restrict__ part, int part_stride) {
+  // K slab: 16 at TB = 128 (48 KiB of LDS, three blocks per CU), 32 at TB = 64
+  constexpr int BK = TB == 128 ? 16 : 32;
+  constexpr int LDSK = BK + 8;             // padded LDS row (bf16), 16-byte aligned, conflict-free
   constexpr int NW = TB / 64;              // 32 x 32 blocks per wave side
   constexpr int NQ = TB * (BK / 4) / 256;  // float4 slab loads per thread and operand
-  // double-buffered slabs: [buffer][hi/lo][row][k] (80 KiB at TB = 128, so two blocks share a CU);
-  // after the main loop the same LDS holds the output tile (TB x (TB+1) floats) for the mirrored
-  // write and the per-wave trace / Frobenius partials
+  // double-buffered slabs: [buffer][hi/lo][row][k]; after the main loop the same LDS holds the
+  // output tile for the mirrored write (in NP passes of TB / NP rows when the whole tile does not
+  // fit) and the per-wave trace / Frobenius partials
+  constexpr int SLAB_BYTES = 2 * 2 * 2 * TB * LDSK * 2;
+  constexpr int NP = TB * (TB + 1) * 4 + 80 <= SLAB_BYTES ? 1 : 2;
+  constexpr int PR = TB / NP;              // tile rows per mirror pass
   __shared__ __attribute__((aligned(16))) __bf16 smem[2 * 2 * 2 * TB * LDSK];
   __bf16(*As)[2][TB][LDSK] = reinterpret_cast<__bf16(*)[2][TB][LDSK]>(smem);
   __bf16(*Bs)[2][TB][LDSK] = reinterpret_cast<__bf16(*)[2][TB][LDSK]>(smem + 2 * 2 * TB * LDSK);
   float* T = reinterpret_cast<float*>(smem);
-  double(*red)[4] = reinterpret_cast<double(*)[4]>(T + TB * (TB + 1));
-  static_assert(TB * (TB + 1) * sizeof(float) + 8 * sizeof(double) <= sizeof(__bf16) * 2 * 2 * 2 * TB * LDSK,
-                "epilogue buffers exceed the slab LDS");
+  double(*red)[4] = reinterpret_cast<double(*)[4]>(T + PR * (TB + 1));
+  int* zsh = reinterpret_cast<int*>(T + PR * (TB + 1) + 16);
+  static_assert(PR * (TB + 1) * 4 + 8 * 8 + 4 <= SLAB_BYTES, "epilogue buffers exceed the slab LDS");
   const int total = nact * ntri, per = (total + 7) / 8;
   const int w = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
   if (w >= total) return;
@@ -237,8 +243,8 @@ __global__ __launch_bounds__(256, TB == 128 ? 2 : 4) void k_sp2_bf16x3(
   auto gload = [&](int k0) {
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
-      const int f = tid + 256 * q;  // float4 index in the TB x 8 slab
-      const int row = f >> 3, kq = f & 7;
+      const int f = tid + 256 * q;  // float4 index in the TB x (BK/4) slab
+      const int row = f / (BK / 4), kq = f % (BK / 4);
       ra[q] = make_float4(0.f, 0.f, 0.f, 0.f);
       rb[q] = ra[q];
       if (i0 + row < n) ra[q] = *reinterpret_cast<const float4*>(Xm + (size_t)(i0 + row) * ld + k0 + 4 * kq);
@@ -263,7 +269,7 @@ __global__ __launch_bounds__(256, TB == 128 ? 2 : 4) void k_sp2_bf16x3(
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
       const int f = tid + 256 * q;
-      const int row = f >> 3, kq = f & 7;
+      const int row = f / (BK / 4), kq = f % (BK / 4);
       split_store(ra[q], As[buf][0], As[buf][1], row, 4 * kq);
       split_store(rb[q], Bs[buf][0], Bs[buf][1], row, 4 * kq);
     }
@@ -303,47 +309,61 @@ __global__ __launch_bounds__(256, TB == 128 ? 2 : 4) void k_sp2_bf16x3(
     cur ^= 1;
   }
   // epilogue: Y = X^2 or 2X - X^2, FP64 partial trace / Frobenius norm; the tile is written
-  // directly and kept in LDS for the mirrored (transposed) write
+  // directly and kept in LDS for the mirrored (transposed) write.  Pass h takes the waves whose
+  // rows fall in tile rows [h PR, (h+1) PR).
   double tr = 0.0, fro = 0.0;
-#pragma unroll
-  for (int a = 0; a < NW; ++a)
-#pragma unroll
-    for (int b = 0; b < NW; ++b) {
-      // X of the 2X - X^2 branch for this 32 x 32 block: all 16 loads issue before the first use
-      // (clamped addresses keep the masked-off elements in bounds)
-      float xv[16];
-      if (!sq) {
-#pragma unroll
-        for (int v = 0; v < 16; ++v) {
-          const int r = min(i0 + wr * (TB / 2) + a * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3), n - 1);
-          const int c = min(j0 + wc * (TB / 2) + b * 32 + (lane & 31), n - 1);
-          xv[v] = Xm[(size_t)r * ld + c];
-        }
-      }
-#pragma unroll
-      for (int v = 0; v < 16; ++v) {
-        const int r = i0 + wr * (TB / 2) + a * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
-        const int c = j0 + wc * (TB / 2) + b * 32 + (lane & 31);
-        if (r < n && c < n && (ti != tj || r <= c)) {
-          const float x2 = acc[a][b][v];
-          const float y = sq ? x2 : 2.0f * xv[v] - x2;
-          Ym[(size_t)r * ld + c] = y;
-          T[(r - i0) * (TB + 1) + (c - j0)] = y;
-          const double wgt = (r == c) ? 1.0 : 2.0;
-          fro += wgt * (double)y * (double)y;
-          if (r == c) tr += y;
-        }
-      }
-    }
+  if (tid == 0) *zsh = 0;
   __syncthreads();
-  {
-    // mirror: Y[j0 + cc][i0 + rr] = tile[rr][cc], consecutive lanes along rr (coalesced rows of Y,
-    // conflict-free LDS reads with the TB+1 float row pitch)
-    const int rmax = min(TB, n - i0), cmax = min(TB, n - j0);
-    for (int e = tid; e < TB * TB; e += 256) {
-      const int cc = e / TB, rr = e % TB;
-      if (rr < rmax && cc < cmax && (ti != tj || rr < cc)) Ym[(size_t)(j0 + cc) * ld + i0 + rr] = T[rr * (TB + 1) + cc];
+#pragma unroll 1
+  for (int h = 0; h < NP; ++h) {
+    // the element indices are offset by this zero, read from LDS in every pass, so that they are
+    // formed inside the pass loop (hoisting them for both passes would exhaust the registers)
+    const int z = *reinterpret_cast<volatile int*>(zsh);
+    if ((wr * (TB / 2)) / PR == h + z) {
+#pragma unroll
+      for (int a = 0; a < NW; ++a)
+#pragma unroll
+        for (int b = 0; b < NW; ++b) {
+          // X of the 2X - X^2 branch for this 32 x 32 block: all 16 loads issue before the first
+          // use (clamped addresses keep the masked-off elements in bounds)
+          float xv[16];
+          if (!sq) {
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+              const int r = min(z + i0 + wr * (TB / 2) + a * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3), n - 1);
+              const int c = min(z + j0 + wc * (TB / 2) + b * 32 + (lane & 31), n - 1);
+              xv[v] = Xm[(size_t)r * ld + c];
+            }
+          }
+#pragma unroll
+          for (int v = 0; v < 16; ++v) {
+            const int r = z + i0 + wr * (TB / 2) + a * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+            const int c = z + j0 + wc * (TB / 2) + b * 32 + (lane & 31);
+            if (r < n && c < n && (ti != tj || r <= c)) {
+              const float x2 = acc[a][b][v];
+              const float y = sq ? x2 : 2.0f * xv[v] - x2;
+              Ym[(size_t)r * ld + c] = y;
+              T[(r - i0 - h * PR) * (TB + 1) + (c - j0)] = y;
+              const double wgt = (r == c) ? 1.0 : 2.0;
+              fro += wgt * (double)y * (double)y;
+              if (r == c) tr += y;
+            }
+          }
+        }
     }
+    __syncthreads();
+    {
+      // mirror: Y[j0 + cc][i0 + rr] = tile[rr][cc], consecutive lanes along rr (coalesced rows of
+      // Y, conflict-free LDS reads with the TB+1 float row pitch)
+      const int rlo = h * PR;
+      const int rmax = min(TB, n - i0), cmax = min(TB, n - j0);
+      for (int e = tid; e < PR * TB; e += 256) {
+        const int cc = e / PR, rr = rlo + e % PR;
+        if (rr < rmax && cc < cmax && (ti != tj || rr < cc))
+          Ym[(size_t)(j0 + cc) * ld + i0 + rr] = T[(rr - rlo) * (TB + 1) + cc];
+      }
+    }
+    __syncthreads();
   }
   tr = wsum(tr);
   fro = wsum(fro);
