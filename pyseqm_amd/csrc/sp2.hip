@@ -190,12 +190,12 @@ __device__ inline void tri_decode(int t, int& ti, int& tj) {
 // of the molecule-major tile list are given to one XCD (block id mod 8), so the tiles that share a
 // molecule's row panels share that XCD's L2.
 template <int TB>
-__global__ __launch_bounds__(256, TB == 128 ? 3 : 4) void k_sp2_bf16x3(
+__global__ __launch_bounds__(256, TB == 128 ? 3 : 6) void k_sp2_bf16x3(
     const int* __restrict__ idx, int nact, int ntri, const int* __restrict__ norb,
     const long long* __restrict__ xoff, const int* __restrict__ xld, const float* __restrict__ Xin,
     float* __restrict__ Xout, double* __restrict__ st, double* __restrict__ part, int part_stride) {
-  // K slab: 16 at TB = 128 (48 KiB of LDS, three blocks per CU), 32 at TB = 64
-  constexpr int BK = TB == 128 ? 16 : 32;
+  // K slab of 16: 48 KiB of LDS at TB = 128 (three blocks per CU), 24 KiB at TB = 64 (six)
+  constexpr int BK = 16;
   constexpr int LDSK = BK + 8;             // padded LDS row (bf16), 16-byte aligned, conflict-free
   constexpr int NW = TB / 64;              // 32 x 32 blocks per wave side
   constexpr int NQ = TB * (BK / 4) / 256;  // float4 slab loads per thread and operand
