@@ -97,7 +97,8 @@ def serve(args, dev, rate=None, xl_trajectories=None):
         for _ in range(3):
             xl.step()
         xl.latencies.clear()
-    srv = SCFServer(dev, lanes=args.serve_lanes, density="sp2", eps=args.serve_eps)
+    srv = SCFServer(dev, lanes=args.serve_lanes, density="sp2", eps=args.serve_eps, base_lanes=args.serve_base_lanes,
+                    backlog=args.serve_backlog)
     stop = threading.Event()
     th = None
     if xl is not None:
@@ -132,7 +133,8 @@ def serve(args, dev, rate=None, xl_trajectories=None):
         "converged": int(sum(1 for r in done if r.result and r.result["converged"])),
         "scf_per_s": len(done) / (t_end - t0), "wall_s": t_end - t0,
         "latency_ms": percentiles(lat_all), "latency_ms_completed": percentiles(lat),
-        "batches": srv.batches, "lanes": args.serve_lanes, "hbm_budget_GiB": srv.hbm_budget / 2**30,
+        "batches": srv.batches, "lanes": args.serve_lanes, "base_lanes": srv.base_lanes, "backlog": srv.backlog,
+        "hbm_budget_GiB": srv.hbm_budget / 2**30,
         "peak_in_flight_GiB": srv.peak_in_use / 2**30, "density": "sp2 (bf16x3 + f64 McWeeny)",
         "eps_eV": args.serve_eps, "mix": f"50% AM1 alkanes n~U[2,30], 50% PM3 (H2O)_N N~U[4,{args.serve_water_max}]",
     }
@@ -252,7 +254,10 @@ def main():
     ap.add_argument("--serve-rate-light", type=float, default=300.0,
                     help="second, lighter Poisson rate whose latencies show the loop below saturation (0 = skip)")
     ap.add_argument("--serve-water-max", type=int, default=256)
-    ap.add_argument("--serve-lanes", type=int, default=3)
+    ap.add_argument("--serve-lanes", type=int, default=6)
+    ap.add_argument("--serve-base-lanes", type=int, default=3,
+                    help="lanes that always take work; the others only while the backlog is deep")
+    ap.add_argument("--serve-backlog", type=int, default=256)
     ap.add_argument("--serve-eps", type=float, default=1e-5)
     ap.add_argument("--serve-drain", type=float, default=20.0)
     ap.add_argument("--xl-trajectories", type=int, default=256)

@@ -71,7 +71,7 @@ class SCFServer:
     """Continuous-batching SCF service on one GPU (see module docstring)."""
 
     def __init__(self, device="cuda:0", lanes=3, density="sp2", eps=1.0e-5, converger=(2,),
-                 hbm_budget=None, batch_bytes=None, max_batch=512):
+                 hbm_budget=None, batch_bytes=None, max_batch=512, base_lanes=None, backlog=256):
         self.device = torch.device(device)
         self.density = density
         self.eps = float(eps)
@@ -82,6 +82,11 @@ class SCFServer:
         self.hbm_budget = int(hbm_budget if hbm_budget is not None else 0.6 * free)
         self.batch_bytes = int(batch_bytes if batch_bytes is not None else min(8 << 30, self.hbm_budget // lanes))
         self.max_batch = int(max_batch)
+        # lanes beyond base_lanes only take work while more than `backlog` requests wait: under
+        # light load a few larger batches keep latency low, under overload the extra concurrent
+        # batches overlap their host work and kernels and raise throughput
+        self.base_lanes = int(lanes if base_lanes is None else min(base_lanes, lanes))
+        self.backlog = int(backlog)
         self.queue = collections.deque()
         self.cv = threading.Condition()
         self.in_use = 0
@@ -114,18 +119,26 @@ class SCFServer:
             raise self.errors[0]
 
     # ----------------------------------------------------------- admission
+    @staticmethod
+    def size_class(norb):
+        """Blocks up to 256 orbitals run on the small-block kernels (64 x 64 SP2 tiles, FP64 DIIS
+        residual); a batch that mixes them with larger blocks would run every block on the
+        large-block ones, so batches keep to one class."""
+        return 0 if norb <= 256 else 1
+
     def _take(self):
-        """Oldest request first; then every queued request of the same method that fits the batch
-        cap, as long as the in-flight footprint stays under the HBM budget."""
+        """Oldest request first; then every queued request of the same method and size class that
+        fits the batch cap, as long as the in-flight footprint stays under the HBM budget."""
         q = self.queue
         head = q[0]
         if self.in_use + head.bytes > self.hbm_budget:
             return None
+        cls = self.size_class(head.norb)
         batch, total, keep = [head], head.bytes, collections.deque()
         q.popleft()
         while q and len(batch) < self.max_batch:
             r = q.popleft()
-            if (r.method == head.method and total + r.bytes <= self.batch_bytes
+            if (r.method == head.method and self.size_class(r.norb) == cls and total + r.bytes <= self.batch_bytes
                     and self.in_use + total + r.bytes <= self.hbm_budget):
                 batch.append(r)
                 total += r.bytes
@@ -143,7 +156,7 @@ class SCFServer:
             with self.cv:
                 got = None
                 while not self.stop or self.queue:
-                    if self.queue:
+                    if self.queue and (k < self.base_lanes or len(self.queue) > self.backlog or self.stop):
                         got = self._take()
                         if got is not None:
                             break
