@@ -134,7 +134,7 @@ class SCFEngine:
         nat.call("pyseqm_pair_integrals", b.L, nat.METHODS[self.method], nat.ptr(self.coords),
                  nat.ptr(self.atom_par), nat.ptr(self.w_xx), nat.ptr(self.w_xh), nat.ptr(self.w_hh),
                  nat.ptr(self.H), nat.ptr(self.part), nat.ptr(self.tile_v), nat.ptr(self.enuc),
-                 nat.ptr(self.enuc_pair), nat.ptr(b.d["pair_ref"]) if enuc_pair else None, self.stream)
+                 nat.ptr(self.enuc_pair), nat.ptr(b.pair_ref_device()) if enuc_pair else None, self.stream)
 
     # ------------------------------------------------------------------ fock
     def fock(self, P, F, active=None, eelec=None):

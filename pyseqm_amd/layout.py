@@ -164,24 +164,17 @@ class PackedBatch:
         self.tiles_host = T
         self.mol_tile_off = np.concatenate([k[1] + b for k, b in zip(kinds, base[:3])]).astype(np.int64)
         nt = len(T)
-        r = np.arange(64) // TILE
-        c = np.arange(64) % TILE
-        i = T[:, 1:2] + r[None]
-        j = T[:, 2:3] + c[None]
-        valid = (r[None] < T[:, 3:4]) & (c[None] < T[:, 4:5]) & ((T[:, 5:6] == 0) | (c[None] > r[None]))
-        mol = T[:, 0:1]
-        a0 = self.mol_atom_off[mol]
-        li, lj = i - a0, j - a0
-        nat = self.nat[mol]
-        ref = self.pair_base[mol] + li * nat - li * (li + 1) // 2 + (lj - li - 1)
-        self.pair_ref_host = np.where(valid, ref, -1).reshape(-1)
-        self.lane_valid = valid.reshape(-1)
+        self.tile_kind = np.concatenate([np.full(n, k) for k, n in enumerate(self.ntile)]).astype(np.int64)
+        # valid lanes per tile in closed form: nrow x ncol, or the strict upper triangle of a
+        # diagonal tile
+        nrow, ncol, diag = T[:, 3], T[:, 4], T[:, 5] != 0
+        rr = np.arange(TILE)
+        tri = (np.clip(ncol[:, None] - rr[None] - 1, 0, None) * (rr[None] < nrow[:, None])).sum(1)
+        nvalid = np.where(diag, tri, nrow * ncol).astype(np.int64)
         # compact integral storage: per tile NW rows of S = round_up(#valid lanes, 16) doubles,
         # valid lanes packed in lane order; offsets are local to each kind's buffer
-        nvalid = valid.sum(1).astype(np.int64)
         S = (nvalid + 15) // 16 * 16
-        kind = np.concatenate([np.full(n, k) for k, n in enumerate(self.ntile)]).astype(np.int64)
-        nw = np.asarray(W_PER_LANE, dtype=np.int64)[kind] if nt else np.zeros(0, np.int64)
+        nw = np.asarray(W_PER_LANE, dtype=np.int64)[self.tile_kind] if nt else np.zeros(0, np.int64)
         woff = np.zeros(nt, dtype=np.int64)
         self.w_size = [0, 0, 0]
         for k in range(3):
@@ -191,18 +184,58 @@ class PackedBatch:
             self.w_size[k] = int(blk.sum())
         self.tile_woff_host = woff
         self.tile_wstride = S
-        rank = np.cumsum(valid, axis=1) - 1
-        self.lane_wpos = np.where(valid, woff[:, None] + rank, -1).reshape(-1)
-        # partial slots: row r of tile t <-> atom row0 + r, column c <-> atom col0 + c
+        # partial slots: row r of tile t <-> atom row0 + r, column c <-> atom col0 + c; each atom's
+        # slots in increasing slot order (a stable sort by atom of the slot enumeration)
         tid = np.repeat(np.arange(nt), 16)
         s = np.tile(np.arange(16), nt)
         atom = np.where(s < 8, T[tid, 1] + s, T[tid, 2] + (s - 8))
         inrange = np.where(s < 8, s < T[tid, 3], (s - 8) < T[tid, 4])
         atom, slot = atom[inrange], (tid * 16 + s)[inrange]
-        order = np.lexsort((slot, atom))
+        order = np.argsort(atom, kind="stable")
         self.csr_idx_host = slot[order].astype(np.int64)
         self.csr_off_host = np.searchsorted(atom[order], np.arange(self.natoms + 1)).astype(np.int64)
-        self.tile_kind = np.concatenate([np.full(n, k) for k, n in enumerate(self.ntile)]).astype(np.int64)
+        self._lanes = None
+
+    def _lane_arrays(self):
+        """Per-lane (tile * 64 + lane) views, built on first use (adapters, tests, reference-order
+        outputs; the device engine does not need them): the lane's pair index in the reference's
+        (mol, i < j) order or -1, its validity, and the position of its first integral in its
+        kind's compact buffer or -1."""
+        if self._lanes is None:
+            T = self.tiles_host
+            r = np.arange(64) // TILE
+            c = np.arange(64) % TILE
+            i = T[:, 1:2] + r[None]
+            j = T[:, 2:3] + c[None]
+            valid = (r[None] < T[:, 3:4]) & (c[None] < T[:, 4:5]) & ((T[:, 5:6] == 0) | (c[None] > r[None]))
+            mol = T[:, 0:1]
+            a0 = self.mol_atom_off[mol]
+            li, lj = i - a0, j - a0
+            nat = self.nat[mol]
+            ref = self.pair_base[mol] + li * nat - li * (li + 1) // 2 + (lj - li - 1)
+            rank = np.cumsum(valid, axis=1) - 1
+            self._lanes = (np.where(valid, ref, -1).reshape(-1), valid.reshape(-1),
+                           np.where(valid, self.tile_woff_host[:, None] + rank, -1).reshape(-1))
+        return self._lanes
+
+    @property
+    def pair_ref_host(self):
+        return self._lane_arrays()[0]
+
+    @property
+    def lane_valid(self):
+        return self._lane_arrays()[1]
+
+    @property
+    def lane_wpos(self):
+        return self._lane_arrays()[2]
+
+    def pair_ref_device(self):
+        """Device copy of pair_ref_host (int32), made on first use."""
+        if "pair_ref" not in self.d:
+            self.d["pair_ref"] = torch.as_tensor(np.ascontiguousarray(self.pair_ref_host), dtype=torch.int32,
+                                                 device=self.device)
+        return self.d["pair_ref"]
 
     # ----------------------------------------------------------------- device
     def _to_device(self):
@@ -213,7 +246,6 @@ class PackedBatch:
             "mol_mat_off": torch.as_tensor(self.mol_mat_off, dtype=torch.int64, device=dev),
             "mol_tile_off": i32(self.mol_tile_off), "atom_mol": i32(self.atom_mol), "atom_ao": i32(self.atom_ao),
             "atom_Z": i32(self.Z), "csr_off": i32(self.csr_off_host), "csr_idx": i32(self.csr_idx_host),
-            "pair_ref": i32(self.pair_ref_host),
             "tile_woff": torch.as_tensor(self.tile_woff_host if len(self.tile_woff_host) else np.zeros(1, np.int64),
                                          dtype=torch.int64, device=dev),
         }
