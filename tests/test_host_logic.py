@@ -1,5 +1,5 @@
 """Host-side logic that needs no GPU: eigensolve size groups, XL-BOMD coefficient regrouping,
-serving footprints and the request stream."""
+serving footprints, the request stream and the server's admission / batching."""
 import numpy as np
 import pytest
 
@@ -54,3 +54,48 @@ def test_serving_stream_and_footprint():
         assert np.all(r.species[:-1] >= r.species[1:])  # the reference's atom order
         assert r.norb == 4 * (r.species > 1).sum() + (r.species == 1).sum()
     assert footprint_bytes(1536, 768 * 767 // 2, "sp2") > footprint_bytes(100, 1000, "sp2") > 0
+
+
+class _Req:
+    def __init__(self, rid, method, norb, nbytes):
+        self.rid, self.method, self.norb, self.bytes = rid, method, norb, nbytes
+
+
+class _Srv:
+    """The attributes SCFServer._take reads, without a device."""
+
+    def __init__(self, queue, budget=10**9, batch_bytes=10**8, max_batch=8):
+        import collections
+        self.queue = collections.deque(queue)
+        self.hbm_budget, self.batch_bytes, self.max_batch = budget, batch_bytes, max_batch
+        self.in_use, self.peak_in_use = 0, 0
+        self.size_class = SCFServer.size_class
+
+
+from pyseqm_amd.serving import SCFServer  # noqa: E402
+
+
+def test_serving_batches_keep_method_and_size_class_in_arrival_order():
+    q = [_Req(0, "PM3", 300, 10), _Req(1, "AM1", 50, 1), _Req(2, "PM3", 100, 2), _Req(3, "PM3", 900, 30),
+         _Req(4, "PM3", 260, 9), _Req(5, "AM1", 80, 1), _Req(6, "PM3", 40, 1)]
+    srv = _Srv(q)
+    batch, total = SCFServer._take(srv)
+    # the oldest request leads; same method and class (> 256 orbitals) follow in arrival order
+    assert [r.rid for r in batch] == [0, 3, 4] and total == 49
+    assert [r.rid for r in srv.queue] == [1, 2, 5, 6]
+    assert srv.in_use == 49
+    batch, _ = SCFServer._take(srv)
+    assert [r.rid for r in batch] == [1, 5]
+    batch, _ = SCFServer._take(srv)
+    assert [r.rid for r in batch] == [2, 6] and not srv.queue
+
+
+def test_serving_admission_respects_budgets():
+    q = [_Req(k, "PM3", 100, 40) for k in range(5)]
+    srv = _Srv(q, budget=100, batch_bytes=90)
+    batch, total = SCFServer._take(srv)
+    assert [r.rid for r in batch] == [0, 1] and total == 80      # batch cap 90
+    assert SCFServer._take(srv) is None                          # 80 in flight + 40 > budget 100
+    srv.in_use = 0
+    batch, _ = SCFServer._take(srv)
+    assert [r.rid for r in batch] == [2, 3]
