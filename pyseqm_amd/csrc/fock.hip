@@ -23,12 +23,14 @@ namespace {
 
 enum { XX = 0, XH = 1, HH = 2 };
 
+// Heavy-H and H-H tiles, one wave per tile (heavy-heavy tiles: k_fock_xx2 below).
 template <int KIND>
 __global__ __launch_bounds__(64) void k_fock_tiles(pyseqm_layout_t L, int tile_base, const double* __restrict__ w,
                                                     const double* __restrict__ P, const double* __restrict__ H,
                                                     double* __restrict__ F, double* __restrict__ part,
                                                     double* __restrict__ etile,
                                                     const unsigned char* __restrict__ active) {
+  static_assert(KIND == XH || KIND == HH, "heavy-heavy tiles run in k_fock_xx2");
   const int tl = xcd_run(blockIdx.x, gridDim.x);
   const int tile = tile_base + tl;
   const int lane = threadIdx.x;
@@ -37,7 +39,7 @@ __global__ __launch_bounds__(64) void k_fock_tiles(pyseqm_layout_t L, int tile_b
   int i, j;
   const bool valid = lane_pair(t, lane, i, j);
   constexpr int NR = KIND == HH ? 1 : 10;
-  constexpr int NC = KIND == XX ? 10 : 1;
+  constexpr int NC = 1;
   double rowv[NR], colv[NC];
 #pragma unroll
   for (int k = 0; k < NR; ++k) rowv[k] = 0.0;
@@ -51,52 +53,7 @@ __global__ __launch_bounds__(64) void k_fock_tiles(pyseqm_layout_t L, int tile_b
     const double* Pm = P + mo;
     const double* Hm = H + mo;
     double* Fm = F + mo;
-    if constexpr (KIND == XX) {
-      const int S = tile_wstride(t);
-      const double* wt = w + L.tile_woff[tile] + lane_rank(t, lane);
-      double pa[10], pb[10], pab[4][4];
-#pragma unroll
-      for (int k = 0; k < 10; ++k) {
-        const int mu = kTriI[k], nu = kTriJ[k];
-        const double f = mu == nu ? 1.0 : 2.0;
-        pa[k] = f * Pm[(size_t)(ai + mu) * n + ai + nu];
-        pb[k] = f * Pm[(size_t)(aj + mu) * n + aj + nu];
-      }
-#pragma unroll
-      for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b) pab[a][b] = Pm[(size_t)(ai + a) * n + aj + b];
-      double K[4][4];
-#pragma unroll
-      for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b) K[a][b] = 0.0;
-#pragma unroll
-      for (int k = 0; k < 10; ++k) {
-        const int mu = kTriI[k], nu = kTriJ[k];
-#pragma unroll
-        for (int l = 0; l < 10; ++l) {
-          const int la = kTriI[l], si = kTriJ[l];
-          const double v = wt[(size_t)(k * 10 + l) * S];
-          colv[l] += v * pa[k];
-          rowv[k] += v * pb[l];
-          K[mu][la] += v * pab[nu][si];
-          if (mu != nu) K[nu][la] += v * pab[mu][si];
-          if (la != si) K[mu][si] += v * pab[nu][la];
-          if (mu != nu && la != si) K[nu][si] += v * pab[mu][la];
-        }
-      }
-#pragma unroll
-      for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b) {
-          const double h = Hm[(size_t)(ai + a) * n + aj + b];
-          const double f = h - 0.5 * K[a][b];
-          Fm[(size_t)(ai + a) * n + aj + b] = f;
-          Fm[(size_t)(aj + b) * n + ai + a] = f;
-          e += pab[a][b] * (h + f);
-        }
-    } else if constexpr (KIND == XH) {
+    if constexpr (KIND == XH) {
       const int S = tile_wstride(t);
       const double* wt = w + L.tile_woff[tile] + lane_rank(t, lane);
       double pa[10], pab[4];
@@ -177,7 +134,8 @@ __device__ inline void xx_half(const double* __restrict__ wt, int S, const doubl
 // Heavy-heavy tiles with two waves per tile: wave q contracts the integrals (kl) with k in its half
 // of the 10 (mu nu) pairs, so each wave streams 50 of the 100 integral columns; wave 1 hands its
 // partial Coulomb column sums, exchange block and row sums to wave 0 through LDS, and wave 0 writes
-// F, the tile partials and the energy.  Same arithmetic as k_fock_tiles<XX>, summed in two halves.
+// F, the tile partials and the energy.  Per pair: exchange block K from the 4 x 4 off-diagonal P
+// block, Coulomb row / column sums from the packed diagonal blocks of P (off-diagonal pairs doubled).
 __global__ __launch_bounds__(128) void k_fock_xx2(pyseqm_layout_t L, const double* __restrict__ w,
                                                   const double* __restrict__ P, const double* __restrict__ H,
                                                   double* __restrict__ F, double* __restrict__ part,
