@@ -46,6 +46,8 @@ _SIGS = {
     "pyseqm_diis_residual": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                              c_int, c_int, c_void_p],
     "pyseqm_gemm_bf16x3": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p],
+    "pyseqm_diis_commutator_bf16x3": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                      c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p],
     "pyseqm_sp2_iterate_many": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                 c_void_p, c_int, c_double, c_int, c_void_p, c_int, c_int, c_void_p],
     "pyseqm_diis_solve": [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p],

@@ -379,6 +379,171 @@ __global__ __launch_bounds__(256, TB == 128 ? 3 : 6) void k_sp2_bf16x3(
   }
 }
 
+// DIIS commutator C = F P - P F on bf16x3 MFMA for the mixed-precision density step, over the
+// upper-triangle 128 x 128 tiles (I <= J) of every listed block only.  With F and P symmetric,
+// C_IJ = sum_k F[I][k] P[J][k] - sum_k P[I][k] F[J][k]: two K passes into one FP32 accumulator,
+// the second with its A operand negated (exact in the hi / lo split).  Operands are FP32
+// zero-padded copies of F and P in the SP2 X layout (xoff, ld = xld).  The epilogue writes the
+// tile's strict upper part of C (FP64) into the packed block (offset poff, leading dimension n)
+// and the tile's max |C| to tmax[k * tstride + tile].  Same slab pipeline as k_sp2_bf16x3<128>.
+__global__ __launch_bounds__(256, 3) void k_comm_bf16x3(const int* __restrict__ idx, int nact, int ntri,
+                                                       const int* __restrict__ norb,
+                                                       const long long* __restrict__ xoff,
+                                                       const int* __restrict__ xld, const float* __restrict__ Fx,
+                                                       const float* __restrict__ Px,
+                                                       const long long* __restrict__ poff, double* __restrict__ C,
+                                                       double* __restrict__ tmax, int tstride) {
+  constexpr int TB = 128, BK = 16, LDSK = BK + 8, NW = 2, NQ = TB * (BK / 4) / 256;
+  __shared__ __attribute__((aligned(16))) __bf16 smem[2 * 2 * 2 * TB * LDSK];
+  __shared__ double wmx[4];
+  __bf16(*As)[2][TB][LDSK] = reinterpret_cast<__bf16(*)[2][TB][LDSK]>(smem);
+  __bf16(*Bs)[2][TB][LDSK] = reinterpret_cast<__bf16(*)[2][TB][LDSK]>(smem + 2 * 2 * TB * LDSK);
+  const int total = nact * ntri, per = (total + 7) / 8;
+  const int w = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+  if (w >= total) return;
+  const int k = w / ntri;
+  const int tile = w % ntri;
+  const int m = idx[k];
+  const int n = norb[m];
+  const int ld = xld[m];
+  int ti, tj;
+  tri_decode(tile, ti, tj);
+  if (tj * TB >= n) return;  // not one of the molecule's tiles (its max slot is never read)
+  const float* Fm = Fx + xoff[m];
+  const float* Pm = Px + xoff[m];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = wave >> 1, wc = wave & 1;
+  const int i0 = ti * TB, j0 = tj * TB;
+  floatx16 acc[NW][NW];
+#pragma unroll
+  for (int a = 0; a < NW; ++a)
+#pragma unroll
+    for (int b = 0; b < NW; ++b)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) acc[a][b][v] = 0.0f;
+  float4 ra[NQ], rb[NQ];
+  // K step t = 0 .. 2 nslab - 1: pass t / nslab (0: F_I P_J, 1: -P_I F_J), slab t % nslab
+  const int nslab = ld / BK;
+  auto gload = [&](int t) {
+    const int pass = t >= nslab;
+    const int k0 = (t - pass * nslab) * BK;
+    const float* Am = pass ? Pm : Fm;
+    const float* Bm = pass ? Fm : Pm;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int f = tid + 256 * q;
+      const int row = f / (BK / 4), kq = f % (BK / 4);
+      ra[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+      rb[q] = ra[q];
+      if (i0 + row < n) ra[q] = *reinterpret_cast<const float4*>(Am + (size_t)(i0 + row) * ld + k0 + 4 * kq);
+      if (j0 + row < n) rb[q] = *reinterpret_cast<const float4*>(Bm + (size_t)(j0 + row) * ld + k0 + 4 * kq);
+    }
+  };
+  auto split_store = [&](const float4& v, float sgn, __bf16 (*hi)[LDSK], __bf16 (*lo)[LDSK], int row, int col) {
+    union {
+      __bf16 h[4];
+      uint2 u;
+    } ph, pl;
+    const float x[4] = {sgn * v.x, sgn * v.y, sgn * v.z, sgn * v.w};
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      ph.h[u] = (__bf16)x[u];
+      pl.h[u] = (__bf16)(x[u] - (float)ph.h[u]);
+    }
+    *reinterpret_cast<uint2*>(&hi[row][col]) = ph.u;
+    *reinterpret_cast<uint2*>(&lo[row][col]) = pl.u;
+  };
+  auto stage = [&](int buf, int t) {
+    const float sgn = t >= nslab ? -1.0f : 1.0f;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int f = tid + 256 * q;
+      const int row = f / (BK / 4), kq = f % (BK / 4);
+      split_store(ra[q], sgn, As[buf][0], As[buf][1], row, 4 * kq);
+      split_store(rb[q], 1.0f, Bs[buf][0], Bs[buf][1], row, 4 * kq);
+    }
+  };
+  gload(0);
+  stage(0, 0);
+  __syncthreads();
+  int cur = 0;
+  for (int t = 0; t < 2 * nslab; ++t) {
+    const bool more = t + 1 < 2 * nslab;
+    if (more) gload(t + 1);
+    {
+      bf16x8 ah[NW], al[NW], bh[NW], bl[NW];
+      const int kofs = 8 * (lane >> 5);
+#pragma unroll
+      for (int a = 0; a < NW; ++a) {
+        const int r = wr * (TB / 2) + a * 32 + (lane & 31);
+        ah[a] = *reinterpret_cast<const bf16x8*>(&As[cur][0][r][kofs]);
+        al[a] = *reinterpret_cast<const bf16x8*>(&As[cur][1][r][kofs]);
+        const int c = wc * (TB / 2) + a * 32 + (lane & 31);
+        bh[a] = *reinterpret_cast<const bf16x8*>(&Bs[cur][0][c][kofs]);
+        bl[a] = *reinterpret_cast<const bf16x8*>(&Bs[cur][1][c][kofs]);
+      }
+#pragma unroll
+      for (int a = 0; a < NW; ++a)
+#pragma unroll
+        for (int b = 0; b < NW; ++b) {
+          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[a], bh[b], acc[a][b], 0, 0, 0);
+          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a], bl[b], acc[a][b], 0, 0, 0);
+          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a], bh[b], acc[a][b], 0, 0, 0);
+        }
+    }
+    if (more) stage(cur ^ 1, t + 1);
+    __syncthreads();
+    cur ^= 1;
+  }
+  double* Cm = C + poff[m];
+  double mx = 0.0;
+#pragma unroll
+  for (int a = 0; a < NW; ++a)
+#pragma unroll
+    for (int b = 0; b < NW; ++b)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const int r = i0 + wr * (TB / 2) + a * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+        const int c = j0 + wc * (TB / 2) + b * 32 + (lane & 31);
+        if (r < n && c < n && r < c) {
+          const double y = (double)acc[a][b][v];
+          Cm[(size_t)r * n + c] = y;
+          mx = fmax(mx, fabs(y));
+        }
+      }
+  mx = wmax(mx);
+  if (lane == 0) wmx[wave] = mx;
+  __syncthreads();
+  if (tid == 0) tmax[(size_t)k * tstride + tile] = fmax(fmax(wmx[0], wmx[1]), fmax(wmx[2], wmx[3]));
+}
+
+// Per listed molecule: the max over its tiles' max |C| (fixed order) -> diis_err[m].
+__global__ __launch_bounds__(64) void k_comm_final(const int* __restrict__ idx, const int* __restrict__ norb,
+                                                   const double* __restrict__ tmax, int tstride,
+                                                   double* __restrict__ diis_err) {
+  const int k = blockIdx.x;
+  const int m = idx[k];
+  const int nt = (norb[m] + 127) / 128;
+  const int ntri = nt * (nt + 1) / 2;
+  double mx = 0.0;
+  for (int t = threadIdx.x; t < ntri; t += 64) mx = fmax(mx, tmax[(size_t)k * tstride + t]);
+  mx = wmax(mx);
+  if (threadIdx.x == 0) diis_err[m] = mx;
+}
+
+// FP64 packed block -> FP32 zero-padded block of the SP2 X layout (rows and columns n .. ld-1 zero).
+__global__ void k_pad32(const int* __restrict__ idx, const int* __restrict__ norb,
+                        const long long* __restrict__ poff, const long long* __restrict__ xoff,
+                        const int* __restrict__ xld, const double* __restrict__ A, float* __restrict__ Ax) {
+  const int m = idx[blockIdx.y];
+  const int n = norb[m], ld = xld[m];
+  const double* src = A + poff[m];
+  float* dst = Ax + xoff[m];
+  for (int r = blockIdx.x; r < ld; r += gridDim.x)
+    for (int c = threadIdx.x; c < ld; c += blockDim.x)
+      dst[(size_t)r * ld + c] = (r < n && c < n) ? (float)src[(size_t)r * n + c] : 0.0f;
+}
+
 // C = A B for FP64 packed square blocks with B symmetric (B is read by rows: B[k][j] = B[j][k]),
 // on bf16x3 MFMA with FP32 accumulation (relative error ~1e-6, enough for a DIIS error vector):
 // the operands are rounded to FP32 and split into bf16 hi + lo while they are staged through LDS.
@@ -731,6 +896,26 @@ extern "C" int pyseqm_sp2_iterate_many(const pyseqm_layout_t* L, const int* idx,
     if (rc != PYSEQM_OK) return rc;
   }
   return PYSEQM_OK;
+}
+
+extern "C" int pyseqm_diis_commutator_bf16x3(const pyseqm_layout_t* L, const int* idx, int nact, int nmax,
+                                             const double* F, const double* P, const long long* xoff, const int* xld,
+                                             float* Fx, float* Px, double* Cout, double* diis_err, double* tmax,
+                                             int tstride, void* stream) {
+  if (!L || nact < 0 || nmax <= 0) return PYSEQM_EINVAL;
+  const int nt = (nmax + 127) / 128;
+  const int ntri = nt * (nt + 1) / 2;
+  if (tstride < ntri) return PYSEQM_EINVAL;
+  if (nact == 0) return PYSEQM_OK;
+  hipStream_t s = (hipStream_t)stream;
+  const int rows = (nmax + 31) / 32 * 32;
+  const dim3 pgrid(rows < 2048 ? rows : 2048, nact);
+  k_pad32<<<pgrid, 256, 0, s>>>(idx, L->mol_norb, L->mol_mat_off, xoff, xld, F, Fx);
+  k_pad32<<<pgrid, 256, 0, s>>>(idx, L->mol_norb, L->mol_mat_off, xoff, xld, P, Px);
+  k_comm_bf16x3<<<8 * ((nact * ntri + 7) / 8), 256, 0, s>>>(idx, nact, ntri, L->mol_norb, xoff, xld, Fx, Px,
+                                                               L->mol_mat_off, Cout, tmax, tstride);
+  k_comm_final<<<nact, 64, 0, s>>>(idx, L->mol_norb, tmax, tstride, diis_err);
+  return st_ok();
 }
 
 extern "C" int pyseqm_gemm_bf16x3(const pyseqm_layout_t* L, const int* idx, int nact, int nmax, const double* A,

@@ -190,6 +190,13 @@ class SCFEngine:
             return e, V, nmax
         return None
 
+    def _sp2_workspace(self):
+        ws = getattr(self, "_sp2ws", None)
+        if ws is None:
+            from .sp2 import SP2Workspace
+            ws = self._sp2ws = SP2Workspace(self)
+        return ws
+
     def density(self, F, idx, nact, Pout, fixed_iters=None):
         if self.density_kind == "eigh":
             self.density_eigh(F, idx, nact, Pout)
@@ -298,9 +305,16 @@ class SCFEngine:
             Fslot = Fring[counter]
             nat.call("pyseqm_copy_active", b.L, nat.ptr(idx), nact, nmax, nat.ptr(F), nat.ptr(Fslot), self.stream)
             nb = nat.red_blocks(nmax)
-            nat.call("pyseqm_diis_residual", b.L, nat.ptr(idx), nact, nmax, nat.ptr(F), nat.ptr(P), nat.ptr(X),
-                     nat.ptr(Cring[counter]), nat.ptr(diis_err), nat.ptr(self.red), nb,
-                     int(self.density_kind == "sp2" and nmax > MIXED_DIIS_MIN_N), self.stream)
+            if self.density_kind == "sp2" and nmax > MIXED_DIIS_MIN_N:
+                # bf16x3 commutator on the upper triangle (the ring is read there only), from FP32
+                # copies of F and P in the SP2 X buffers, which are free until the density step
+                ws = self._sp2_workspace()
+                nat.call("pyseqm_diis_commutator_bf16x3", b.L, nat.ptr(idx), nact, nmax, nat.ptr(F), nat.ptr(P),
+                         nat.ptr(ws.xoff), nat.ptr(ws.xld), nat.ptr(ws.Xa), nat.ptr(ws.Xb), nat.ptr(Cring[counter]),
+                         nat.ptr(diis_err), nat.ptr(self.red), nat.MAX_RED_BLOCKS * 16, self.stream)
+            else:
+                nat.call("pyseqm_diis_residual", b.L, nat.ptr(idx), nact, nmax, nat.ptr(F), nat.ptr(P), nat.ptr(X),
+                         nat.ptr(Cring[counter]), nat.ptr(diis_err), nat.ptr(self.red), nb, 0, self.stream)
             nat.call("pyseqm_diis_dots", b.L, nat.ptr(idx), nact, nat.ptr(Cring), total, counter, cF, NFOCK,
                      nat.ptr(EMAT), nat.ptr(self.red), nb, self.stream)
             reset_t = None

@@ -140,3 +140,47 @@ def test_diis_residual_is_commutator():
         assert (got - ref).abs().max().item() < 1e-11 * max(1.0, ref.abs().max().item()), m
         assert torch.equal(got, -got.T), m
         assert abs(err[m].item() - got.abs().max().item()) == 0.0, m
+
+
+def test_diis_commutator_bf16x3():
+    """pyseqm_diis_commutator_bf16x3 (mixed-precision DIIS residual): the strict upper triangle of
+    F P - P F and its max-norm, on blocks of 270-420 orbitals (ragged, not multiples of the 128
+    tiles); the lower triangle of the output is left untouched.  FP - PF is a cancelling difference
+    of two products, so its bf16x3 error scales with max|FP| (~1e-5 relative, as for the plain
+    bf16x3 product); the error must also stay at the level of the product-then-transpose route
+    (pyseqm_gemm_bf16x3, then X - X^T) on the same inputs."""
+    from pyseqm_amd.sp2 import SP2Workspace
+    mols = [W.water_cluster(nw, seed=k) for k, nw in enumerate((45, 60, 70))]
+    sp, c = W._pad(mols)
+    eng = SCFEngine(sp, c, "PM3", DEV)
+    r = eng.scf(eps=1e-5, converger=(2,), density="sp2")
+    b = eng.b
+    F = torch.empty_like(r.P)
+    eng.fock(r.P, F)
+    ws = SP2Workspace(eng)
+    idx = torch.arange(b.nmol, dtype=torch.int32, device=DEV)
+    C = torch.full_like(F, 7.0)
+    err = torch.zeros(b.nmol, dtype=torch.float64, device=DEV)
+    nat.call("pyseqm_diis_commutator_bf16x3", b.L, nat.ptr(idx), b.nmol, int(b.nmax), nat.ptr(F), nat.ptr(r.P),
+             nat.ptr(ws.xoff), nat.ptr(ws.xld), nat.ptr(ws.Xa), nat.ptr(ws.Xb), nat.ptr(C), nat.ptr(err),
+             nat.ptr(eng.red), nat.MAX_RED_BLOCKS * 16, eng.stream)
+    X = torch.zeros_like(F)
+    nat.call("pyseqm_gemm_bf16x3", b.L, nat.ptr(idx), b.nmol, int(b.nmax), nat.ptr(F), nat.ptr(r.P), nat.ptr(X),
+             eng.stream)
+    torch.cuda.synchronize()
+    for m in range(b.nmol):
+        n, o = int(b.norb[m]), int(b.mol_mat_off[m])
+        Fm = F[o:o + n * n].view(n, n)
+        Pm = r.P[o:o + n * n].view(n, n)
+        FP = Fm @ Pm
+        ref = FP - Pm @ Fm
+        got = C[o:o + n * n].view(n, n)
+        Xm = X[o:o + n * n].view(n, n)
+        old = Xm - Xm.T
+        up = torch.triu(torch.ones(n, n, dtype=torch.bool, device=DEV), 1)
+        scale = FP.abs().max().item()
+        e_new = (got[up] - ref[up]).abs().max().item()
+        e_old = (old[up] - ref[up]).abs().max().item()
+        assert e_new < 1e-5 * scale and e_new < 3.0 * e_old + 1e-12, (m, e_new, e_old, scale)
+        assert torch.all(got[~up] == 7.0), m
+        assert abs(err[m].item() - ref[up].abs().max().item()) < 1e-5 * scale, m
