@@ -174,11 +174,12 @@ int pyseqm_diis_residual(const pyseqm_layout_t* L, const int* idx, int nact, int
                          const double* P, double* X, double* Cout, double* diis_err, double* red, int nb,
                          int mixed, void* stream);
 /* The DIIS residual of the mixed-precision density step (scf_loop.py:997-1002): C = F P - P F on
- * bf16x3 MFMA over the upper-triangle 128 x 128 tiles only; writes the strict upper triangle of
+ * bf16x3 MFMA over the upper-triangle tiles only (edge tb = 64 when nmax <= 256, else 128); writes
+ * the strict upper triangle of
  * each listed block of Cout (packed layout; the lower triangle is left as it was) and max|C| to
  * diis_err[m].  Fx, Px: FP32 scratch in the SP2 X layout (xoff, xld), overwritten with zero-padded
  * copies of F and P; tmax: scratch of nact * tstride doubles, tstride >= nt (nt + 1) / 2 with
- * nt = ceil(nmax / 128). */
+ * nt = ceil(nmax / tb). */
 int pyseqm_diis_commutator_bf16x3(const pyseqm_layout_t* L, const int* idx, int nact, int nmax, const double* F,
                                   const double* P, const long long* xoff, const int* xld, float* Fx, float* Px,
                                   double* Cout, double* diis_err, double* tmax, int tstride, void* stream);

@@ -380,20 +380,23 @@ __global__ __launch_bounds__(256, TB == 128 ? 3 : 6) void k_sp2_bf16x3(
 }
 
 // DIIS commutator C = F P - P F on bf16x3 MFMA for the mixed-precision density step, over the
-// upper-triangle 128 x 128 tiles (I <= J) of every listed block only.  With F and P symmetric,
+// upper-triangle TB x TB tiles (I <= J) of every listed block only (TB = 64 while the largest
+// block has at most 256 orbitals, else 128).  With F and P symmetric,
 // C_IJ = sum_k F[I][k] P[J][k] - sum_k P[I][k] F[J][k]: two K passes into one FP32 accumulator,
 // the second with its A operand negated (exact in the hi / lo split).  Operands are FP32
 // zero-padded copies of F and P in the SP2 X layout (xoff, ld = xld).  The epilogue writes the
 // tile's strict upper part of C (FP64) into the packed block (offset poff, leading dimension n)
-// and the tile's max |C| to tmax[k * tstride + tile].  Same slab pipeline as k_sp2_bf16x3<128>.
-__global__ __launch_bounds__(256, 3) void k_comm_bf16x3(const int* __restrict__ idx, int nact, int ntri,
+// and the tile's max |C| to tmax[k * tstride + tile].  Same slab pipeline as k_sp2_bf16x3<TB>.
+template <int TB>
+__global__ __launch_bounds__(256, TB == 128 ? 3 : 6) void k_comm_bf16x3(const int* __restrict__ idx, int nact,
+                                                                       int ntri,
                                                        const int* __restrict__ norb,
                                                        const long long* __restrict__ xoff,
                                                        const int* __restrict__ xld, const float* __restrict__ Fx,
                                                        const float* __restrict__ Px,
                                                        const long long* __restrict__ poff, double* __restrict__ C,
                                                        double* __restrict__ tmax, int tstride) {
-  constexpr int TB = 128, BK = 16, LDSK = BK + 8, NW = 2, NQ = TB * (BK / 4) / 256;
+  constexpr int BK = 16, LDSK = BK + 8, NW = TB / 64, NQ = TB * (BK / 4) / 256;
   __shared__ __attribute__((aligned(16))) __bf16 smem[2 * 2 * 2 * TB * LDSK];
   __shared__ double wmx[4];
   __bf16(*As)[2][TB][LDSK] = reinterpret_cast<__bf16(*)[2][TB][LDSK]>(smem);
@@ -519,11 +522,11 @@ __global__ __launch_bounds__(256, 3) void k_comm_bf16x3(const int* __restrict__ 
 
 // Per listed molecule: the max over its tiles' max |C| (fixed order) -> diis_err[m].
 __global__ __launch_bounds__(64) void k_comm_final(const int* __restrict__ idx, const int* __restrict__ norb,
-                                                   const double* __restrict__ tmax, int tstride,
+                                                   const double* __restrict__ tmax, int tstride, int tb,
                                                    double* __restrict__ diis_err) {
   const int k = blockIdx.x;
   const int m = idx[k];
-  const int nt = (norb[m] + 127) / 128;
+  const int nt = (norb[m] + tb - 1) / tb;
   const int ntri = nt * (nt + 1) / 2;
   double mx = 0.0;
   for (int t = threadIdx.x; t < ntri; t += 64) mx = fmax(mx, tmax[(size_t)k * tstride + t]);
@@ -903,7 +906,8 @@ extern "C" int pyseqm_diis_commutator_bf16x3(const pyseqm_layout_t* L, const int
                                              float* Fx, float* Px, double* Cout, double* diis_err, double* tmax,
                                              int tstride, void* stream) {
   if (!L || nact < 0 || nmax <= 0) return PYSEQM_EINVAL;
-  const int nt = (nmax + 127) / 128;
+  const int tb = nmax <= kSp2SmallTileMaxN ? 64 : 128;
+  const int nt = (nmax + tb - 1) / tb;
   const int ntri = nt * (nt + 1) / 2;
   if (tstride < ntri) return PYSEQM_EINVAL;
   if (nact == 0) return PYSEQM_OK;
@@ -912,9 +916,14 @@ extern "C" int pyseqm_diis_commutator_bf16x3(const pyseqm_layout_t* L, const int
   const dim3 pgrid(rows < 2048 ? rows : 2048, nact);
   k_pad32<<<pgrid, 256, 0, s>>>(idx, L->mol_norb, L->mol_mat_off, xoff, xld, F, Fx);
   k_pad32<<<pgrid, 256, 0, s>>>(idx, L->mol_norb, L->mol_mat_off, xoff, xld, P, Px);
-  k_comm_bf16x3<<<8 * ((nact * ntri + 7) / 8), 256, 0, s>>>(idx, nact, ntri, L->mol_norb, xoff, xld, Fx, Px,
-                                                               L->mol_mat_off, Cout, tmax, tstride);
-  k_comm_final<<<nact, 64, 0, s>>>(idx, L->mol_norb, tmax, tstride, diis_err);
+  const int grid = 8 * ((nact * ntri + 7) / 8);
+  if (tb == 64)
+    k_comm_bf16x3<64><<<grid, 256, 0, s>>>(idx, nact, ntri, L->mol_norb, xoff, xld, Fx, Px, L->mol_mat_off, Cout,
+                                           tmax, tstride);
+  else
+    k_comm_bf16x3<128><<<grid, 256, 0, s>>>(idx, nact, ntri, L->mol_norb, xoff, xld, Fx, Px, L->mol_mat_off, Cout,
+                                            tmax, tstride);
+  k_comm_final<<<nact, 64, 0, s>>>(idx, L->mol_norb, tmax, tstride, tb, diis_err);
   return st_ok();
 }
 

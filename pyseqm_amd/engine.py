@@ -29,8 +29,11 @@ class SCFResult:
 EIGH_MIN_N = 68
 # The DIIS product F P goes to the bf16x3 GEMM (128-wide tiles) only for blocks larger than this;
 # smaller blocks stay on the FP64 GEMM with 64-wide tiles, which is faster at that size.
-MIXED_DIIS_MIN_N = 256
 EIGH_MIN_N_BATCH = 16
+# largest block whose DIIS residual stays on the FP64 GEMM with the mixed-precision density (the
+# bf16x3 commutator is faster per iteration, but on small blocks its coarser residual costs SCF
+# iterations: C2 converged in 11 instead of 9)
+MIXED_DIIS_MIN_N = 256
 
 
 def size_groups(norb, ratio=0.8, min_group=32, floor=EIGH_MIN_N):

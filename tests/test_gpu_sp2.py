@@ -142,15 +142,16 @@ def test_diis_residual_is_commutator():
         assert abs(err[m].item() - got.abs().max().item()) == 0.0, m
 
 
-def test_diis_commutator_bf16x3():
+@pytest.mark.parametrize("sizes", [(45, 60, 70), (10, 21, 40)])
+def test_diis_commutator_bf16x3(sizes):
     """pyseqm_diis_commutator_bf16x3 (mixed-precision DIIS residual): the strict upper triangle of
-    F P - P F and its max-norm, on blocks of 270-420 orbitals (ragged, not multiples of the 128
-    tiles); the lower triangle of the output is left untouched.  FP - PF is a cancelling difference
+    F P - P F and its max-norm, on water clusters of 270-420 orbitals (128 x 128 tiles) and of 60-240
+    orbitals (64 x 64 tiles), ragged sizes; the lower triangle of the output is left untouched.  FP - PF is a cancelling difference
     of two products, so its bf16x3 error scales with max|FP| (~1e-5 relative, as for the plain
     bf16x3 product); the error must also stay at the level of the product-then-transpose route
     (pyseqm_gemm_bf16x3, then X - X^T) on the same inputs."""
     from pyseqm_amd.sp2 import SP2Workspace
-    mols = [W.water_cluster(nw, seed=k) for k, nw in enumerate((45, 60, 70))]
+    mols = [W.water_cluster(nw, seed=k) for k, nw in enumerate(sizes)]
     sp, c = W._pad(mols)
     eng = SCFEngine(sp, c, "PM3", DEV)
     r = eng.scf(eps=1e-5, converger=(2,), density="sp2")
