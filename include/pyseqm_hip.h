@@ -219,12 +219,15 @@ int pyseqm_sp2_iterate(const pyseqm_layout_t* L, const int* idx, int nact, int n
 int pyseqm_sp2_iterate_many(const pyseqm_layout_t* L, const int* idx, int nact, int nmax, const long long* xoff,
                             const int* xld, float* Xa, float* Xb, double* state, double* part, int part_stride,
                             double eps, int max_iter, int* ndone, int it0, int count, void* stream);
-/* FP64 McWeeny purification of the final X (SP2.py:9-85 followed by the FP64 correction this
- * path adds): Y2 = X X, then P = 2 (3 Y2 - 2 Y2 X) written to the packed density P (offsets
- * mol_mat_off).  X is each molecule's FP32 block in Xa or Xb by the parity of its iteration count,
- * widened exactly to FP64 as it is loaded; Y2 is an FP64 scratch with the X layout. */
+/* McWeeny purification of the final X (SP2.py:9-85 followed by the FP64 correction this path
+ * adds), P = 2 (3 X^2 - 2 X^3) written to the packed density P (offsets mol_mat_off), computed as
+ * Y2 = X X on the FP64 MFMA GEMM and P = 2 (Y2 - 2 X D), D = Y2 - X, with X D on bf16x3 MFMA (D is
+ * of the size of X's idempotency error, so P keeps FP64 accuracy: ~1e-11 for a converged X).
+ * X is each molecule's FP32 block in Xa or Xb by the parity of its iteration count, widened
+ * exactly to FP64 as it is loaded; D (FP32) overwrites the molecule's block in the other buffer;
+ * Y2 is an FP64 scratch with the X layout (upper-triangle tiles written). */
 int pyseqm_sp2_mcweeny(const pyseqm_layout_t* L, const int* idx, int nact, int nmax, const long long* xoff,
-                       const int* xld, const double* state, const float* Xa, const float* Xb, double* Y2, double* P,
+                       const int* xld, const double* state, float* Xa, float* Xb, double* Y2, double* P,
                        void* stream);
 
 /* ------------------------------------------------------------------ XL-BOMD / MD

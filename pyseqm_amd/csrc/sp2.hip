@@ -379,45 +379,22 @@ __global__ __launch_bounds__(256, TB == 128 ? 3 : 6) void k_sp2_bf16x3(
   }
 }
 
-// DIIS commutator C = F P - P F on bf16x3 MFMA for the mixed-precision density step, over the
-// upper-triangle TB x TB tiles (I <= J) of every listed block only (TB = 64 while the largest
-// block has at most 256 orbitals, else 128).  With F and P symmetric,
-// C_IJ = sum_k F[I][k] P[J][k] - sum_k P[I][k] F[J][k]: two K passes into one FP32 accumulator,
-// the second with its A operand negated (exact in the hi / lo split).  Operands are FP32
-// zero-padded copies of F and P in the SP2 X layout (xoff, ld = xld).  The epilogue writes the
-// tile's strict upper part of C (FP64) into the packed block (offset poff, leading dimension n)
-// and the tile's max |C| to tmax[k * tstride + tile].  Same slab pipeline as k_sp2_bf16x3<TB>.
+// One TB x TB output tile of sum_p sgn_p A_p[I] B_p[J]^T on bf16x3 MFMA with FP32 accumulation,
+// over npass (1 or 2) products: the rows i0 .. i0+TB-1 of A_p and j0 .. j0+TB-1 of B_p are FP32
+// with leading dimension ld, zero-padded to ld columns (rows at or beyond n read as zero); pass 1's
+// A operand is negated (exact in the hi / lo split).  4 waves in 2 x 2, each TB/2 x TB/2 of 32 x 32
+// MFMA blocks; K slabs of 16 staged through double-buffered LDS as bf16 hi / lo (row pitch 24), the
+// next slab's global loads in flight while the current slab is multiplied.  smem holds
+// 2 x 2 x 2 x TB x 24 bf16 (48 KiB at TB = 128).
 template <int TB>
-__global__ __launch_bounds__(256, TB == 128 ? 3 : 6) void k_comm_bf16x3(const int* __restrict__ idx, int nact,
-                                                                       int ntri,
-                                                       const int* __restrict__ norb,
-                                                       const long long* __restrict__ xoff,
-                                                       const int* __restrict__ xld, const float* __restrict__ Fx,
-                                                       const float* __restrict__ Px,
-                                                       const long long* __restrict__ poff, double* __restrict__ C,
-                                                       double* __restrict__ tmax, int tstride) {
+__device__ inline void bf16x3_tile(__bf16* smem, floatx16 (&acc)[TB / 64][TB / 64], const float* A0,
+                                   const float* B0, const float* A1, const float* B1, int npass, int n, int ld,
+                                   int i0, int j0) {
   constexpr int BK = 16, LDSK = BK + 8, NW = TB / 64, NQ = TB * (BK / 4) / 256;
-  __shared__ __attribute__((aligned(16))) __bf16 smem[2 * 2 * 2 * TB * LDSK];
-  __shared__ double wmx[4];
   __bf16(*As)[2][TB][LDSK] = reinterpret_cast<__bf16(*)[2][TB][LDSK]>(smem);
   __bf16(*Bs)[2][TB][LDSK] = reinterpret_cast<__bf16(*)[2][TB][LDSK]>(smem + 2 * 2 * TB * LDSK);
-  const int total = nact * ntri, per = (total + 7) / 8;
-  const int w = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
-  if (w >= total) return;
-  const int k = w / ntri;
-  const int tile = w % ntri;
-  const int m = idx[k];
-  const int n = norb[m];
-  const int ld = xld[m];
-  int ti, tj;
-  tri_decode(tile, ti, tj);
-  if (tj * TB >= n) return;  // not one of the molecule's tiles (its max slot is never read)
-  const float* Fm = Fx + xoff[m];
-  const float* Pm = Px + xoff[m];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = wave >> 1, wc = wave & 1;
-  const int i0 = ti * TB, j0 = tj * TB;
-  floatx16 acc[NW][NW];
 #pragma unroll
   for (int a = 0; a < NW; ++a)
 #pragma unroll
@@ -425,13 +402,13 @@ __global__ __launch_bounds__(256, TB == 128 ? 3 : 6) void k_comm_bf16x3(const in
 #pragma unroll
       for (int v = 0; v < 16; ++v) acc[a][b][v] = 0.0f;
   float4 ra[NQ], rb[NQ];
-  // K step t = 0 .. 2 nslab - 1: pass t / nslab (0: F_I P_J, 1: -P_I F_J), slab t % nslab
+  // K step t = 0 .. npass nslab - 1: pass t / nslab, slab t % nslab
   const int nslab = ld / BK;
   auto gload = [&](int t) {
     const int pass = t >= nslab;
     const int k0 = (t - pass * nslab) * BK;
-    const float* Am = pass ? Pm : Fm;
-    const float* Bm = pass ? Fm : Pm;
+    const float* Am = pass ? A1 : A0;
+    const float* Bm = pass ? B1 : B0;
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
       const int f = tid + 256 * q;
@@ -470,8 +447,8 @@ __global__ __launch_bounds__(256, TB == 128 ? 3 : 6) void k_comm_bf16x3(const in
   stage(0, 0);
   __syncthreads();
   int cur = 0;
-  for (int t = 0; t < 2 * nslab; ++t) {
-    const bool more = t + 1 < 2 * nslab;
+  for (int t = 0; t < npass * nslab; ++t) {
+    const bool more = t + 1 < npass * nslab;
     if (more) gload(t + 1);
     {
       bf16x8 ah[NW], al[NW], bh[NW], bl[NW];
@@ -498,6 +475,46 @@ __global__ __launch_bounds__(256, TB == 128 ? 3 : 6) void k_comm_bf16x3(const in
     __syncthreads();
     cur ^= 1;
   }
+}
+
+// DIIS commutator C = F P - P F on bf16x3 MFMA for the mixed-precision density step, over the
+// upper-triangle TB x TB tiles (I <= J) of every listed block only (TB = 64 while the largest
+// block has at most 256 orbitals, else 128).  With F and P symmetric,
+// C_IJ = sum_k F[I][k] P[J][k] - sum_k P[I][k] F[J][k]: two K passes into one FP32 accumulator,
+// the second with its A operand negated.  Operands are FP32 zero-padded copies of F and P in the
+// SP2 X layout (xoff, ld = xld).  The epilogue writes the tile's strict upper part of C (FP64) into
+// the packed block (offset poff, leading dimension n) and the tile's max |C| to
+// tmax[k * tstride + tile].
+template <int TB>
+__global__ __launch_bounds__(256, TB == 128 ? 3 : 6) void k_comm_bf16x3(const int* __restrict__ idx, int nact,
+                                                                       int ntri,
+                                                       const int* __restrict__ norb,
+                                                       const long long* __restrict__ xoff,
+                                                       const int* __restrict__ xld, const float* __restrict__ Fx,
+                                                       const float* __restrict__ Px,
+                                                       const long long* __restrict__ poff, double* __restrict__ C,
+                                                       double* __restrict__ tmax, int tstride) {
+  constexpr int LDSK = 16 + 8, NW = TB / 64;
+  __shared__ __attribute__((aligned(16))) __bf16 smem[2 * 2 * 2 * TB * LDSK];
+  __shared__ double wmx[4];
+  const int total = nact * ntri, per = (total + 7) / 8;
+  const int w = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+  if (w >= total) return;
+  const int k = w / ntri;
+  const int tile = w % ntri;
+  const int m = idx[k];
+  const int n = norb[m];
+  const int ld = xld[m];
+  int ti, tj;
+  tri_decode(tile, ti, tj);
+  if (tj * TB >= n) return;  // not one of the molecule's tiles (its max slot is never read)
+  const float* Fm = Fx + xoff[m];
+  const float* Pm = Px + xoff[m];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = wave >> 1, wc = wave & 1;
+  const int i0 = ti * TB, j0 = tj * TB;
+  floatx16 acc[NW][NW];
+  bf16x3_tile<TB>(smem, acc, Fm, Pm, Pm, Fm, 2, n, ld, i0, j0);
   double* Cm = C + poff[m];
   double mx = 0.0;
 #pragma unroll
@@ -532,6 +549,95 @@ __global__ __launch_bounds__(64) void k_comm_final(const int* __restrict__ idx, 
   for (int t = threadIdx.x; t < ntri; t += 64) mx = fmax(mx, tmax[(size_t)k * tstride + t]);
   mx = wmax(mx);
   if (threadIdx.x == 0) diis_err[m] = mx;
+}
+
+// Second half of the McWeeny step P = 2 (3 X^2 - 2 X^3) of the final SP2 matrix X, written as
+// P = 2 (Y2 - 2 X D) with Y2 = X^2 (FP64) and D = Y2 - X (X^3 = X Y2 = X^2 + X D): D is of the size
+// of X's idempotency error (~1e-6 after SP2), so X D on bf16x3 MFMA (relative error ~1e-6) adds
+// ~1e-12 to P and the step keeps FP64 accuracy with one FP64 GEMM instead of two.  X is each
+// molecule's FP32 block in Xa or Xb by the parity of its iteration count, D (FP32, both triangles)
+// the other buffer; Y2 (FP64, X layout) is read on the upper-triangle tiles.  Upper-triangle
+// TB x TB tiles; the epilogue writes P to the packed block (offset poff, leading dimension n) and
+// mirrors each tile through LDS in passes of 32 rows as coalesced rows of the lower triangle.
+template <int TB>
+__global__ __launch_bounds__(256, TB == 128 ? 3 : 6) void k_mcw_bf16x3(
+    const int* __restrict__ idx, int nact, int ntri, const int* __restrict__ norb,
+    const long long* __restrict__ xoff, const int* __restrict__ xld, const float* __restrict__ Xa,
+    const float* __restrict__ Xb, const double* __restrict__ st, const double* __restrict__ Y2,
+    const long long* __restrict__ poff, double* __restrict__ P) {
+  constexpr int LDSK = 16 + 8, NW = TB / 64;
+  constexpr int PR = 32, NP = TB / PR, LDT = TB + 1;  // mirror passes of PR tile rows, FP64 LDS pitch
+  constexpr int SLAB_BYTES = 2 * 2 * 2 * TB * LDSK * 2;
+  __shared__ __attribute__((aligned(16))) __bf16 smem[2 * 2 * 2 * TB * LDSK];
+  const int total = nact * ntri, per = (total + 7) / 8;
+  const int w = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+  if (w >= total) return;
+  const int tile = w % ntri;
+  const int m = idx[w / ntri];
+  const int n = norb[m];
+  const int ld = xld[m];
+  int ti, tj;
+  tri_decode(tile, ti, tj);
+  if (tj * TB >= n) return;
+  const bool odd = (((long long)st[(size_t)m * S_NST + S_IT]) & 1) != 0;
+  const float* Xm = (odd ? Xb : Xa) + xoff[m];
+  const float* Dm = (odd ? Xa : Xb) + xoff[m];
+  const double* Ym = Y2 + xoff[m];
+  double* Pm = P + poff[m];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = wave >> 1, wc = wave & 1;
+  const int i0 = ti * TB, j0 = tj * TB;
+  floatx16 acc[NW][NW];
+  // (X D)_IJ = sum_k X[I][k] D[J][k] (D symmetric)
+  bf16x3_tile<TB>(smem, acc, Xm, Dm, Xm, Dm, 1, n, ld, i0, j0);
+  double* T = reinterpret_cast<double*>(smem);
+  int* zsh = reinterpret_cast<int*>(T + PR * LDT);
+  static_assert(PR * LDT * 8 + 4 <= SLAB_BYTES, "mirror buffer exceeds the slab LDS");
+  if (tid == 0) *zsh = 0;
+  __syncthreads();
+  const int rmax = min(TB, n - i0), cmax = min(TB, n - j0);
+#pragma unroll 1
+  for (int h = 0; h < NP; ++h) {
+    // the element indices are offset by this zero, read from LDS in every pass, so that they are
+    // formed inside the pass loop (hoisting them for all passes would exhaust the registers)
+    const int z = *reinterpret_cast<volatile int*>(zsh);
+    // pass h: tile rows [h PR, (h+1) PR) = the 32-row MFMA block a = ha of the waves in row wr = hw;
+    // those waves put their X D block into LDS
+    const int hw = (h * PR) / (TB / 2), ha = ((h * PR) % (TB / 2)) / 32;
+    if (wr == hw + z) {
+#pragma unroll
+      for (int a = 0; a < NW; ++a) {
+        if (a != ha) continue;
+#pragma unroll
+        for (int b = 0; b < NW; ++b)
+#pragma unroll
+          for (int v = 0; v < 16; ++v) {
+            const int rr = z + a * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);  // row in the pass
+            const int cc = z + wc * (TB / 2) + b * 32 + (lane & 31);
+            T[(rr - a * 32) * LDT + cc] = (double)acc[a][b][v];
+          }
+      }
+    }
+    __syncthreads();
+    // P = 2 (Y2 - 2 X D) on the tile's upper part, consecutive threads along a row
+    for (int e = tid; e < PR * TB; e += 256) {
+      const int rp = e / TB, cc = e % TB, rr = h * PR + rp;
+      if (rr < rmax && cc < cmax && (ti != tj || rr <= cc)) {
+        double* t = &T[rp * LDT + cc];
+        const double pv = 2.0 * (Ym[(size_t)(i0 + rr) * ld + j0 + cc] - 2.0 * *t);
+        Pm[(size_t)(i0 + rr) * n + j0 + cc] = pv;
+        *t = pv;
+      }
+    }
+    __syncthreads();
+    // mirror: P[j0 + cc][i0 + rr] = tile[rr][cc], consecutive threads along rr
+    for (int e = tid; e < PR * TB; e += 256) {
+      const int cc = e / PR, rr = h * PR + e % PR;
+      if (rr < rmax && cc < cmax && (ti != tj || rr < cc))
+        Pm[(size_t)(j0 + cc) * n + i0 + rr] = T[(rr - h * PR) * LDT + cc];
+    }
+    __syncthreads();
+  }
 }
 
 // FP64 packed block -> FP32 zero-padded block of the SP2 X layout (rows and columns n .. ld-1 zero).
@@ -717,17 +823,19 @@ __global__ __launch_bounds__(64) void k_sp2_decide(const int* __restrict__ idx, 
 // AX / BX: the A / B operand is the final FP32 SP2 matrix X of each molecule (buffer Xa or Xb by
 // the parity of its iteration count), widened to FP64 as it is loaded (exact); CPACK: C is written
 // to the packed layout (offset coff[m], leading dimension n) instead of the operands' layout.
+// DOUT (with AX, symmetric mode): the McWeeny first half; C = X^2 is written on the upper-triangle
+// tiles only, and D = C - X (FP32, both triangles) into the SP2 buffer that does not hold X.
 template <typename T>
 __device__ inline double ld64(const T* p) {
   return (double)*p;
 }
-template <int DT, bool AX, bool BX, bool CPACK>
+template <int DT, bool AX, bool BX, bool CPACK, bool DOUT = false>
 __global__ __launch_bounds__(256, 2) void k_dgemm_sym(const int* __restrict__ idx, const int* __restrict__ norb,
                                                    const long long* __restrict__ off, const int* __restrict__ lds,
                                                    const double* __restrict__ A, const double* __restrict__ B,
                                                    double* __restrict__ C, int mode, int ntile_side,
-                                                   const float* __restrict__ Xa, const float* __restrict__ Xb,
-                                                   const double* __restrict__ st, const long long* __restrict__ coff) {
+                                                   float* Xa, float* Xb, const double* __restrict__ st,
+                                                   const long long* __restrict__ coff) {
   constexpr int WT = DT / 2;    // per-wave tile edge
   constexpr int NB = WT / 16;   // 16x16 MFMA blocks per wave side
   constexpr int NQ = DT * 16 / 256;
@@ -745,8 +853,14 @@ __global__ __launch_bounds__(256, 2) void k_dgemm_sym(const int* __restrict__ id
   if (ti * DT >= n || tj * DT >= n) return;
   const int ld = lds ? lds[m] : n;
   const size_t o = off[m];
+  static_assert(!DOUT || AX, "D = X^2 - X needs X as the A operand");
   const float* Xm = nullptr;
-  if (AX || BX) Xm = ((((long long)st[(size_t)m * S_NST + S_IT]) & 1) ? Xb : Xa) + o;
+  float* Dm = nullptr;
+  if (AX || BX) {
+    const bool odd = (((long long)st[(size_t)m * S_NST + S_IT]) & 1) != 0;
+    Xm = (odd ? Xb : Xa) + o;
+    if (DOUT) Dm = (odd ? Xa : Xb) + o;
+  }
   using TA = typename std::conditional<AX, float, double>::type;
   using TB = typename std::conditional<BX, float, double>::type;
   const TA* Am;
@@ -808,7 +922,7 @@ __global__ __launch_bounds__(256, 2) void k_dgemm_sym(const int* __restrict__ id
     for (int b = 0; b < NB; ++b) {
       // A of the McWeeny form for this 16 x 16 block, loaded before the first use
       double av[4];
-      if (mode & 1) {
+      if ((mode & 1) || DOUT) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           const int r = min(i0 + wr * WT + a * 16 + 4 * v + (lane >> 4), n - 1);
@@ -824,7 +938,13 @@ __global__ __launch_bounds__(256, 2) void k_dgemm_sym(const int* __restrict__ id
           const double x = acc[a][b][v];
           const double y = (mode & 1) ? 2.0 * (3.0 * av[v] - 2.0 * x) : x;
           Cm[(size_t)r * ldc + c] = y;
-          if (symout && ti != tj) Cm[(size_t)c * ldc + r] = y;
+          if constexpr (DOUT) {
+            const float d = (float)(y - av[v]);
+            Dm[(size_t)r * ld + c] = d;
+            if (ti != tj) Dm[(size_t)c * ld + r] = d;
+          } else if (symout && ti != tj) {
+            Cm[(size_t)c * ldc + r] = y;
+          }
         }
       }
     }
@@ -956,23 +1076,24 @@ extern "C" int pyseqm_dgemm_sym(const pyseqm_layout_t* L, const int* idx, int na
   return st_ok();
 }
 
-// McWeeny purification of the final SP2 matrix, FP64: Y2 = X X, then P = 2 (3 Y2 - 2 Y2 X) into the
-// packed layout.  X (FP32, buffer by iteration parity) is widened as it is loaded.
+// McWeeny purification of the final SP2 matrix: Y2 = X X on the FP64 MFMA GEMM (upper-triangle tiles,
+// X widened exactly as it is loaded) with D = Y2 - X into the other SP2 buffer, then
+// P = 2 (Y2 - 2 X D) (= 2 (3 X^2 - 2 X^3)) on bf16x3 MFMA into the packed layout.
 template <int DT>
 static void launch_mcweeny(const pyseqm_layout_t* L, const int* idx, int nact, int nmax, const long long* xoff,
-                           const int* xld, const double* state, const float* Xa, const float* Xb, double* Y2,
-                           double* P, hipStream_t s) {
+                           const int* xld, const double* state, float* Xa, float* Xb, double* Y2, double* P,
+                           hipStream_t s) {
   const int nt = (nmax + DT - 1) / DT;
-  const dim3 grid(nt * (nt + 1) / 2, nact);
-  k_dgemm_sym<DT, true, true, false><<<grid, 256, 0, s>>>(idx, L->mol_norb, xoff, xld, nullptr, nullptr, Y2, 2, nt,
-                                                          Xa, Xb, state, nullptr);
-  k_dgemm_sym<DT, false, true, true><<<grid, 256, 0, s>>>(idx, L->mol_norb, xoff, xld, Y2, nullptr, P, 3, nt, Xa,
-                                                          Xb, state, L->mol_mat_off);
+  const int ntri = nt * (nt + 1) / 2;
+  k_dgemm_sym<DT, true, true, false, true><<<dim3(ntri, nact), 256, 0, s>>>(
+      idx, L->mol_norb, xoff, xld, nullptr, nullptr, Y2, 2, nt, Xa, Xb, state, nullptr);
+  k_mcw_bf16x3<DT><<<8 * ((nact * ntri + 7) / 8), 256, 0, s>>>(idx, nact, ntri, L->mol_norb, xoff, xld, Xa, Xb,
+                                                                state, Y2, L->mol_mat_off, P);
 }
 
 extern "C" int pyseqm_sp2_mcweeny(const pyseqm_layout_t* L, const int* idx, int nact, int nmax,
-                                  const long long* xoff, const int* xld, const double* state, const float* Xa,
-                                  const float* Xb, double* Y2, double* P, void* stream) {
+                                  const long long* xoff, const int* xld, const double* state, float* Xa,
+                                  float* Xb, double* Y2, double* P, void* stream) {
   if (!L || nact < 0 || nmax <= 0) return PYSEQM_EINVAL;
   if (nact == 0) return PYSEQM_OK;
   hipStream_t s = (hipStream_t)stream;

@@ -2,8 +2,9 @@
 
 Replaces pack -> SP2 -> unpack of make_Pnew_factory (seqm/seqm_functions/scf_loop.py:66-78,
 SP2.py:9-85): trace-correcting SP2 with X in FP32 and X^2 on bf16x3 MFMA (FP32 accumulate),
-branch / stop decided on the device, then one FP64 McWeeny step P = 2 (3 X^2 - 2 X^3) on the
-FP64 MFMA GEMM.  The stop rule is the reference's reduced-precision one (trace error below eps and
+branch / stop decided on the device, then one McWeeny step P = 2 (3 X^2 - 2 X^3) to FP64
+accuracy: X^2 on the FP64 MFMA GEMM, X^3 = X^2 + X (X^2 - X) with the small correction product on
+bf16x3 MFMA.  The stop rule is the reference's reduced-precision one (trace error below eps and
 no longer improving).
 """
 import numpy as np
@@ -92,7 +93,8 @@ def density_sp2(eng, F, idx, nact, Pout, eps=SP2_EPS, max_iter=SP2_MAX_ITER, fix
     if ws.timing is not None:
         ev1.record()
         ws.timing.append((ev0, ev1, idx.clone(), nact))
-    # FP64 McWeeny correction: P = 2 (3 X^2 - 2 X^3)
+    # McWeeny correction P = 2 (3 X^2 - 2 X^3), as P = 2 (Y2 - 2 X D) with Y2 = X^2 in FP64 and
+    # D = Y2 - X (small) multiplied on bf16x3: FP64-accurate P
     nat.call("pyseqm_sp2_mcweeny", L, nat.ptr(idx), nact, nmax, nat.ptr(ws.xoff), nat.ptr(ws.xld), nat.ptr(ws.state),
              nat.ptr(ws.Xa), nat.ptr(ws.Xb), nat.ptr(ws.Y2), nat.ptr(Pout), s)
     ws.last_iters = ws.state[:, 7]

@@ -185,3 +185,43 @@ def test_diis_commutator_bf16x3(sizes):
         assert e_new < 1e-5 * scale and e_new < 3.0 * e_old + 1e-12, (m, e_new, e_old, scale)
         assert torch.all(got[~up] == 7.0), m
         assert abs(err[m].item() - ref[up].abs().max().item()) < 1e-5 * scale, m
+
+
+@pytest.mark.parametrize("sizes", [(10, 21, 40), (45, 60, 70)])
+def test_mcweeny_matches_fp64_closed_form(sizes):
+    """pyseqm_sp2_mcweeny (Y2 = X^2 in FP64, then P = 2 (Y2 - 2 X D), D = Y2 - X, with X D on bf16x3)
+    against the McWeeny form 2 (3 X^2 - 2 X^3) evaluated in float64 from the same final FP32 X, on
+    water clusters of 60-240 orbitals (64 x 64 tiles) and 270-420 orbitals (128 x 128 tiles), ragged
+    sizes.  With D of the size of X's idempotency error the bf16x3 product adds ~1e-11 at most: the
+    bar is 1e-9 absolute (a dropped or mis-signed term, or X in place of D, is off by >= 1e-6), P
+    exactly symmetric, and the lower-precision X D must matter (P differs from 2 Y2 by > 1e-9)."""
+    from pyseqm_amd.sp2 import density_sp2
+    mols = [W.water_cluster(nw, seed=k) for k, nw in enumerate(sizes)]
+    sp, c = W._pad(mols)
+    eng = SCFEngine(sp, c, "PM3", DEV)
+    r = eng.scf(eps=1e-5, converger=(2,), density="sp2")
+    b = eng.b
+    F = torch.empty_like(r.P)
+    eng.fock(r.P, F)
+    idx = torch.arange(b.nmol, dtype=torch.int32, device=DEV)
+    eng._idx_host = np.arange(b.nmol)
+    P = torch.full_like(F, 7.0)
+    density_sp2(eng, F, idx, b.nmol, P)
+    torch.cuda.synchronize()
+    ws = eng._sp2ws
+    its = ws.state[:, 7].cpu().numpy()
+    xoff = ws.xoff.cpu().numpy()
+    xld = ws.xld.cpu().numpy()
+    for m in range(b.nmol):
+        n, o = int(b.norb[m]), int(b.mol_mat_off[m])
+        ld, xo = int(xld[m]), int(xoff[m])
+        buf = ws.Xb if int(its[m]) & 1 else ws.Xa
+        X = buf[xo:xo + ld * ld].view(ld, ld)[:n, :n].double()
+        X2 = X @ X
+        ref = 2.0 * (3.0 * X2 - 2.0 * X2 @ X)
+        got = P[o:o + n * n].view(n, n)
+        err = (got - ref).abs().max().item()
+        assert err < 1e-9, (m, n, err)
+        assert torch.equal(got, got.T), m
+        assert (got - 2.0 * X2).abs().max().item() > 1e-9, m
+        assert abs(got.trace().item() - 2 * b.nocc[m]) < 1e-6, m
