@@ -23,6 +23,29 @@ namespace {
 
 enum { XX = 0, XH = 1, HH = 2 };
 
+// Two consecutive doubles at an 8-byte aligned address: one 16-byte access (global_load/store_dwordx4;
+// the rows of the 4 x 4 orbital blocks start at any element of a packed block)
+typedef double d2u __attribute__((ext_vector_type(2), aligned(8)));
+__device__ inline d2u ld2(const double* p) { return *reinterpret_cast<const d2u*>(p); }
+__device__ inline void st2(double* p, double x, double y) { *reinterpret_cast<d2u*>(p) = d2u{x, y}; }
+
+// The 10 packed elements (kTriI, kTriJ order: rows 0..3 of the lower triangle) of the 4 x 4 block at
+// p, row stride n, in 6 loads of row segments.
+__device__ inline void ld_tri10(const double* p, int n, double (&v)[10]) {
+  const d2u r0 = ld2(p), r1 = ld2(p + n), r2a = ld2(p + 2 * n), r2b = ld2(p + 2 * n + 2);
+  const d2u r3a = ld2(p + 3 * n), r3b = ld2(p + 3 * n + 2);
+  v[0] = r0[0];
+  v[1] = r1[0];
+  v[2] = r1[1];
+  v[3] = r2a[0];
+  v[4] = r2a[1];
+  v[5] = r2b[0];
+  v[6] = r3a[0];
+  v[7] = r3a[1];
+  v[8] = r3b[0];
+  v[9] = r3b[1];
+}
+
 // Heavy-H and H-H tiles, one wave per tile (heavy-heavy tiles: k_fock_xx2 below).
 template <int KIND>
 __global__ __launch_bounds__(64) void k_fock_tiles(pyseqm_layout_t L, int tile_base, const double* __restrict__ w,
@@ -57,11 +80,10 @@ __global__ __launch_bounds__(64) void k_fock_tiles(pyseqm_layout_t L, int tile_b
       const int S = tile_wstride(t);
       const double* wt = w + L.tile_woff[tile] + lane_rank(t, lane);
       double pa[10], pab[4];
+      ld_tri10(Pm + (size_t)ai * n + ai, n, pa);
 #pragma unroll
-      for (int k = 0; k < 10; ++k) {
-        const int mu = kTriI[k], nu = kTriJ[k];
-        pa[k] = (mu == nu ? 1.0 : 2.0) * Pm[(size_t)(ai + mu) * n + ai + nu];
-      }
+      for (int k = 0; k < 10; ++k)
+        if (kTriI[k] != kTriJ[k]) pa[k] *= 2.0;
       const double pb = Pm[(size_t)aj * n + aj];
 #pragma unroll
       for (int a = 0; a < 4; ++a) pab[a] = Pm[(size_t)(ai + a) * n + aj];
@@ -75,14 +97,16 @@ __global__ __launch_bounds__(64) void k_fock_tiles(pyseqm_layout_t L, int tile_b
         K[mu] += v * pab[nu];
         if (mu != nu) K[nu] += v * pab[mu];
       }
+      double f[4];
 #pragma unroll
       for (int a = 0; a < 4; ++a) {
         const double h = Hm[(size_t)(ai + a) * n + aj];
-        const double f = h - 0.5 * K[a];
-        Fm[(size_t)(ai + a) * n + aj] = f;
-        Fm[(size_t)aj * n + ai + a] = f;
-        e += pab[a] * (h + f);
+        f[a] = h - 0.5 * K[a];
+        Fm[(size_t)(ai + a) * n + aj] = f[a];
+        e += pab[a] * (h + f[a]);
       }
+      st2(Fm + (size_t)aj * n + ai, f[0], f[1]);
+      st2(Fm + (size_t)aj * n + ai + 2, f[2], f[3]);
     } else {
       const double v = w[L.tile_woff[tile] + lane_rank(t, lane)];
       const double pij = Pm[(size_t)ai * n + aj];
@@ -110,9 +134,10 @@ __device__ inline void xx_half(const double* __restrict__ wt, int S, const doubl
     const int k = K0 + kk;
     pa[kk] = (kTriI[k] == kTriJ[k] ? 1.0 : 2.0) * Pm[(size_t)(ai + kTriI[k]) * n + ai + kTriJ[k]];
   }
+  ld_tri10(Pm + (size_t)aj * n + aj, n, pb);
 #pragma unroll
   for (int l = 0; l < 10; ++l)
-    pb[l] = (kTriI[l] == kTriJ[l] ? 1.0 : 2.0) * Pm[(size_t)(aj + kTriI[l]) * n + aj + kTriJ[l]];
+    if (kTriI[l] != kTriJ[l]) pb[l] *= 2.0;
 #pragma unroll
   for (int kk = 0; kk < 5; ++kk) {
     const int k = K0 + kk;
@@ -170,9 +195,13 @@ __global__ __launch_bounds__(128) void k_fock_xx2(pyseqm_layout_t L, const doubl
     const int S = tile_wstride(t);
     const double* wt = w + L.tile_woff[tile] + lane_rank(t, lane);
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-      for (int b = 0; b < 4; ++b) pab[a][b] = Pm[(size_t)(ai + a) * n + aj + b];
+    for (int a = 0; a < 4; ++a) {
+      const d2u u0 = ld2(Pm + (size_t)(ai + a) * n + aj), u1 = ld2(Pm + (size_t)(ai + a) * n + aj + 2);
+      pab[a][0] = u0[0];
+      pab[a][1] = u0[1];
+      pab[a][2] = u1[0];
+      pab[a][3] = u1[1];
+    }
     if (q == 0)
       xx_half<0>(wt, S, Pm, n, ai, aj, pab, rowv, colv, K);
     else
@@ -202,17 +231,25 @@ __global__ __launch_bounds__(128) void k_fock_xx2(pyseqm_layout_t L, const doubl
   if (valid) {
     const double* Hm = H + mo;
     double* Fm = F + mo;
+    double f[4][4];
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+    for (int a = 0; a < 4; ++a) {
+      const d2u h0 = ld2(Hm + (size_t)(ai + a) * n + aj), h1 = ld2(Hm + (size_t)(ai + a) * n + aj + 2);
+      const double h[4] = {h0[0], h0[1], h1[0], h1[1]};
 #pragma unroll
       for (int b = 0; b < 4; ++b) {
         const double kt = K[a][b] + xch[10 + 4 * a + b][lane];
-        const double h = Hm[(size_t)(ai + a) * n + aj + b];
-        const double f = h - 0.5 * kt;
-        Fm[(size_t)(ai + a) * n + aj + b] = f;
-        Fm[(size_t)(aj + b) * n + ai + a] = f;
-        e += pab[a][b] * (h + f);
+        f[a][b] = h[b] - 0.5 * kt;
+        e += pab[a][b] * (h[b] + f[a][b]);
       }
+      st2(Fm + (size_t)(ai + a) * n + aj, f[a][0], f[a][1]);
+      st2(Fm + (size_t)(ai + a) * n + aj + 2, f[a][2], f[a][3]);
+    }
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      st2(Fm + (size_t)(aj + b) * n + ai, f[0][b], f[1][b]);
+      st2(Fm + (size_t)(aj + b) * n + ai + 2, f[2][b], f[3][b]);
+    }
   }
   write_partials_c(part, L, tile, lane, row10, colv);
   const double es = wave_sum(e);
