@@ -192,7 +192,7 @@ def fock_roofline(eng, P, reps):
     ms = ev0.elapsed_time(ev1) / reps
     nbytes, (nxx, nxh, nhh) = fock_algorithmic_bytes(eng.b)
     ach = nbytes / (ms * 1e-3) / 1e9
-    return {"kernel": "pyseqm_fock (k_fock_tiles<XX/XH/HH> + k_fock_diag, fused Eelec)", "bound": "hbm",
+    return {"kernel": "pyseqm_fock (k_fock_xx2 + k_fock_tiles<XH/HH> + k_fock_diag + k_mol_sum, fused Eelec)", "bound": "hbm",
             "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None,
             "algorithmic_bytes": nbytes, "launch_ms": ms, "pairs": {"XX": nxx, "XH": nxh, "HH": nhh}}
 
