@@ -43,6 +43,17 @@ class SP2Workspace:
         self.timing = None  # list of (start event, end event, idx, nact) of the SP2 GEMM loops
 
 
+def _xcd_lpt_order(eng):
+    """The active molecules by decreasing size, dealt round-robin into 8 runs (the SP2 kernels give
+    each XCD a contiguous run of the molecule-major tile list): every XCD gets an even share, and
+    within its run the longest K loops start first, so the short ones fill the tail.  Only the
+    launch order changes; every kernel indexes its data by molecule."""
+    ih = np.asarray(eng._idx_host)
+    srt = ih[np.argsort(-eng.b.norb[ih], kind="stable")]
+    perm = np.concatenate([srt[x::8] for x in range(8)]).astype(np.int32)
+    return torch.as_tensor(perm, device=eng.device)
+
+
 def density_sp2(eng, F, idx, nact, Pout, eps=SP2_EPS, max_iter=SP2_MAX_ITER, fixed_iters=None):
     """fixed_iters: launch exactly that many SP2 iterations and never read the stop counter on the
     host (molecules that stopped make the remaining launches no-ops); the launch sequence is then
@@ -52,6 +63,8 @@ def density_sp2(eng, F, idx, nact, Pout, eps=SP2_EPS, max_iter=SP2_MAX_ITER, fix
     if ws is None:
         ws = eng._sp2ws = SP2Workspace(eng)
     nmax = int(b.norb[eng._idx_host].max())
+    if fixed_iters is None and nact > 8 and not torch.cuda.is_current_stream_capturing():
+        idx = _xcd_lpt_order(eng)
     s = eng.stream
     L = b.L
     nat.call("pyseqm_sp2_init", L, nat.ptr(F), nat.ptr(idx), nact, nat.ptr(ws.xoff), nat.ptr(ws.xld),
