@@ -11,8 +11,9 @@
 //     tile and its mirror, and returns FP64 partial sums of tr(X') and ||X'||_F^2 = tr(X'^2), which
 //     is exactly the next iteration's branch test, so no second pass over X is needed;
 //   * the branch and stopping decisions are taken on the device (one thread per molecule);
-//   * a final McWeeny step X <- 3X^2 - 2X^3 in FP64 (v_mfma_f64_16x16x4_f64) restores FP64
-//     idempotency: P = 2 (3 X^2 - 2 X^3).
+//   * a final McWeeny step P = 2 (3 X^2 - 2 X^3) restores FP64 idempotency: X^2 on the FP64 MFMA
+//     GEMM (v_mfma_f64_16x16x4_f64), X^3 = X^2 + X (X^2 - X) with the small correction product on
+//     the bf16x3 tile loop.
 #include "common.h"
 
 #include <type_traits>

@@ -144,7 +144,7 @@ def density_eigh_fp64(layout: int, F: torch.Tensor, idx: torch.Tensor, nmax: int
 def density_sp2(layout: int, F: torch.Tensor, idx: torch.Tensor, nmax: int, xoff: torch.Tensor, xld: torch.Tensor,
                 Xa: torch.Tensor, Xb: torch.Tensor, state: torch.Tensor, part: torch.Tensor, Y2: torch.Tensor,
                 red: torch.Tensor, ndone: torch.Tensor, P: torch.Tensor, eps: float, iters: int) -> None:
-    """SP2 (SP2.py:9-85) in mixed precision on the listed blocks, then the FP64 McWeeny step into the packed
+    """SP2 (SP2.py:9-85) in mixed precision on the listed blocks, then the McWeeny step (FP64 accuracy) into the packed
     density P.  `iters` iterations are launched (molecules that stop earlier make the rest no-ops), so the
     call makes no host synchronisation.  Buffers as pyseqm_amd.sp2.SP2Workspace lays them out: X blocks
     (FP32, Xa / Xb) at xoff with leading dimension xld, state (nmol, 8), part (nmol, part_stride), Y2 (FP64,

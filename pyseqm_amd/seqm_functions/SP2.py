@@ -4,7 +4,7 @@
 Every D x D block is one system, zero rows included, exactly as the reference treats its input.
 X runs in FP32 with X^2 on bf16x3 MFMA (FP32 accumulation); the stop rule is the reference's
 reduced-precision one (trace error below max(eps, 1e-2) and no longer improving, SP2.py:21-28,
-74-78); the returned density gets one FP64 McWeeny step, factor * (3 X^2 - 2 X^3)."""
+74-78); the returned density gets one McWeeny step to FP64 accuracy, factor * (3 X^2 - 2 X^3)."""
 import numpy as np
 import torch
 

@@ -3,7 +3,7 @@ fock, sym_eig_trunc, SP2, elec_energy, pair_nuclear_energy) on the GPU, fed the 
 the reference's layouts and compared with the oracle's outputs.
 
 Bars: integrals / Fock / energies 1e-12 relative to the matrix scale (FP64, summation order);
-FP64 density step 1e-9; mixed-precision SP2 density 1e-5 (bf16x3 + one FP64 McWeeny step)."""
+FP64 density step 1e-9; mixed-precision SP2 density 1e-5 (bf16x3 + one McWeeny step to FP64 accuracy)."""
 import numpy as np
 import pytest
 import torch

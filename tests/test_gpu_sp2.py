@@ -1,4 +1,4 @@
-"""SP2 (bf16x3 MFMA + FP64 McWeeny) and the FP64 MFMA GEMM on the GPU.
+"""SP2 (bf16x3 MFMA + McWeeny to FP64 accuracy), the DIIS residual kernels and the FP64 MFMA GEMM on the GPU.
 
 Bars: FP64 MFMA GEMM within 1e-12 (relative to |A||B| n) of a float64 matmul; one SP2 density
 step reproduces the eigensolver's projector (trace exact, idempotent to 1e-7, energy within
