@@ -99,3 +99,27 @@ def test_serving_admission_respects_budgets():
     srv.in_use = 0
     batch, _ = SCFServer._take(srv)
     assert [r.rid for r in batch] == [2, 3]
+
+
+def test_sp2_launch_order_is_balanced_largest_first():
+    """The SP2 launch order: a permutation of the active molecules, dealt into 8 runs (one per XCD) of
+    sizes differing by at most one, each run in non-increasing orbital count, and run x holding the
+    (x + 8 r)-th largest molecules (so every run gets an even share of the work)."""
+    import types
+    from pyseqm_amd.sp2 import _xcd_lpt_order
+    rng = np.random.default_rng(3)
+    norb = rng.integers(14, 183, size=300)
+    for nact in (9, 64, 203, 300):
+        act = np.sort(rng.choice(300, size=nact, replace=False))
+        eng = types.SimpleNamespace(_idx_host=act, b=types.SimpleNamespace(norb=norb), device="cpu")
+        perm = _xcd_lpt_order(eng).numpy()
+        assert perm.dtype == np.int32 and sorted(perm.tolist()) == act.tolist()
+        srt = act[np.argsort(-norb[act], kind="stable")]
+        runs, pos = [], 0
+        for x in range(8):
+            r = perm[pos:pos + len(srt[x::8])]
+            pos += len(r)
+            assert r.tolist() == srt[x::8].tolist()
+            assert np.all(np.diff(norb[r]) <= 0)
+            runs.append(len(r))
+        assert max(runs) - min(runs) <= 1
