@@ -246,10 +246,10 @@ __global__ __launch_bounds__(256, TB == 128 ? 3 : 6) void k_sp2_bf16x3(
     for (int q = 0; q < NQ; ++q) {
       const int f = tid + 256 * q;  // float4 index in the TB x (BK/4) slab
       const int row = f / (BK / 4), kq = f % (BK / 4);
-      ra[q] = make_float4(0.f, 0.f, 0.f, 0.f);
-      rb[q] = ra[q];
-      if (i0 + row < n) ra[q] = *reinterpret_cast<const float4*>(Xm + (size_t)(i0 + row) * ld + k0 + 4 * kq);
-      if (j0 + row < n) rb[q] = *reinterpret_cast<const float4*>(Xm + (size_t)(j0 + row) * ld + k0 + 4 * kq);
+      // rows at or beyond n are clamped to row n - 1: they only feed output rows / columns >= n,
+      // which are never written (no branches around the loads)
+      ra[q] = *reinterpret_cast<const float4*>(Xm + (size_t)min(i0 + row, n - 1) * ld + k0 + 4 * kq);
+      rb[q] = *reinterpret_cast<const float4*>(Xm + (size_t)min(j0 + row, n - 1) * ld + k0 + 4 * kq);
     }
   };
   auto split_store = [&](const float4& v, __bf16 (*hi)[LDSK], __bf16 (*lo)[LDSK], int row, int col) {
@@ -382,7 +382,8 @@ __global__ __launch_bounds__(256, TB == 128 ? 3 : 6) void k_sp2_bf16x3(
 
 // One TB x TB output tile of sum_p sgn_p A_p[I] B_p[J]^T on bf16x3 MFMA with FP32 accumulation,
 // over npass (1 or 2) products: the rows i0 .. i0+TB-1 of A_p and j0 .. j0+TB-1 of B_p are FP32
-// with leading dimension ld, zero-padded to ld columns (rows at or beyond n read as zero); pass 1's
+// with leading dimension ld, zero-padded to ld columns (rows at or beyond n are read as row n - 1:
+// only output rows / columns >= n depend on them, and callers write none of those); pass 1's
 // A operand is negated (exact in the hi / lo split).  4 waves in 2 x 2, each TB/2 x TB/2 of 32 x 32
 // MFMA blocks; K slabs of 16 staged through double-buffered LDS as bf16 hi / lo (row pitch 24), the
 // next slab's global loads in flight while the current slab is multiplied.  smem holds
@@ -414,10 +415,9 @@ __device__ inline void bf16x3_tile(__bf16* smem, floatx16 (&acc)[TB / 64][TB / 6
     for (int q = 0; q < NQ; ++q) {
       const int f = tid + 256 * q;
       const int row = f / (BK / 4), kq = f % (BK / 4);
-      ra[q] = make_float4(0.f, 0.f, 0.f, 0.f);
-      rb[q] = ra[q];
-      if (i0 + row < n) ra[q] = *reinterpret_cast<const float4*>(Am + (size_t)(i0 + row) * ld + k0 + 4 * kq);
-      if (j0 + row < n) rb[q] = *reinterpret_cast<const float4*>(Bm + (size_t)(j0 + row) * ld + k0 + 4 * kq);
+      // rows at or beyond n clamped to row n - 1 (they only feed outputs that are never written)
+      ra[q] = *reinterpret_cast<const float4*>(Am + (size_t)min(i0 + row, n - 1) * ld + k0 + 4 * kq);
+      rb[q] = *reinterpret_cast<const float4*>(Bm + (size_t)min(j0 + row, n - 1) * ld + k0 + 4 * kq);
     }
   };
   auto split_store = [&](const float4& v, float sgn, __bf16 (*hi)[LDSK], __bf16 (*lo)[LDSK], int row, int col) {
