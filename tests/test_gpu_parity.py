@@ -224,3 +224,41 @@ def test_diis_solve_matches_eigh_pseudo_inverse():
             if cond <= 1e7:
                 np.testing.assert_allclose(coef[m, :cF].cpu().numpy(), cref, rtol=0,
                                            atol=1e-8 * max(1.0, np.abs(cref).max()))
+
+
+def _tile_kind_batch():
+    """H2 (H-H pairs only), HF (heavy-H only), N2 (heavy-heavy only) and H2O, padded to 3 atoms."""
+    sp = np.array([[1, 1, 0], [9, 1, 0], [7, 7, 0], [8, 1, 1]])
+    xyz = np.zeros((4, 3, 3))
+    xyz[0, :2] = [[0.0, 0.0, 0.0], [0.74, 0.02, 0.0]]
+    xyz[1, :2] = [[0.0, 0.0, 0.0], [0.0, 0.03, 0.92]]
+    xyz[2, :2] = [[0.0, 0.0, 0.0], [1.10, 0.0, 0.01]]
+    xyz[3] = [[0.0, 0.0, 0.0], [0.96, 0.0, 0.0], [-0.24, 0.93, 0.0]]
+    return sp, xyz
+
+
+@pytest.mark.parametrize("method", ["MNDO", "AM1", "PM3"])
+def test_single_tile_kind_molecules_match_oracle(method):
+    """Molecules whose pairs are all of one tile kind, in one ragged batch and alone (H2 alone and N2
+    alone are batches with no tiles at all of two of the three kinds): SCF energies within 1e-6 eV of
+    the oracle, each molecule alone equal to its batch result, and the mixed-precision density step
+    within 1e-4 eV/atom."""
+    sp, c = _tile_kind_batch()
+    eng = SCFEngine(sp, c, method, DEV)
+    r = eng.scf(eps=1e-8, converger=(2,))
+    en = eng.energies(r.P)
+    ro = R.single_point(sp, c, method, eps=1e-8, converger=(2,))
+    assert not r.notconverged.any() and not ro["notconverged"].any()
+    for k in ("Hf", "Etot", "Eelec", "Enuc"):
+        np.testing.assert_allclose(en[k].cpu().numpy(), ro[k], atol=1e-6, rtol=0)
+    hf = en["Hf"].cpu().numpy()
+    for m in (0, 2):
+        e1 = SCFEngine(sp[m:m + 1, :2], c[m:m + 1, :2], method, DEV)
+        h1 = e1.energies(e1.scf(eps=1e-8).P)["Hf"].cpu().numpy()
+        assert abs(h1[0] - hf[m]) < 1e-7, (m, h1[0], hf[m])
+    e2 = SCFEngine(sp, c, method, DEV)
+    r2 = e2.scf(eps=1e-5, converger=(2,), density="sp2")
+    assert not r2.notconverged.any()
+    h2 = e2.energies(r2.P)["Hf"].cpu().numpy()
+    nat = (sp > 0).sum(1)
+    assert np.all(np.abs(h2 - ro["Hf"]) / nat < 1e-4), (h2 - ro["Hf"]) / nat
