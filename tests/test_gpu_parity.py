@@ -49,6 +49,8 @@ def _inputs(name):
         return W.alkanes(6, seed=5, nmax=9)
     if name == "benzene4":
         return W.benzene(4, seed=2)
+    if name == "tilekinds":
+        return _tile_kind_batch()
     raise KeyError(name)
 
 
@@ -173,7 +175,7 @@ def test_forces_match_goldens(files, method, eps, gname):
     np.testing.assert_allclose(f, g, atol=1e-6, rtol=0)
 
 
-@pytest.mark.parametrize("name,method", [("sulfur", "PM3"), ("water8", "MNDO")])
+@pytest.mark.parametrize("name,method", [("sulfur", "PM3"), ("water8", "MNDO"), ("tilekinds", "AM1")])
 def test_forces_match_oracle(name, method):
     sp, c = _inputs(name)
     eng = SCFEngine(sp, c, method, DEV)
