@@ -36,11 +36,13 @@ EIGH_MIN_N_BATCH = 16
 MIXED_DIIS_MIN_N = 256
 
 
-def size_groups(norb, ratio=0.8, min_group=32, floor=EIGH_MIN_N):
+def size_groups(norb, ratio=0.7, min_group=32, floor=EIGH_MIN_N):
     """Partition molecules (by position) into groups of similar orbital counts: sorted by size,
     a group takes molecules down to `ratio` of its largest; groups smaller than `min_group` join
     the next larger one; sizes below `floor` count as `floor` (they are padded to it).  Returns a
-    list of index arrays."""
+    list of index arrays.  ratio 0.7 was the fastest on 1024 alkanes of 14-182 orbitals (3 groups,
+    374 ms per FP64 SCF against 394 at 0.8 / 5 groups, 392 at 0.6 / 2 groups, 496 with one group):
+    each batched solve costs a fixed chain of small kernels, which outweighs some padding."""
     norb = np.maximum(np.asarray(norb), floor)
     order = np.argsort(-norb, kind="stable")
     groups, start = [], 0
