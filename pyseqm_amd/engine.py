@@ -25,10 +25,11 @@ class SCFResult:
 
 # Batched FP64 eigensolves of blocks with n <= 64 take a far slower solver path than n >= 68
 # (measured on MI355X with the ROCm 7 solver: 1024 blocks of 64 take 417 ms, of 68 take 8 ms), so
-# a group of many small blocks is padded up to this size.
+# a group of many small blocks is padded up to this size (on 1024 alkanes a floor of 32 or 48 made
+# the FP64 SCF 2.3-2.9x slower, 96 or 128 3-33 % slower).
 EIGH_MIN_N = 68
-# The DIIS product F P goes to the bf16x3 GEMM (128-wide tiles) only for blocks larger than this;
-# smaller blocks stay on the FP64 GEMM with 64-wide tiles, which is faster at that size.
+# the floor applies to batched solves of at least this many blocks (a handful of small blocks is
+# solved unpadded; the slow small-block path only costs at batch scale)
 EIGH_MIN_N_BATCH = 16
 # largest block whose DIIS residual stays on the FP64 GEMM with the mixed-precision density (the
 # bf16x3 commutator is faster per iteration, but on small blocks its coarser residual costs SCF
