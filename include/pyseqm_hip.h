@@ -132,7 +132,8 @@ int pyseqm_fock(const pyseqm_layout_t* L, const double* atom_par, const double* 
  * Gershgorin range (diag.py:172-192), for a batched symmetric eigensolver. */
 int pyseqm_pad_for_eigh(const pyseqm_layout_t* L, const double* F, const int* idx, int nact, int nmax,
                         double* out, double* red, int nb, void* stream);
-/* P = 2 C_occ C_occ^T from the padded eigenvectors (columns) (diag.py:230-232). */
+/* P = 2 C_occ C_occ^T from the padded eigenvectors (columns) (diag.py:230-232).  V holds nact
+ * blocks of nmax x nmax; nmax must be at least the orbital count of every listed molecule. */
 int pyseqm_density_from_evecs(const pyseqm_layout_t* L, const double* V, int nmax, const int* idx,
                               int nact, double* P, void* stream);
 

@@ -68,12 +68,13 @@ _PARAM_DIR = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file
                           "pyseqm_amd", "params")
 
 
-def load_params(method):
-    """Per-element MOPAC parameters indexed by Z (seqm/seqm_functions/parameters.py:4-46)."""
+def load_params(method, param_dir=None):
+    """Per-element MOPAC parameters indexed by Z (seqm/seqm_functions/parameters.py:4-46), from the
+    package's parameter files or the files under `param_dir` (seqm_parameters["parameter_file_dir"])."""
     if method not in N_GAUSS:
         raise ValueError(f"unsupported method {method}")
     table = {k: np.zeros(19) for k in PARAM_NAMES}
-    with open(os.path.join(_PARAM_DIR, f"parameters_{method}.csv")) as fh:
+    with open(os.path.join(param_dir or _PARAM_DIR, f"parameters_{method}.csv")) as fh:
         hdr = fh.readline().strip().split(",")
         for line in fh:
             t = line.strip().split(",")
@@ -88,7 +89,7 @@ class Batch:
     pass
 
 
-def parse(species, coords, method, charges=None):
+def parse(species, coords, method, charges=None, param_dir=None):
     """Pair list and per-atom data, seqm/basics.py:219-403 and Molecule.py:188-206."""
     species = np.asarray(species, dtype=np.int64)
     coords = np.asarray(coords, dtype=np.float64)
@@ -136,7 +137,7 @@ def parse(species, coords, method, charges=None):
     b.pair_mol = pm
     b.species = species
     b.coords = coords
-    b.prm = load_params(method)
+    b.prm = load_params(method, param_dir)
     return b
 
 
@@ -1036,10 +1037,10 @@ def _sub(b, mols):
 
 
 def single_point(species, coords, method, eps=1e-8, converger=(2,), density="eigh", sp2_eps=1e-4,
-                 charges=None, max_iter=MAX_ITER):
+                 charges=None, max_iter=MAX_ITER, param_dir=None):
     """Energies of a batch: scf_loop (scf_loop.py:2034-2394) + Energy.forward (basics.py:813-892,
     1220-1239).  Returns a dict of numpy arrays."""
-    b = parse(species, coords, method, charges)
+    b = parse(species, coords, method, charges, param_dir)
     H, T, di, gam, mp = hcore(b)
     P, nc, it = scf(b, eps, converger, density, sp2_eps, H=H, T=T, max_iter=max_iter)
     F = fock(b, P, H, T)

@@ -39,6 +39,9 @@ class Molecule(torch.nn.Module):
         (self.nmol, self.molsize, self.nSuperHeavy, self.nHeavy, self.nHydro, self.nocc, self.Z, self.maskd,
          self.atom_molid, self.mask, self.mask_l, self.pair_molid, self.ni, self.nj, self.idxi, self.idxj,
          self.xij, self.rij) = self.parser(self, self.method, return_mask_l=True)
+        if callable(learned_parameters):
+            learned_parameters = learned_parameters(self.species, self.coordinates)
+        self.learned_parameters = dict(learned_parameters)
         self.parameters, self.alp, self.chi = self.packpar(self.Z, learned_params=dict(learned_parameters))
         self.norb = self.nHydro + 4 * self.nHeavy
         non_zero = self.species != 0
@@ -60,13 +63,32 @@ class Molecule(torch.nn.Module):
         self.active_state = 0
         self._engine = None
 
-    # device engine bound to this batch (built on first use, reused across MD steps)
+    # device engine bound to this batch (built on first use, reused across MD steps); it runs with
+    # the parameter file of seqm_parameters["parameter_file_dir"] and this molecule's learned values
     def engine(self):
         if self._engine is None:
             from .engine import SCFEngine
             self._engine = SCFEngine(self.species, None, self.method, self.coordinates.device,
-                                     charges=self.tot_charge)
+                                     charges=self.tot_charge,
+                                     param_dir=self.seqm_parameters.get("parameter_file_dir"))
+            if self.learned_parameters:
+                self._engine.set_parameters(self.parameters)
         return self._engine
+
+    def use_parameters(self, learned_parameters):
+        """Re-pack the parameters with `learned_parameters` (a dict or a callable of (species,
+        coordinates)) and load them into the engine, as Energy.forward does on every call
+        (basics.py:781-789): an empty dict means the parameter file's values."""
+        if callable(learned_parameters):
+            learned_parameters = learned_parameters(self.species, self.coordinates)
+        learned_parameters = dict(learned_parameters)
+        eng = self.engine()
+        if not learned_parameters and eng.default_params and not self.learned_parameters:
+            return
+        self.parameters, self.alp, self.chi = self.packpar(self.Z, learned_params=dict(learned_parameters))
+        self.learned_parameters = learned_parameters
+        eng.set_parameters(self.parameters)
+        eng.default_params = not learned_parameters
 
     def get_coordinates(self):
         return self.coordinates

@@ -98,7 +98,7 @@ class Pack_Parameters(torch.nn.Module):
         for k in ("zeta_d", "s_orb_exp_tail", "p_orb_exp_tail", "d_orb_exp_tail", "U_dd", "F0SD", "G2SD",
                   "rho_core"):
             out[k] = zeros
-        out["Kbeta"] = None
+        out["Kbeta"] = out.get("Kbeta", None)
         alpha = torch.zeros(int(Z.max().item()) + 1 if len(Z) else 1, int(Z.max().item()) + 1 if len(Z) else 1,
                             dtype=torch.float64, device=dev)
         return out, alpha, alpha.clone()
@@ -141,6 +141,8 @@ class Energy(torch.nn.Module):
         self.Hf_flag = seqm_parameters.get("Hf_flag", True)
 
     def forward(self, molecule, learned_parameters=dict(), all_terms=False, P0=None, *args, **kwargs):
+        # the reference re-packs the parameters with this call's learned values (basics.py:781-789)
+        molecule.use_parameters(learned_parameters)
         eng = molecule.engine()
         res = _run_scf(molecule, self.hamiltonian, P0)
         en = res["energies"]

@@ -52,11 +52,13 @@ __global__ void k_density_from_evecs(pyseqm_layout_t L, const double* __restrict
   __shared__ double Va[16][33], Vb[16][33];
   const int k = blockIdx.y;
   const int m = idx[k];
-  const int n = L.mol_norb[m];
+  // V holds nmax x nmax per listed block: reads stay inside it even if a caller breaks the
+  // nmax >= norb precondition (the host wrappers check it)
+  const int n = min(L.mol_norb[m], nmax);
   const int nt = (nmax + 15) / 16;
   const int ti = blockIdx.x / nt, tj = blockIdx.x % nt;
   if (ti * 16 >= n || tj * 16 >= n) return;
-  const int nocc = L.mol_nocc[m];
+  const int nocc = min(L.mol_nocc[m], nmax);
   const double* Vm = V + (size_t)k * nmax * nmax;
   const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
   const int r = ti * 16 + ty, c = tj * 16 + tx;

@@ -4,6 +4,7 @@ Host code only schedules kernels and reads one byte per molecule per SCF iterati
 convergence flags), like the reference's ``int(notconverged.sum())`` host sync
 (seqm/seqm_functions/scf_loop.py:1101).  Every floating-point array lives on the device.
 """
+import collections
 import math
 
 import numpy as np
@@ -66,7 +67,7 @@ def size_groups(norb, ratio=0.7, min_group=32, floor=EIGH_MIN_N):
 class SCFEngine:
     """Pair integrals, Fock builds, density steps and the SCF loop for one packed batch."""
 
-    def __init__(self, species, coordinates, method, device="cuda", charges=None, batch=None):
+    def __init__(self, species, coordinates, method, device="cuda", charges=None, batch=None, param_dir=None):
         self.device = nat.require_device(device)
         nat.lib()
         self.method = method
@@ -74,7 +75,8 @@ class SCFEngine:
         b = self.b
         dev = self.device
         f64 = dict(dtype=torch.float64, device=dev)
-        tab = param_table(method)
+        tab = param_table(method, param_dir)
+        self.default_params = True  # atom_par holds the parameter file's values
         par = np.zeros((b.natoms, nat.NPAR))
         par[:, :len(COLUMNS)] = tab[b.Z]
         self.atom_par = torch.as_tensor(par, **f64)
@@ -102,26 +104,40 @@ class SCFEngine:
                                      **f64)
         self.coords = None
         self.trace = None  # set to a list to collect per-iteration (dE, diis, dm, dm_max) metrics
+        self.eigh_log = collections.deque(maxlen=256)  # (blocks, padded size) of the recent batched eigensolves
         if coordinates is not None:
             self.set_coordinates(coordinates)
 
+    # per-atom parameters the sp methods zero or derive (Molecule.py:101-113, basics.py:791-811): a
+    # value passed for them does not change the reference's result either
+    DERIVED_PARAMS = ("beta", "zeta_d", "s_orb_exp_tail", "p_orb_exp_tail", "d_orb_exp_tail", "U_dd", "F0SD",
+                      "G2SD", "rho_core")
+
     def set_parameters(self, params):
-        """Override per-atom MOPAC parameters ({name: (natoms,) tensor}, names of the parameter
-        files, e.g. learned parameters of Pack_Parameters, basics.py:406-448) and re-derive the
-        multipoles; the isolated-atom energies follow the new U/G values."""
-        from .seqm_functions.parameters import COLUMNS
+        """Override per-atom MOPAC parameters ({name: (natoms,) tensor} in the batch's atom order, the
+        dict Pack_Parameters returns with learned values merged in, basics.py:406-448, 781-789) and
+        re-derive the multipoles, isolated-atom energies and, if coordinates are set, the integrals.
+        Names the reference ignores for MNDO/AM1/PM3 are ignored; a per-pair ``Kbeta`` scaling of the
+        resonance integrals (hcore.py:166-171) is not on this path and raises."""
         b = self.b
+        kb = params.get("Kbeta", None)
+        if torch.is_tensor(kb):
+            raise NotImplementedError("per-pair Kbeta resonance scaling is not supported by the device engine")
         for name, v in params.items():
-            if name in COLUMNS and v is not None:
-                self.atom_par[:, COLUMNS.index(name)] = torch.as_tensor(v, dtype=torch.float64,
-                                                                        device=self.device).reshape(-1)[:b.natoms]
+            if name not in COLUMNS or v is None:
+                continue
+            v = torch.as_tensor(v, dtype=torch.float64, device=self.device).detach().reshape(-1)
+            if v.numel() != b.natoms:
+                raise ValueError(f"parameter {name!r}: {v.numel()} values for {b.natoms} atoms")
+            self.atom_par[:, COLUMNS.index(name)] = v
+        self.default_params = False
         nat.call("pyseqm_atom_setup", b.L, nat.ptr(self.atom_par), self.stream)
         Z = torch.as_tensor(b.Z, device=self.device).long()
         ap = self.atom_par
-        eiso = (ap[:, 0] * torch.as_tensor(USSC, device=self.device)[Z] + ap[:, 1] * torch.as_tensor(UPPC, device=self.device)[Z]
-                + ap[:, 6] * torch.as_tensor(GSSC, device=self.device)[Z] + ap[:, 8] * torch.as_tensor(GPPC, device=self.device)[Z]
-                + ap[:, 7] * torch.as_tensor(GSPC, device=self.device)[Z] + ap[:, 9] * torch.as_tensor(GP2C, device=self.device)[Z]
-                + ap[:, 10] * torch.as_tensor(HSPC, device=self.device)[Z])
+        tab = lambda t: torch.as_tensor(t, dtype=torch.float64, device=self.device)[Z]
+        eiso = (ap[:, 0] * tab(USSC) + ap[:, 1] * tab(UPPC) + ap[:, 6] * tab(GSSC) + ap[:, 8] * tab(GPPC)
+                + ap[:, 7] * tab(GSPC) + ap[:, 9] * tab(GP2C) + ap[:, 10] * tab(HSPC))
+        self.eiso_atom = eiso.cpu().numpy()
         self.eiso = torch.zeros(b.nmol, dtype=torch.float64, device=self.device).index_add_(
             0, torch.as_tensor(b.atom_mol, device=self.device).long(), eiso)
         if self.coords is not None:
@@ -188,6 +204,7 @@ class SCFEngine:
         Fp = torch.empty(nact, nmax, nmax, dtype=torch.float64, device=self.device)
         nat.call("pyseqm_pad_for_eigh", b.L, nat.ptr(F), nat.ptr(idx), nact, nmax, nat.ptr(Fp), nat.ptr(self.red),
                  nat.red_blocks(nmax), self.stream)
+        self.eigh_log.append((nact, nmax))
         e, V = torch.linalg.eigh(Fp)
         V = V.contiguous()
         nat.call("pyseqm_density_from_evecs", b.L, nat.ptr(V), nmax, nat.ptr(idx), nact, nat.ptr(Pout),
@@ -216,8 +233,9 @@ class SCFEngine:
         self._idx_host = idx
         return torch.as_tensor(idx.astype(np.int32), device=self.device), len(idx)
 
-    def scf(self, eps=1e-8, converger=(2,), density="eigh", sp2_eps=1e-4, P=None, max_iter=MAX_ITER):
-        """scf_loop (scf_loop.py:2034-2394) with mixers [0] linear, [1] adaptive, [2] Pulay DIIS."""
+    def scf(self, eps=1e-8, converger=(2,), density="eigh", sp2_eps=1e-4, P=None, max_iter=MAX_ITER, diis_tol=None):
+        """scf_loop (scf_loop.py:2034-2394) with mixers [0] linear, [1] adaptive, [2] Pulay DIIS.
+        `diis_tol` overrides the max|FP - PF| convergence test (default below)."""
         b = self.b
         dev = self.device
         self.density_kind = density
@@ -225,6 +243,8 @@ class SCFEngine:
         # max|FP - PF| test: 50 eps as the reference (scf_loop.py:52); the bf16x3 SP2 density has a
         # commutator floor of ~1e-3 (its energy error stays ~1e-8 eV/atom), so it is tested at 1e-2
         self.diis_tol = 50.0 * eps if density == "eigh" else max(50.0 * eps, SP2_DIIS_TOL)
+        if diis_tol is not None:
+            self.diis_tol = float(diis_tol)
         f64 = dict(dtype=torch.float64, device=dev)
         P = self.initial_density() if P is None else P.clone()
         self._metrics = torch.zeros(b.nmol, 4, **f64)

@@ -43,6 +43,18 @@ def _p(t):
     return nat.ptr(t)
 
 
+def _check_nmax(layout, idx, nmax, V=None):
+    """Every listed block must fit the padded nmax x nmax stack (V: nact blocks of nmax^2)."""
+    b = _LAYOUTS[layout] if layout in _LAYOUTS else None
+    if b is None:
+        raise ValueError(f"unknown pyseqm layout handle {layout}")
+    ids = idx.detach().cpu().numpy()
+    if len(ids) and int(b.norb[ids].max()) > nmax:
+        raise ValueError(f"nmax = {nmax} is below the largest listed block ({int(b.norb[ids].max())} orbitals)")
+    if V is not None and V.numel() < len(ids) * nmax * nmax:
+        raise ValueError(f"V holds {V.numel()} values, fewer than {len(ids)} blocks of {nmax} x {nmax}")
+
+
 @torch.library.custom_op("pyseqm::atom_setup", mutates_args=("atom_par",))
 def atom_setup(layout: int, atom_par: torch.Tensor) -> None:
     """Derived multipole parameters per atom (two_elec_two_center_int.py:117-147, cal_par.py)."""
@@ -72,6 +84,7 @@ def fock_build(layout: int, atom_par: torch.Tensor, w_xx: torch.Tensor, w_xh: to
 def pad_for_eigh(layout: int, F: torch.Tensor, idx: torch.Tensor, nmax: int, out: torch.Tensor,
                  red: torch.Tensor) -> None:
     """Gershgorin-shifted padded stack of the listed Fock blocks (diag.py:130-160)."""
+    _check_nmax(layout, idx, nmax, out)
     nat.call("pyseqm_pad_for_eigh", _L(layout), _p(F), _p(idx), int(idx.numel()), nmax, _p(out), _p(red),
              nat.red_blocks(nmax), _s())
 
@@ -79,6 +92,7 @@ def pad_for_eigh(layout: int, F: torch.Tensor, idx: torch.Tensor, nmax: int, out
 @torch.library.custom_op("pyseqm::density_from_evecs", mutates_args=("P",))
 def density_from_evecs(layout: int, V: torch.Tensor, nmax: int, idx: torch.Tensor, P: torch.Tensor) -> None:
     """P = 2 C_occ C_occ^T for the listed molecules (diag.py:230-232)."""
+    _check_nmax(layout, idx, nmax, V)
     nat.call("pyseqm_density_from_evecs", _L(layout), _p(V), nmax, _p(idx), int(idx.numel()), _p(P), _s())
 
 
@@ -131,6 +145,7 @@ def density_eigh_fp64(layout: int, F: torch.Tensor, idx: torch.Tensor, nmax: int
     FP64 eigensolve, P = 2 C_occ C_occ^T into the packed density.  Returns the padded eigenvalues
     (nact, nmax) and eigenvectors (nact, nmax, nmax, columns)."""
     nact = int(idx.numel())
+    _check_nmax(layout, idx, nmax)
     Fp = torch.empty(nact, nmax, nmax, dtype=torch.float64, device=F.device)
     nat.call("pyseqm_pad_for_eigh", _L(layout), _p(F), _p(idx), nact, nmax, _p(Fp), _p(red), nat.red_blocks(nmax),
              _s())

@@ -32,10 +32,26 @@ def orbital_charges(molecule, v):
     return charge
 
 
+def _check_pair_cutoff(molecule):
+    """The device engine couples every atom pair of a molecule; a pair_outer_cutoff that would drop
+    pairs (the reference's Parser keeps only r < cutoff, basics.py:313-328) is refused."""
+    cut = float(molecule.seqm_parameters.get("pair_outer_cutoff", 1.0e10))
+    if cut >= 1.0e9:
+        return
+    x = molecule.coordinates.detach()
+    real = molecule.species > 0
+    d = torch.cdist(x, x)
+    d = torch.where(real[:, :, None] & real[:, None, :], d, torch.zeros_like(d))
+    if bool((d >= cut).any()):
+        raise NotImplementedError(f"pair_outer_cutoff = {cut} A drops atom pairs; the device engine uses every "
+                                  "pair of a molecule (leave the cutoff unset or above the largest distance)")
+
+
 def scf_loop(molecule, eps=1.0e-4, P=None, sp2=[False], scf_converger=[1], eig=False, scf_backward=0,
              scf_backward_eps=1.0e-2):
     if scf_backward not in (0,):
         raise NotImplementedError("only scf_backward = 0 (Hellmann-Feynman forces) is on this path")
+    _check_pair_cutoff(molecule)
     eng = molecule.engine()
     eng.set_coordinates(molecule.coordinates.detach())
     b = eng.b

@@ -120,3 +120,19 @@ def gemm_flops(eng, idx, iters):
     n = torch.as_tensor(eng.b.norb, device=iters.device, dtype=torch.float64)
     il = idx.long()
     return float((2.0 * n[il] ** 3 * iters[il]).sum())
+
+
+SMALL_TILE_MAX_N = 256  # largest active block for which the SP2 kernels use 64 x 64 output tiles
+
+
+def executed_mfma_flops(eng, idx, iters):
+    """FLOPs the SP2 kernels issue on the MFMA pipe: per molecule and iteration, the upper-triangle
+    output tiles (edge TB = 64 while the largest active block has <= 256 orbitals, else 128) that
+    hold rows of the block, each a K loop over the padded leading dimension, three bf16 products."""
+    il = idx.long().cpu().numpy()
+    n = eng.b.norb[il].astype(np.float64)
+    tb = 64 if n.max() <= SMALL_TILE_MAX_N else 128
+    nt = np.ceil(n / tb)
+    ld = np.ceil(n / 32) * 32
+    it = iters.double().cpu().numpy()[il]
+    return float((3.0 * 2.0 * tb * tb * ld * nt * (nt + 1) / 2 * it).sum())

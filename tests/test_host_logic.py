@@ -22,6 +22,37 @@ def test_size_groups_partition_and_bound_padding():
     assert size_groups(np.array([5])) [0].tolist() == [0]
 
 
+def test_size_groups_defaults_on_the_alkane_batch():
+    """The default grouping on the 1024-alkane batch (14-182 orbitals): every molecule once, at least
+    two groups, each group spans down to 0.7 of its largest (sizes below the solver floor count as the
+    floor), and the padded O(n^3) work is below padding every block to the largest."""
+    from pyseqm_amd import workloads as W
+    from pyseqm_amd.engine import EIGH_MIN_N
+    sp, _ = W.alkanes(1024, seed=0)
+    norb = 4 * (sp > 1).sum(1) + (sp == 1).sum(1)
+    g = size_groups(norb)
+    np.testing.assert_array_equal(np.sort(np.concatenate(g)), np.arange(len(norb)))
+    assert len(g) >= 2
+    eff = np.maximum(norb, EIGH_MIN_N)
+    for sel in g[:-1]:
+        assert eff[sel].min() >= 0.7 * eff[sel].max()
+    work = sum(len(s) * eff[s].max() ** 3 for s in g)
+    assert work < 0.6 * len(norb) * norb.max() ** 3
+
+
+def test_finite_pair_cutoff_that_drops_pairs_is_refused():
+    import torch
+    import pyseqm_amd
+    from pyseqm_amd.seqm_functions.scf_loop import scf_loop
+    from tests.conftest import read_xyz
+    sp, c = read_xyz(["benzene.xyz"])
+    p = {"method": "AM1", "scf_eps": 1.0e-6, "scf_converger": [2], "pair_outer_cutoff": 3.0}
+    mol = pyseqm_amd.Molecule(pyseqm_amd.Constants(), p, torch.as_tensor(c), torch.as_tensor(sp))
+    assert len(mol.idxi) < 66  # the parser drops the pairs beyond 3 A, as the reference's does
+    with pytest.raises(NotImplementedError, match="pair_outer_cutoff"):
+        scf_loop(mol, eps=1e-6)
+
+
 @pytest.mark.parametrize("k", [3, 4, 5, 6, 7, 8, 9])
 def test_xl_regrouped_coefficients_equal_raw_recursion(k):
     """The regrouped coefficients used on the device (coeff_D, coeff[cindx:cindx+m] against the
