@@ -104,6 +104,7 @@ class SCFEngine:
                                      **f64)
         self.coords = None
         self.trace = None  # set to a list to collect per-iteration (dE, diis, dm, dm_max) metrics
+        self.diis_log = None  # set to a list to collect (cF, slot, B matrices, coefficients, resets, errors)
         self.eigh_log = collections.deque(maxlen=256)  # (blocks, padded size) of the recent batched eigensolves
         if coordinates is not None:
             self.set_coordinates(coordinates)
@@ -314,7 +315,9 @@ class SCFEngine:
         X = torch.empty(total, **f64)
         diis_err = torch.full((b.nmol,), torch.finfo(torch.float64).max, **f64)
         emat0 = torch.tril(torch.eye(NFOCK + 1, **f64) - 1.0)
-        EMAT = emat0.expand(b.nmol, -1, -1).contiguous()
+        # clone, not contiguous(): for one molecule the expanded view already counts as contiguous and
+        # would alias emat0, so the reset below would copy the updated matrix onto itself
+        EMAT = emat0.expand(b.nmol, -1, -1).clone()
         coef = torch.zeros(b.nmol, NFOCK, **f64)
         reset_flags = torch.zeros(b.nmol, dtype=torch.uint8, device=dev)
         nc_host = np.ones(b.nmol, dtype=bool)
@@ -351,6 +354,9 @@ class SCFEngine:
                 nat.call("pyseqm_diis_solve", b.L, nat.ptr(idx), nact, NFOCK, cF, counter, nat.ptr(EMAT),
                          nat.ptr(coef), nat.ptr(reset_flags), self.stream)
                 reset_t = reset_flags.any()  # read with the convergence flags (one host sync)
+                if self.diis_log is not None:
+                    self.diis_log.append((cF, counter, EMAT.cpu().numpy().copy(), coef.cpu().numpy().copy(),
+                                          reset_flags.cpu().numpy().copy(), diis_err.cpu().numpy().copy()))
                 nat.call("pyseqm_diis_combine", b.L, nat.ptr(idx), nact, nmax, nat.ptr(Fring), total, cF,
                          nat.ptr(coef), NFOCK, nat.ptr(F), self.stream)
             self.density(F, idx, nact, Pnew)

@@ -4,8 +4,8 @@ made by tests/golden/make_reference_vectors.py from lanl/PYSEQM on the CPU).
 Bars: FP64 density (diagonalisation) Hf / Etot / Eelec / Enuc within 1e-6 eV of the reference and the
 same SCF iteration count within one; the mixed-precision SP2 density within 1e-4 eV/atom.  The 128
 alkanes are the first 128 of the 1024-alkane benchmark batch: the test also asserts that this run
-takes the benchmark's density path (more than one size group, blocks padded up to the batched
-solver's floor) and that it holds atom pairs beyond the 40-bohr overlap cutoff (hcore.py:83).
+takes the benchmark's density path (more than one size group of batched eigensolves) and that it
+holds atom pairs beyond the 40-bohr overlap cutoff (hcore.py:83).
 """
 import numpy as np
 import pytest
@@ -41,7 +41,7 @@ def test_fp64_engine_matches_reference(name):
 
 
 def test_c2_slice_takes_the_benchmark_path():
-    """First 128 of the 1024 alkanes: size-grouped, floor-padded eigensolves and >40-bohr pairs."""
+    """First 128 of the 1024 alkanes: size-grouped batched eigensolves and >40-bohr pairs."""
     c = CASES["am1_alkanes_c2_first128"]
     sp, x = case_input(c)
     # atom pairs beyond the overlap cutoff (their overlap, hence beta term, is zero)
@@ -54,9 +54,9 @@ def test_c2_slice_takes_the_benchmark_path():
     eng, r, en, _, _ = _run(c)
     groups = E.size_groups(eng.b.norb)
     assert len(groups) >= 2
-    sizes = {nmax for _, nmax in eng.eigh_log}
-    assert E.EIGH_MIN_N in sizes, sizes  # the smallest group is padded up to the solver floor
-    assert any(nb >= E.EIGH_MIN_N_BATCH for nb, _ in eng.eigh_log)
+    first = list(eng.eigh_log)[:len(groups)]  # the first SCF iteration solves every group
+    assert sorted(n for _, n in first) == sorted(int(eng.b.norb[g].max()) for g in groups), first
+    assert sum(nb for nb, _ in first) == eng.b.nmol
     np.testing.assert_allclose(en["Hf"], c["Hf"], rtol=0, atol=1e-6)
 
 
