@@ -27,6 +27,7 @@ extern "C" {
 #define PYSEQM_EINVAL (-1)      /* bad size / argument */
 #define PYSEQM_ELAUNCH (-2)     /* kernel launch failed */
 #define PYSEQM_EUNSUPPORTED (-3) /* element pair or method outside MNDO/AM1/PM3 sp scope */
+#define PYSEQM_ETOOBIG (-4)     /* batch larger than a fixed-size library scratch buffer */
 
 #define PYSEQM_NPAR 32 /* doubles per atom in the atom-parameter array (see below) */
 #define PYSEQM_MAX_RED_BLOCKS 256 /* max blocks per molecule of the two-stage reductions */
@@ -120,6 +121,22 @@ int pyseqm_fock(const pyseqm_layout_t* L, const double* atom_par, const double* 
                 const double* w_xh, const double* w_hh, const double* P, const double* H,
                 double* F, double* part, double* etile, double* eatom, double* eelec,
                 const unsigned char* active, void* stream);
+
+/* Fock build and Eelec that allocate nothing and take no scratch argument: the same kernels as
+ * pyseqm_fock, with part / etile / eatom in a library-owned device array of
+ * PYSEQM_FOCK_SCRATCH_DOUBLES doubles (48 MiB; one per GPU, held until the process ends, its
+ * address looked up once per device on the first call with that device current).  No
+ * counterpart in the reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  Returns
+ * PYSEQM_ETOOBIG, before any HIP call, when pyseqm_fock_scratch_doubles(L) exceeds the array
+ * (48 MiB holds a 3072-atom / 6144-orbital water cluster).  Calls share the array, so calls that
+ * overlap on one GPU (concurrent streams) race; the scratch keeps the last call's contents and
+ * needs no clearing (every slot a build reads it writes first). */
+#define PYSEQM_FOCK_SCRATCH_DOUBLES (6LL * 1024 * 1024)
+long long pyseqm_fock_scratch_doubles(const pyseqm_layout_t* L);
+int pyseqm_fock_noalloc(const pyseqm_layout_t* L, const double* atom_par, const double* w_xx,
+                        const double* w_xh, const double* w_hh, const double* P, const double* H,
+                        double* F, double* eelec, const unsigned char* active, void* stream);
 
 /* ------------------------------------------------------------------ SCF loop plumbing
  * The functions below work on a list of active molecules idx[0..nact) (device int array): the

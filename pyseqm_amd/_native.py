@@ -36,6 +36,8 @@ _SIGS = {
     "pyseqm_pair_forces": [c_int, c_void_p, c_void_p, c_void_p, c_double, c_void_p, c_void_p, c_void_p],
     "pyseqm_fock": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                     c_void_p, c_void_p, c_void_p, c_void_p],
+    "pyseqm_fock_noalloc": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                            c_void_p, c_void_p],
     "pyseqm_pad_for_eigh": [c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_int, c_void_p],
     "pyseqm_density_from_evecs": [c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p],
     "pyseqm_mix": [c_void_p, c_int, c_int, c_double, c_void_p, c_void_p, c_void_p, c_void_p],
@@ -99,12 +101,14 @@ def lib():
             f = getattr(L, name)
             f.argtypes = args
             f.restype = c_int
+        L.pyseqm_fock_scratch_doubles.argtypes = [_P]
+        L.pyseqm_fock_scratch_doubles.restype = c_longlong
         _lib = L
     return _lib
 
 
 def exported_symbols():
-    return list(_SIGS) + list(_RAW_SIGS)
+    return list(_SIGS) + list(_RAW_SIGS) + ["pyseqm_fock_scratch_doubles"]
 
 
 def stream_ptr(device=None):
@@ -118,7 +122,9 @@ def ptr(t):
     return ctypes.c_void_p(t.data_ptr())
 
 
-_CODES = {-1: "invalid argument", -2: "kernel launch failed", -3: "unsupported element/method"}
+_CODES = {-1: "invalid argument", -2: "kernel launch failed", -3: "unsupported element/method",
+          -4: "batch exceeds the library's fixed scratch buffer (PYSEQM_FOCK_SCRATCH_DOUBLES)"}
+FOCK_SCRATCH_DOUBLES = 6 * 1024 * 1024  # PYSEQM_FOCK_SCRATCH_DOUBLES of include/pyseqm_hip.h
 
 
 def call(name, layout, *args):

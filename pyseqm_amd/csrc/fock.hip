@@ -12,6 +12,8 @@
 // w is the compact, tile-major integral storage written by pairs.hip.
 #include "common.h"
 
+#include <mutex>
+
 using namespace pyseqm;
 
 namespace pyseqm {
@@ -328,12 +330,11 @@ __global__ __launch_bounds__(256) void k_fock_diag(pyseqm_layout_t L, const doub
 
 }  // namespace
 
-extern "C" int pyseqm_fock(const pyseqm_layout_t* L, const double* atom_par, const double* w_xx,
-                           const double* w_xh, const double* w_hh, const double* P, const double* H, double* F,
-                           double* part, double* etile, double* eatom, double* eelec,
-                           const unsigned char* active, void* stream) {
-  if (!L) return PYSEQM_EINVAL;
-  hipStream_t s = (hipStream_t)stream;
+namespace {
+
+int launch_fock(const pyseqm_layout_t* L, const double* atom_par, const double* w_xx, const double* w_xh,
+                const double* w_hh, const double* P, const double* H, double* F, double* part, double* etile,
+                double* eatom, double* eelec, const unsigned char* active, hipStream_t s) {
   int base = 0;
   if (L->ntile[0]) k_fock_xx2<<<L->ntile[0], 128, 0, s>>>(*L, w_xx, P, H, F, part, etile, active);
   base += L->ntile[0];
@@ -348,4 +349,56 @@ extern "C" int pyseqm_fock(const pyseqm_layout_t* L, const double* atom_par, con
   }
   pyseqm::launch_mol_sum(L, etile, eatom, active, eelec, s);
   return hipGetLastError() == hipSuccess ? PYSEQM_OK : PYSEQM_ELAUNCH;
+}
+
+// Scratch of pyseqm_fock_noalloc: the tile partials, per-tile and per-atom energies of one Fock
+// build, in a file-scope device array (one copy per GPU, held until the process ends).  Host code
+// never passes the array's name to a kernel (on the host it is a stand-in, not a device address):
+// it looks up each device's address once with hipGetSymbolAddress and passes that.
+static __device__ __align__(256) double scratch[PYSEQM_FOCK_SCRATCH_DOUBLES];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+}  // namespace
+
+extern "C" int pyseqm_fock(const pyseqm_layout_t* L, const double* atom_par, const double* w_xx,
+                           const double* w_xh, const double* w_hh, const double* P, const double* H, double* F,
+                           double* part, double* etile, double* eatom, double* eelec,
+                           const unsigned char* active, void* stream) {
+  if (!L) return PYSEQM_EINVAL;
+  return launch_fock(L, atom_par, w_xx, w_xh, w_hh, P, H, F, part, etile, eatom, eelec, active,
+                     (hipStream_t)stream);
+}
+
+extern "C" long long pyseqm_fock_scratch_doubles(const pyseqm_layout_t* L) {
+  if (!L) return -1;
+  return 160LL * L->ntile[0] + 88LL * L->ntile[1] + 16LL * L->ntile[2] +
+         ((long long)L->ntile[0] + L->ntile[1] + L->ntile[2]) + L->natoms;
+}
+
+extern "C" int pyseqm_fock_noalloc(const pyseqm_layout_t* L, const double* atom_par, const double* w_xx,
+                                   const double* w_xh, const double* w_hh, const double* P, const double* H,
+                                   double* F, double* eelec, const unsigned char* active, void* stream) {
+  if (!L) return PYSEQM_EINVAL;
+  const long long need = pyseqm_fock_scratch_doubles(L);
+  if (need > PYSEQM_FOCK_SCRATCH_DOUBLES) return PYSEQM_ETOOBIG;  // checked before any HIP call
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return PYSEQM_ELAUNCH;
+  double* base;
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+      void* p = nullptr;
+      if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return PYSEQM_ELAUNCH;
+      g_scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    base = g_scratch_ptr[dev];
+  }
+  double* part = base;
+  double* etile = part + (160LL * L->ntile[0] + 88LL * L->ntile[1] + 16LL * L->ntile[2]);
+  double* eatom = etile + ((long long)L->ntile[0] + L->ntile[1] + L->ntile[2]);
+  return launch_fock(L, atom_par, w_xx, w_xh, w_hh, P, H, F, part, etile, eatom, eelec, active,
+                     (hipStream_t)stream);
 }
