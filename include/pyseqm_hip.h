@@ -4,7 +4,8 @@
  * Every entry point takes plain device pointers, sizes and a hipStream_t (passed as void*),
  * enqueues its kernels on that stream and returns 0 on success or a negative PYSEQM_E* code.
  * No entry point synchronises the stream or allocates memory: every buffer, including
- * scratch, is owned by the caller.  Entry points are reentrant (no global state).
+ * scratch, is owned by the caller, and entry points are reentrant (no global state), except
+ * pyseqm_fock_noalloc, whose scratch is a library-owned device array (see its declaration).
  *
  * Batch layout ("packed", padding-free): molecule m owns the dense row-major norb_m x norb_m
  * block starting at mat_off[m] of every matrix buffer (P, F, H, ...).  Its atoms are stored
@@ -157,6 +158,15 @@ int pyseqm_density_from_evecs(const pyseqm_layout_t* L, const double* V, int nma
 /* Density mixing: Pold = P; P = alpha P + (1 - alpha) Pnew (scf_loop.py:266-267, 1052-1056). */
 int pyseqm_mix(const pyseqm_layout_t* L, const int* idx, int nact, int nmax, double alpha, double* P,
                const double* Pnew, double* Pold, void* stream);
+/* Adaptive mixing, scf_converger [1] (adaptive_mix / compute_fac, scf_loop.py:350-420, as
+ * scf_forward1 applies it at iteration k >= 1, scf_loop.py:516-548): Pold = P; off the diagonal
+ * P = (1 + FAC) Pnew - FAC P every third iteration (else Pnew); the diagonal damped, clamped to
+ * [0, 2] and renormalised to tr Pnew in the reference's batch-wide passes; old2 (the diagonal of P
+ * two steps back) updated.  old2 holds one value per orbital, molecule m's from orb_off[m];
+ * fac [nmol] and mask [1] are scratch.  nmax <= 8192.  No host synchronisation. */
+int pyseqm_adaptive_mix(const pyseqm_layout_t* L, const int* idx, int nact, int nmax, int k, double* P,
+                        const double* Pnew, double* Pold, double* old2, const long long* orb_off, double* fac,
+                        unsigned* mask, void* stream);
 /* dst = src on the upper triangle (c >= r) of the active blocks: the DIIS Fock ring keeps only
  * the upper triangles current (the matrices are symmetric; pyseqm_diis_combine reads only them). */
 int pyseqm_copy_active(const pyseqm_layout_t* L, const int* idx, int nact, int nmax, const double* src,

@@ -42,6 +42,8 @@ _SIGS = {
     "pyseqm_density_from_evecs": [c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p],
     "pyseqm_mix": [c_void_p, c_int, c_int, c_double, c_void_p, c_void_p, c_void_p, c_void_p],
     "pyseqm_copy_active": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p],
+    "pyseqm_adaptive_mix": [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                            c_void_p, c_void_p, c_void_p],
     "pyseqm_scf_check": [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_double,
                          c_void_p, c_void_p, c_void_p, c_int, c_void_p],
     "pyseqm_gemm": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p],

@@ -91,6 +91,147 @@ __global__ void k_mix(pyseqm_layout_t L, const int* __restrict__ idx, double a, 
   }
 }
 
+// ------------------------------------------------------------------ adaptive mixing
+// scf_converger [1] (adaptive_mix / compute_fac, scf_loop.py:350-420) on the active molecules:
+//   every third iteration P = (1 + FAC) Pnew - FAC P off the diagonal, FAC = sqrt(|dD|^2 / |d2D|^2)
+//   from the diagonals (D = diag P, d2D = Dnew - 2 D + D_old2; FAC = 0 unless d2D != 0 and
+//   |dD|^2 < 100 |d2D|^2), else P = Pnew; the diagonal steps by at most DAMP = 0.05 after
+//   iteration 4 (else by Dnew - D + FAC (Dnew - D)), clamped to [0, 2], then renormalised to the
+//   trace of Dnew: up to 20 passes that scale every active diagonal by SUM0 / SUM2, pin values
+//   above 2 at 2 and move them out of SUM0.  The reference stops the passes when every active
+//   molecule is renormalised (|SUM0 / SUM2 - 1| <= 1e-5, or SUM2 <= 1e-3), so a molecule keeps
+//   being scaled while any other one is not done.  Each molecule's sequence of passes depends on
+//   its own diagonal only: k_amix_passes runs the 20 passes of every molecule and ANDs into `mask`
+//   the passes at which it is done; the batch stops at the lowest pass all molecules agree on, and
+//   k_amix_diag applies exactly that many passes (the same arithmetic, so the same values).
+constexpr int kMixPasses = 20;
+constexpr unsigned kMixAllPasses = (1u << kMixPasses) - 1u;
+
+// Diagonal after damping / clamping (left in di[0..n)) and FAC, SUM0 of one molecule.
+__device__ inline void amix_start(const double* __restrict__ Pprev, const double* __restrict__ Pcur,
+                                  const double* __restrict__ old2, int n, int k, double* di, double& fac,
+                                  double& sum0, double* sh) {
+  const bool third = k % 3 == 0;
+  const double damp = k > 4 ? 0.05 : 1.0e10;
+  fac = 0.0;
+  if (third) {
+    double num = 0.0, den = 0.0;
+    for (int r = threadIdx.x; r < n; r += blockDim.x) {
+      const double dp = Pprev[(size_t)r * (n + 1)], dc = Pcur[(size_t)r * (n + 1)];
+      const double d1 = dc - dp, d2 = dc - 2.0 * dp + old2[r];
+      num += d1 * d1;
+      den += d2 * d2;
+    }
+    num = block_sum(num, sh);
+    den = block_sum(den, sh);
+    if (threadIdx.x == 0) sh[8] = (den > 0.0 && num < 100.0 * den) ? sqrt(num / den) : 0.0;
+    __syncthreads();
+    fac = sh[8];
+    __syncthreads();
+  }
+  double s0 = 0.0;
+  for (int r = threadIdx.x; r < n; r += blockDim.x) {
+    const double dp = Pprev[(size_t)r * (n + 1)], dc = Pcur[(size_t)r * (n + 1)];
+    const double delta = dc - dp;
+    double v = fabs(delta) > damp ? dp + (delta > 0.0 ? damp : -damp) : dc + fac * delta;
+    di[r] = fmin(fmax(v, 0.0), 2.0);
+    s0 += dc;
+  }
+  sum0 = block_sum(s0, sh);  // broadcast to every thread (block_sum ends with a barrier)
+  if (threadIdx.x == 0) sh[9] = sum0;
+  __syncthreads();
+  sum0 = sh[9];
+  __syncthreads();
+}
+
+// One renormalisation pass; returns whether the molecule was already done at its start.
+__device__ inline bool amix_pass(double* di, int n, double& sum0, double* sh) {
+  double s2 = 0.0;
+  for (int r = threadIdx.x; r < n; r += blockDim.x) s2 += di[r];
+  s2 = block_sum(s2, sh);
+  if (threadIdx.x == 0) sh[10] = s2;
+  __syncthreads();
+  s2 = sh[10];
+  const bool large = s2 > 1.0e-3;
+  const double sum3 = large ? sum0 / s2 : 0.0;
+  const bool done = !large || fabs(sum3 - 1.0) <= 1.0e-5;
+  double nfull = 0.0;
+  for (int r = threadIdx.x; r < n; r += blockDim.x) {
+    const double sc = fmax(di[r] * sum3, 0.0);
+    const bool full = sc > 2.0;
+    di[r] = full ? 2.0 : sc;
+    nfull += full ? 1.0 : 0.0;
+  }
+  nfull = block_sum(nfull, sh);
+  if (threadIdx.x == 0) sh[11] = nfull;
+  __syncthreads();
+  sum0 -= 2.0 * sh[11];
+  __syncthreads();
+  return done;
+}
+
+__global__ void k_amix_init(unsigned* mask) {
+  if (threadIdx.x == 0) *mask = kMixAllPasses;
+}
+
+__global__ __launch_bounds__(kEl) void k_amix_passes(pyseqm_layout_t L, const int* __restrict__ idx, int k,
+                                                     const double* __restrict__ P, const double* __restrict__ Pnew,
+                                                     const double* __restrict__ old2,
+                                                     const long long* __restrict__ orb_off,
+                                                     double* __restrict__ fac_out, unsigned* __restrict__ mask) {
+  extern __shared__ double di[];
+  __shared__ double sh[16];
+  const int m = idx[blockIdx.x];
+  const int n = L.mol_norb[m];
+  const size_t off = L.mol_mat_off[m];
+  double fac, sum0;
+  amix_start(P + off, Pnew + off, old2 + orb_off[m], n, k, di, fac, sum0, sh);
+  unsigned bits = 0;
+  for (int q = 0; q < kMixPasses; ++q)
+    if (amix_pass(di, n, sum0, sh)) bits |= 1u << q;
+  if (threadIdx.x == 0) {
+    fac_out[m] = fac;
+    atomicAnd(mask, bits);
+  }
+}
+
+// Off-diagonal mixing (and Pold = P) over every element; k_amix_diag then rewrites the diagonal.
+__global__ void k_amix_elems(pyseqm_layout_t L, const int* __restrict__ idx, int third,
+                             const double* __restrict__ fac, double* __restrict__ P, const double* __restrict__ Pnew,
+                             double* __restrict__ Pold) {
+  const int m = idx[blockIdx.y];
+  const size_t n2 = (size_t)L.mol_norb[m] * L.mol_norb[m];
+  const size_t off = L.mol_mat_off[m];
+  const double f = third ? fac[m] : 0.0;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n2; e += (size_t)gridDim.x * blockDim.x) {
+    const double p = P[off + e], pn = Pnew[off + e];
+    Pold[off + e] = p;
+    P[off + e] = third ? (1.0 + f) * pn - f * p : pn;
+  }
+}
+
+__global__ __launch_bounds__(kEl) void k_amix_diag(pyseqm_layout_t L, const int* __restrict__ idx, int k,
+                                                   double* __restrict__ P, const double* __restrict__ Pnew,
+                                                   const double* __restrict__ Pold, double* __restrict__ old2,
+                                                   const long long* __restrict__ orb_off,
+                                                   const unsigned* __restrict__ mask) {
+  extern __shared__ double di[];
+  __shared__ double sh[16];
+  const int m = idx[blockIdx.x];
+  const int n = L.mol_norb[m];
+  const size_t off = L.mol_mat_off[m];
+  double* o2 = old2 + orb_off[m];
+  double fac, sum0;
+  amix_start(Pold + off, Pnew + off, o2, n, k, di, fac, sum0, sh);  // Pold holds the unmixed P
+  const unsigned agree = *mask & kMixAllPasses;
+  const int npass = agree ? __builtin_ctz(agree) : kMixPasses;
+  for (int q = 0; q < npass; ++q) amix_pass(di, n, sum0, sh);
+  for (int r = threadIdx.x; r < n; r += blockDim.x) {
+    P[off + (size_t)r * (n + 1)] = di[r];
+    o2[r] = Pold[off + (size_t)r * (n + 1)];
+  }
+}
+
 // Convergence test of get_error (scf_loop.py:106-147): |dE| <= eps, max|FP-PF| <= diis_tol
 // (50 eps in the reference; the mixed-precision density step passes its precision floor),
 // ||dP||_F / (4 natoms) <= 2 eps, max|dP| <= 15 eps.  Two stages: per-block partial sums of
@@ -484,6 +625,20 @@ extern "C" int pyseqm_mix(const pyseqm_layout_t* L, const int* idx, int nact, in
   if (!L || nact < 0) return PYSEQM_EINVAL;
   if (nact == 0) return PYSEQM_OK;
   k_mix<<<dim3(elem_blocks(nmax), nact), kEl, 0, (hipStream_t)stream>>>(*L, idx, alpha, P, Pnew, Pold);
+  return status();
+}
+
+extern "C" int pyseqm_adaptive_mix(const pyseqm_layout_t* L, const int* idx, int nact, int nmax, int k, double* P,
+                                   const double* Pnew, double* Pold, double* old2, const long long* orb_off,
+                                   double* fac, unsigned* mask, void* stream) {
+  if (!L || nact < 0 || k < 0 || nmax < 0 || nmax > 8192) return PYSEQM_EINVAL;
+  if (nact == 0) return PYSEQM_OK;
+  hipStream_t s = (hipStream_t)stream;
+  const size_t lds = (size_t)nmax * sizeof(double);
+  k_amix_init<<<1, 64, 0, s>>>(mask);
+  k_amix_passes<<<nact, kEl, lds, s>>>(*L, idx, k, P, Pnew, old2, orb_off, fac, mask);
+  k_amix_elems<<<dim3(elem_blocks(nmax), nact), kEl, 0, s>>>(*L, idx, k % 3 == 0, fac, P, Pnew, Pold);
+  k_amix_diag<<<nact, kEl, lds, s>>>(*L, idx, k, P, Pnew, Pold, old2, orb_off, mask);
   return status();
 }
 

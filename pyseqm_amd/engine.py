@@ -277,8 +277,11 @@ class SCFEngine:
         nc_host = np.ones(b.nmol, dtype=bool)
         it = 0
         if kind == 1:
-            from .mixing import AdaptiveMixer
-            mixer = AdaptiveMixer(self)
+            dev = self.device
+            old2 = torch.zeros(max(int(b.norb.sum()), 1), dtype=torch.float64, device=dev)  # diag of P two steps back
+            orb_off = torch.as_tensor(np.concatenate([[0], np.cumsum(b.norb)[:-1]]).astype(np.int64), device=dev)
+            fac = torch.zeros(b.nmol, dtype=torch.float64, device=dev)
+            mask = torch.zeros(1, dtype=torch.int32, device=dev)
         krange = range(0, max_iter + 1) if kind == 0 else range(1, max_iter + 1)
         for k in krange:
             idx, nact = self._active(nc_host)
@@ -291,7 +294,8 @@ class SCFEngine:
                 nat.call("pyseqm_mix", b.L, nat.ptr(idx), nact, nmax, float(converger[1]), nat.ptr(P),
                          nat.ptr(Pnew), nat.ptr(Pold), self.stream)
             else:
-                mixer.step(k, P, Pnew, Pold, idx, self._idx_host)
+                nat.call("pyseqm_adaptive_mix", b.L, nat.ptr(idx), nact, nmax, int(k), nat.ptr(P), nat.ptr(Pnew),
+                         nat.ptr(Pold), nat.ptr(old2), nat.ptr(orb_off), nat.ptr(fac), nat.ptr(mask), self.stream)
             act = self._act_mask(nc_host)
             self.fock(P, F, active=act, eelec=e_new)
             nat.call("pyseqm_scf_check", b.L, nat.ptr(idx), nact, nat.ptr(P), nat.ptr(Pold), nat.ptr(e_new),

@@ -221,3 +221,43 @@ def test_diis_step_op_extrapolates():
         Fref = sum(cref[j].item() * Fs[j][o:o + n * n] for j in range(3))
         got = F[o:o + n * n]
         assert (got - Fref).abs().max().item() < 1e-8 * Fref.abs().max().item(), m
+
+
+@pytest.mark.parametrize("k", [3, 5, 6])
+def test_adaptive_mix_kernel_matches_oracle(k):
+    """pyseqm_adaptive_mix (scf_converger [1], scf_loop.py:350-420) against the oracle's
+    adaptive_mix on a ragged batch whose molecules need different numbers of renormalisation
+    passes (the batch-wide pass count applies to all), at the extrapolating third iterations
+    (k = 3, 6; damped at 6) and a damped plain one (k = 5)."""
+    from pyseqm_amd import _native as nat
+    from pyseqm_amd.engine import SCFEngine
+    sp, c = W.alkanes(7, seed=11, nmin=2, nmax=12)
+    ob = R.parse(sp, c, "AM1")
+    H, T, _, _, _ = R.hcore(ob)
+    P0 = R.initial_density(ob)
+    _, P1, _ = R.density_eigh(ob, R.fock(ob, P0, H, T))
+    _, P2, _ = R.density_eigh(ob, R.fock(ob, 0.5 * (P0 + P1), H, T))
+    rng = np.random.default_rng(k)
+    n = P0.shape[1]
+    old2 = np.einsum("nii->ni", P0) * (1.0 + 0.3 * rng.random((ob.nmol, n)))
+    Pprev = 0.5 * (P0 + P1)
+    Pref, dprev = R._adaptive_mix(k, Pprev, P2, old2)
+    eng = SCFEngine(sp, c, "AM1", DEV)
+    b = eng.b
+    P = b.from_padded(torch.as_tensor(Pprev, device=DEV))
+    Pn = b.from_padded(torch.as_tensor(P2, device=DEV))
+    Pold = torch.zeros_like(P)
+    # old2 packed per orbital (the padded diagonal's real orbitals, molecule by molecule)
+    pm, pr = b.padded_orbital_index()
+    o2 = torch.as_tensor(old2[pm, pr], device=DEV)
+    orb_off = torch.as_tensor(np.concatenate([[0], np.cumsum(b.norb)[:-1]]).astype(np.int64), device=DEV)
+    idx = torch.arange(b.nmol, dtype=torch.int32, device=DEV)
+    fac = torch.zeros(b.nmol, dtype=torch.float64, device=DEV)
+    mask = torch.zeros(1, dtype=torch.int32, device=DEV)
+    Pin = P.clone()
+    nat.call("pyseqm_adaptive_mix", b.L, nat.ptr(idx), b.nmol, b.nmax, k, nat.ptr(P), nat.ptr(Pn), nat.ptr(Pold),
+             nat.ptr(o2), nat.ptr(orb_off), nat.ptr(fac), nat.ptr(mask), eng.stream)
+    torch.cuda.synchronize()
+    np.testing.assert_allclose(b.to_padded(P).cpu().numpy(), Pref, atol=1e-12, rtol=0)
+    assert torch.equal(Pold, Pin)
+    np.testing.assert_allclose(o2.cpu().numpy(), np.einsum("nii->ni", Pprev)[pm, pr], atol=0, rtol=0)
