@@ -150,6 +150,18 @@ int pyseqm_fock_noalloc(const pyseqm_layout_t* L, const double* atom_par, const 
  * Gershgorin range (diag.py:172-192), for a batched symmetric eigensolver. */
 int pyseqm_pad_for_eigh(const pyseqm_layout_t* L, const double* F, const int* idx, int nact, int nmax,
                         double* out, double* red, int nb, void* stream);
+/* FP64 density step by diagonalisation with one workgroup per listed molecule, for blocks of up to
+ * 192 orbitals: P = 2 C_occ C_occ^T of each packed Fock block F (sym_eig_trunc, diag.py:110-241,
+ * and the density of make_Pnew, scf_loop.py:55-103), written to the packed P, with no padding.
+ * Householder tridiagonalisation in LDS, multisection for the nocc lowest eigenvalues, inverse
+ * iteration, Gram-Schmidt inside clusters of close eigenvalues, back-transformation.  nmax /
+ * nocc_max bound the listed blocks (pyseqm_eigh_density_lds_ok(nmax, nocc_max) must hold); ws
+ * holds pyseqm_eigh_density_workspace(nact, nmax) doubles; fail [1] counts vectors that
+ * Gram-Schmidt reduced to zero (dependent vectors; it stays 0). */
+long long pyseqm_eigh_density_workspace(int nact, int nmax);
+int pyseqm_eigh_density_lds_ok(int norb, int nocc);
+int pyseqm_eigh_density(const pyseqm_layout_t* L, const int* idx, int nact, int nmax, int nocc_max,
+                        const double* F, double* P, double* ws, int* fail, void* stream);
 /* P = 2 C_occ C_occ^T from the padded eigenvectors (columns) (diag.py:230-232).  V holds nact
  * blocks of nmax x nmax; nmax must be at least the orbital count of every listed molecule. */
 int pyseqm_density_from_evecs(const pyseqm_layout_t* L, const double* V, int nmax, const int* idx,

@@ -39,6 +39,7 @@ _SIGS = {
     "pyseqm_fock_noalloc": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                             c_void_p, c_void_p],
     "pyseqm_pad_for_eigh": [c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_int, c_void_p],
+    "pyseqm_eigh_density": [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
     "pyseqm_density_from_evecs": [c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p],
     "pyseqm_mix": [c_void_p, c_int, c_int, c_double, c_void_p, c_void_p, c_void_p, c_void_p],
     "pyseqm_copy_active": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p],
@@ -105,12 +106,17 @@ def lib():
             f.restype = c_int
         L.pyseqm_fock_scratch_doubles.argtypes = [_P]
         L.pyseqm_fock_scratch_doubles.restype = c_longlong
+        L.pyseqm_eigh_density_workspace.argtypes = [c_int, c_int]
+        L.pyseqm_eigh_density_workspace.restype = c_longlong
+        L.pyseqm_eigh_density_lds_ok.argtypes = [c_int, c_int]
+        L.pyseqm_eigh_density_lds_ok.restype = c_int
         _lib = L
     return _lib
 
 
 def exported_symbols():
-    return list(_SIGS) + list(_RAW_SIGS) + ["pyseqm_fock_scratch_doubles"]
+    return list(_SIGS) + list(_RAW_SIGS) + ["pyseqm_fock_scratch_doubles", "pyseqm_eigh_density_workspace",
+                                            "pyseqm_eigh_density_lds_ok"]
 
 
 def stream_ptr(device=None):

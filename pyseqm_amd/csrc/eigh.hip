@@ -1,0 +1,483 @@
+// FP64 density step by diagonalisation, one workgroup per molecule (blocks up to 192 orbitals):
+// P = 2 C_occ C_occ^T of the packed Fock block (sym_eig_trunc, seqm/seqm_functions/diag.py:110-241,
+// as the SCF's make_Pnew uses it, scf_loop.py:55-103), without padding the block to a batch size.
+//
+//   1. Householder reduction of the block to tridiagonal form T = Q^T F Q, with the lower triangle
+//      held packed in LDS (row r at r (r + 1) / 2); the reflectors go to the global workspace.
+//   2. The nocc lowest eigenvalues of T by multisection on Sturm counts: a group of lanes per
+//      eigenvalue evaluates the count at evenly spaced points of its bracket each round.
+//   3. Their eigenvectors by inverse iteration (tridiagonal LU with partial pivoting, three solves
+//      from a fixed pseudo-random start), one thread per vector, the vectors in LDS.
+//   4. Vectors of eigenvalues closer than 1e-5 |T| (a cluster) orthogonalised against each other
+//      by two Gram-Schmidt passes, every vector normalised.  Inverse iteration leaves vectors of
+//      eigenvalues further apart orthogonal to eps |T| / gap < 3e-11; a degenerate eigenvalue's
+//      vectors come from different start vectors, so they span its subspace and the passes make
+//      them orthonormal inside it (P depends on the occupied subspace only).
+//   5. Back-transformation C = Q Z in LDS (groups of lanes per column of Z), then P = 2 C C^T.
+// Every reduction runs in a fixed order: reruns are bitwise identical.
+#include "common.h"
+
+using namespace pyseqm;
+
+namespace {
+
+constexpr int kNT = 1024;           // threads per workgroup (16 waves)
+constexpr int kNW = kNT / 64;
+constexpr int kEighMaxN = 192;      // largest block this solver takes
+constexpr int kLdsDoubles = 20480;  // 160 KiB of LDS
+
+// workspace per listed molecule, in doubles, at slot * 4 * nmax^2:
+//   [0, nmax^2)          reflectors: row i holds v_i (v_i[i+1] = 1 implicit) and tau_i at [i][i]
+//   [nmax^2, 4 nmax^2)   the U factor of each inverse-iteration LU (reciprocal diagonal and two
+//                        superdiagonals), element k of vector j at k * nmax + j
+__device__ inline size_t tri(int r, int c) { return (size_t)r * (r + 1) / 2 + c; }
+
+__device__ inline double eig_block_sum(double v, double* red) {
+  v = wave_sum(v);
+  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  __syncthreads();
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  double s = 0.0;
+#pragma unroll
+  for (int k = 0; k < kNW; ++k) s += red[k];
+  return s;  // every thread
+}
+
+__device__ inline double eig_block_max(double v, double* red) {
+  for (int o = 1; o < 64; o <<= 1) v = fmax(v, __shfl_xor(v, o));
+  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  __syncthreads();
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  double s = red[0];
+#pragma unroll
+  for (int k = 1; k < kNW; ++k) s = fmax(s, red[k]);
+  return s;
+}
+
+__device__ inline double eig_block_min(double v, double* red) {
+  for (int o = 1; o < 64; o <<= 1) v = fmin(v, __shfl_xor(v, o));
+  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  __syncthreads();
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  double s = red[0];
+#pragma unroll
+  for (int k = 1; k < kNW; ++k) s = fmin(s, red[k]);
+  return s;
+}
+
+// threads per row of the row-group mappings: the largest power of two with m * tpr <= kNT (<= 64)
+__device__ inline int rows_tpr(int m) {
+  int t = 64;
+  while (t > 1 && m * t > kNT) t >>= 1;
+  return t;
+}
+
+template <int G>
+__device__ inline double group_max_g(double v) {
+#pragma unroll
+  for (int o = 1; o < G; o <<= 1) v = fmax(v, __shfl_xor(v, o));
+  return v;
+}
+template <int G>
+__device__ inline double group_min_g(double v) {
+#pragma unroll
+  for (int o = 1; o < G; o <<= 1) v = fmin(v, __shfl_xor(v, o));
+  return v;
+}
+__device__ inline double group_max_dyn(double v, int t) {
+  switch (t) {
+    case 64: return group_max_g<64>(v);
+    case 32: return group_max_g<32>(v);
+    case 16: return group_max_g<16>(v);
+    case 8: return group_max_g<8>(v);
+    case 4: return group_max_g<4>(v);
+    case 2: return group_max_g<2>(v);
+    default: return v;
+  }
+}
+__device__ inline double group_min_dyn(double v, int t) {
+  switch (t) {
+    case 64: return group_min_g<64>(v);
+    case 32: return group_min_g<32>(v);
+    case 16: return group_min_g<16>(v);
+    case 8: return group_min_g<8>(v);
+    case 4: return group_min_g<4>(v);
+    case 2: return group_min_g<2>(v);
+    default: return v;
+  }
+}
+
+// number of eigenvalues of the tridiagonal (d, e) below x (Sturm count of the LDL^T pivots)
+__device__ inline int sturm_count(const double* d, const double* e, int n, double x, double pivmin) {
+  int cnt = 0;
+  double q = d[0] - x;
+  if (fabs(q) < pivmin) q = -pivmin;
+  cnt += q < 0.0;
+  for (int k = 1; k < n; ++k) {
+    q = (d[k] - x) - (e[k - 1] * e[k - 1]) / q;
+    if (fabs(q) < pivmin) q = -pivmin;
+    cnt += q < 0.0;
+  }
+  return cnt;
+}
+
+template <int G>
+__device__ inline double group_reduce(double v) {
+#pragma unroll
+  for (int o = 1; o < G; o <<= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+__device__ inline double group_sum_dyn(double v, int tpr) {
+  switch (tpr) {
+    case 64: return group_reduce<64>(v);
+    case 32: return group_reduce<32>(v);
+    case 16: return group_reduce<16>(v);
+    case 8: return group_reduce<8>(v);
+    case 4: return group_reduce<4>(v);
+    case 2: return group_reduce<2>(v);
+    default: return v;
+  }
+}
+
+// deterministic pseudo-random start vector element in (-1, 1)
+__device__ inline double start_value(int j, int k) {
+  unsigned h = (unsigned)j * 2654435761u + (unsigned)k * 40503u + 12345u;
+  h ^= h >> 15;
+  h *= 2246822519u;
+  h ^= h >> 13;
+  return (double)h * (2.0 / 4294967296.0) - 1.0;
+}
+
+__global__ __launch_bounds__(kNT) void k_eigh_density(pyseqm_layout_t L, const int* __restrict__ idx,
+                                                       const double* __restrict__ F, double* __restrict__ P,
+                                                       double* __restrict__ ws, int nmax,
+                                                       int* __restrict__ fail) {
+  extern __shared__ double lds[];
+  const int slot = blockIdx.x;
+  const int m = idx[slot];
+  const int n = L.mol_norb[m];
+  const int nocc = L.mol_nocc[m];
+  const size_t mo = L.mol_mat_off[m];
+  const int tid = threadIdx.x;
+  const size_t nm2 = (size_t)nmax * nmax;
+  double* HH = ws + (size_t)slot * 4 * nm2;
+  double* U0 = HH + nm2;
+  double* U1 = HH + 2 * nm2;
+  double* U2 = HH + 3 * nm2;
+  const int big = max(n * (n + 1) / 2, n * nocc);
+  double* A = lds;  // packed lower triangle, then the eigenvectors (n x nocc)
+  double* d = lds + big;
+  double* e = d + n;
+  double* vb = e + n;
+  double* pb = vb + n;
+  double* lam = pb + n;
+  double* red = lam + n;  // 16
+  const double* Fm = F + mo;
+
+  // ---- 1. load the lower triangle (waves over rows, lanes over columns)
+  for (int r = tid >> 6; r < n; r += kNW)
+    for (int c = tid & 63; c <= r; c += 64) A[tri(r, c)] = Fm[(size_t)r * n + c];
+  __syncthreads();
+
+  // ---- Householder tridiagonalisation (dsytd2, lower)
+  for (int i = 0; i + 2 < n; ++i) {
+    const int mlen = n - i - 1;
+    for (int k = tid; k < mlen; k += kNT) vb[k] = A[tri(i + 1 + k, i)];
+    __syncthreads();
+    double part = 0.0;
+    for (int k = 1 + tid; k < mlen; k += kNT) part += vb[k] * vb[k];
+    const double xn2 = eig_block_sum(part, red);
+    const double alpha = vb[0];
+    double tau = 0.0, beta = alpha, scal = 0.0;
+    if (xn2 > 0.0) {
+      beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
+      tau = (beta - alpha) / beta;
+      scal = 1.0 / (alpha - beta);
+    }
+    if (tid == 0) {
+      d[i] = A[tri(i, i)];
+      e[i] = beta;
+      HH[(size_t)i * nmax + i] = tau;
+    }
+    __syncthreads();
+    if (tau == 0.0) continue;  // the same decision in every thread
+    for (int k = tid; k < mlen; k += kNT) {
+      const double v = k == 0 ? 1.0 : vb[k] * scal;
+      vb[k] = v;
+      HH[(size_t)i * nmax + i + 1 + k] = v;
+    }
+    __syncthreads();
+    // p = tau A22 v (row groups of tpr lanes)
+    const int tpr = rows_tpr(mlen);
+    const int ng = kNT / tpr, g = tid / tpr, gl = tid % tpr;
+    for (int r0 = 0; r0 < mlen; r0 += ng) {
+      const int rr = r0 + g;
+      double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+      if (rr < mlen) {
+        const int r = i + 1 + rr;
+        int cc = gl;
+        for (; cc + 3 * tpr < mlen; cc += 4 * tpr) {
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int c = i + 1 + cc + u * tpr;
+            const double a = c <= r ? A[tri(r, c)] : A[tri(c, r)];
+            const double t = a * vb[cc + u * tpr];
+            if (u == 0) s0 += t;
+            else if (u == 1) s1 += t;
+            else if (u == 2) s2 += t;
+            else s3 += t;
+          }
+        }
+        for (; cc < mlen; cc += tpr) {
+          const int c = i + 1 + cc;
+          s0 += (c <= r ? A[tri(r, c)] : A[tri(c, r)]) * vb[cc];
+        }
+      }
+      double s = group_sum_dyn((s0 + s1) + (s2 + s3), tpr);
+      if (rr < mlen && gl == 0) pb[rr] = tau * s;
+    }
+    __syncthreads();
+    part = 0.0;
+    for (int k = tid; k < mlen; k += kNT) part += pb[k] * vb[k];
+    const double pv = eig_block_sum(part, red);
+    const double a2 = -0.5 * tau * pv;
+    for (int k = tid; k < mlen; k += kNT) pb[k] += a2 * vb[k];  // w
+    __syncthreads();
+    // A22 -= v w^T + w v^T (lower triangle)
+    for (int r0 = 0; r0 < mlen; r0 += ng) {
+      const int rr = r0 + g;
+      if (rr < mlen) {
+        const int r = i + 1 + rr;
+        const double vr = vb[rr], wr = pb[rr];
+        for (int cc = gl; cc <= rr; cc += tpr) A[tri(r, i + 1 + cc)] -= vr * pb[cc] + wr * vb[cc];
+      }
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    if (n >= 2) {
+      d[n - 2] = A[tri(n - 2, n - 2)];
+      e[n - 2] = A[tri(n - 1, n - 2)];
+    }
+    d[n - 1] = A[tri(n - 1, n - 1)];
+  }
+  __syncthreads();
+
+  // ---- 2. the nocc lowest eigenvalues: multisection, a group of tb lanes per eigenvalue
+  double lo_p = 1.0e300, hi_p = -1.0e300, e2max = 0.0;
+  for (int k = tid; k < n; k += kNT) {
+    const double el = k > 0 ? fabs(e[k - 1]) : 0.0, er = k + 1 < n ? fabs(e[k]) : 0.0;
+    lo_p = fmin(lo_p, d[k] - el - er);
+    hi_p = fmax(hi_p, d[k] + el + er);
+    if (k + 1 < n) e2max = fmax(e2max, e[k] * e[k]);
+  }
+  double glo = eig_block_min(lo_p, red);
+  double ghi = eig_block_max(hi_p, red);
+  e2max = eig_block_max(e2max, red);
+  const double tnorm = fmax(fabs(glo), fabs(ghi));
+  const double pivmin = 1.0e-300 * fmax(1.0, e2max);
+  glo -= 2.2e-16 * tnorm * 4.0 + 2.0 * pivmin;
+  ghi += 2.2e-16 * tnorm * 4.0 + 2.0 * pivmin;
+  {
+    int tb = 64;
+    while (tb > 1 && nocc * tb > kNT) tb >>= 1;
+    const int ngrp = kNT / tb, gi = tid / tb, gl = tid % tb;
+    for (int j0 = 0; j0 < nocc; j0 += ngrp) {
+      const int j = j0 + gi;
+      if (j < nocc) {  // uniform over the lane group
+        double lo = glo, hi = ghi;
+        for (int round = 0; round < 64; ++round) {
+          if (!(hi - lo > 4.4e-16 * fmax(fabs(lo), fabs(hi)) + 2.0 * pivmin)) break;
+          const double x = lo + (hi - lo) * (double)(gl + 1) / (double)(tb + 1);
+          const int cnt = (x > lo && x < hi) ? sturm_count(d, e, n, x, pivmin) : -1;
+          const double nlo = group_max_dyn(cnt >= 0 && cnt <= j ? x : lo, tb);
+          const double nhi = group_min_dyn(cnt > j ? x : hi, tb);
+          if (nlo == lo && nhi == hi) break;  // no representable interior point left
+          lo = nlo;
+          hi = nhi;
+        }
+        if (gl == 0) lam[j] = 0.5 * (lo + hi);
+      }
+    }
+  }
+  __syncthreads();
+
+  // ---- 3. inverse iteration, one thread per eigenvector; vector j in LDS at Z[k * nocc + j]
+  double* Z = A;
+  const double tiny = 2.2e-16 * fmax(tnorm, 1.0e-300);
+  for (int j = tid; j < nocc; j += kNT) {
+    for (int k = 0; k < n; ++k) Z[k * nocc + j] = start_value(j, k);
+    const double sig = lam[j];
+    for (int rep = 0; rep < 3; ++rep) {
+      // LU of T - sig I with partial pivoting, the right-hand side eliminated alongside; the
+      // running row and right-hand side stay in registers, U goes to the workspace
+      double diag = d[0] - sig, sup = n > 1 ? e[0] : 0.0;
+      double yk = Z[j];
+      for (int k = 0; k + 1 < n; ++k) {
+        const double sub = e[k], nd = d[k + 1] - sig, ns = k + 2 < n ? e[k + 1] : 0.0;
+        const double yk1 = Z[(k + 1) * nocc + j];
+        const size_t o = (size_t)k * nmax + j;
+        if (fabs(diag) >= fabs(sub)) {
+          const double piv = fabs(diag) < tiny ? copysign(tiny, diag) : diag;
+          const double rp = 1.0 / piv;
+          const double mlt = sub * rp;
+          U0[o] = rp;
+          U1[o] = sup;
+          U2[o] = 0.0;
+          Z[k * nocc + j] = yk;
+          yk = yk1 - mlt * yk;
+          diag = nd - mlt * sup;
+          sup = ns;
+        } else {
+          const double rp = 1.0 / sub;
+          const double mlt = diag * rp;
+          U0[o] = rp;
+          U1[o] = nd;
+          U2[o] = ns;
+          Z[k * nocc + j] = yk1;
+          yk = yk - mlt * yk1;
+          diag = sup - mlt * nd;
+          sup = -mlt * ns;
+        }
+      }
+      const double rlast = 1.0 / (fabs(diag) < tiny ? copysign(tiny, diag) : diag);
+      double x1 = yk * rlast, x2 = 0.0, xmax = fabs(x1);
+      Z[(n - 1) * nocc + j] = x1;
+      // back substitution; U of row k - 1 is loaded while row k is solved
+      double u0n = 0.0, u1n = 0.0, u2n = 0.0;
+      if (n >= 2) {
+        const size_t o = (size_t)(n - 2) * nmax + j;
+        u0n = U0[o];
+        u1n = U1[o];
+        u2n = U2[o];
+      }
+      for (int k = n - 2; k >= 0; --k) {
+        const double u0 = u0n, u1 = u1n, u2 = u2n;
+        if (k >= 1) {
+          const size_t o = (size_t)(k - 1) * nmax + j;
+          u0n = U0[o];
+          u1n = U1[o];
+          u2n = U2[o];
+        }
+        const double x = (Z[k * nocc + j] - u1 * x1 - u2 * x2) * u0;
+        Z[k * nocc + j] = x;
+        xmax = fmax(xmax, fabs(x));
+        x2 = x1;
+        x1 = x;
+      }
+      const double sc = xmax > 0.0 ? 1.0 / xmax : 1.0;
+      for (int k = 0; k < n; ++k) Z[k * nocc + j] *= sc;
+    }
+  }
+  __syncthreads();
+
+  // ---- 4. clusters (eigenvalues closer than ctol) orthogonalised by the thread of their first
+  //         member, two Gram-Schmidt passes; every vector normalised
+  const double ctol = 1.0e-5 * fmax(tnorm, 1.0e-300);
+  for (int j = tid; j < nocc; j += kNT) {
+    if (j > 0 && lam[j] - lam[j - 1] <= ctol) continue;  // inside a cluster: its first thread works
+    int jend = j + 1;
+    while (jend < nocc && lam[jend] - lam[jend - 1] <= ctol) ++jend;
+    for (int q = j; q < jend; ++q) {
+      for (int pass = 0; pass < 2; ++pass)
+        for (int p = j; p < q; ++p) {
+          double dot = 0.0;
+          for (int k = 0; k < n; ++k) dot += Z[k * nocc + p] * Z[k * nocc + q];
+          for (int k = 0; k < n; ++k) Z[k * nocc + q] -= dot * Z[k * nocc + p];
+        }
+      double nr = 0.0;
+      for (int k = 0; k < n; ++k) nr += Z[k * nocc + q] * Z[k * nocc + q];
+      if (!(nr > 1.0e-200)) {  // a vector lost to its cluster's span: flagged, left unscaled
+        atomicAdd(fail, 1);
+        continue;
+      }
+      const double sc = 1.0 / sqrt(nr);
+      for (int k = 0; k < n; ++k) Z[k * nocc + q] *= sc;
+    }
+  }
+  __syncthreads();
+
+  // ---- 5. C = Q Z in place (reflectors last to first), tc lanes per column of Z
+  double* C = Z;
+  {
+    int tc = 64;
+    while (tc > 1 && nocc * tc > kNT) tc >>= 1;
+    const int a = tid / tc, cl = tid % tc;
+    for (int i = n - 3; i >= 0; --i) {
+      const double tau = HH[(size_t)i * nmax + i];
+      if (tau == 0.0) continue;  // uniform
+      const int mlen = n - i - 1;
+      for (int k = tid; k < mlen; k += kNT) vb[k] = HH[(size_t)i * nmax + i + 1 + k];
+      __syncthreads();
+      double sdot = 0.0;
+      if (a < nocc)
+        for (int k = cl; k < mlen; k += tc) sdot += vb[k] * C[(i + 1 + k) * nocc + a];
+      sdot = tau * group_sum_dyn(sdot, tc);
+      if (a < nocc)
+        for (int k = cl; k < mlen; k += tc) C[(i + 1 + k) * nocc + a] -= sdot * vb[k];
+      __syncthreads();
+    }
+  }
+  // ---- P = 2 C C^T, 2 x 2 output blocks per thread over the upper triangle, both triangles written
+  double* Pm = P + mo;
+  const int nb2 = (n + 1) / 2;
+  for (int t = tid; t < nb2 * nb2; t += kNT) {
+    const int bi = t / nb2, bj = t % nb2;
+    if (bj < bi) continue;
+    const int r0 = 2 * bi, c0 = 2 * bj;
+    const int r1 = min(r0 + 1, n - 1), c1 = min(c0 + 1, n - 1);
+    double s00 = 0.0, s01 = 0.0, s10 = 0.0, s11 = 0.0;
+    for (int a = 0; a < nocc; ++a) {
+      const double x0 = C[(size_t)r0 * nocc + a], x1 = C[(size_t)r1 * nocc + a];
+      const double y0 = C[(size_t)c0 * nocc + a], y1 = C[(size_t)c1 * nocc + a];
+      s00 += x0 * y0;
+      s01 += x0 * y1;
+      s10 += x1 * y0;
+      s11 += x1 * y1;
+    }
+    const bool hr = r0 + 1 < n, hc = c0 + 1 < n;
+    Pm[(size_t)r0 * n + c0] = 2.0 * s00;
+    Pm[(size_t)c0 * n + r0] = 2.0 * s00;
+    if (hc) {
+      Pm[(size_t)r0 * n + c0 + 1] = 2.0 * s01;
+      Pm[(size_t)(c0 + 1) * n + r0] = 2.0 * s01;
+    }
+    if (hr) {
+      Pm[(size_t)(r0 + 1) * n + c0] = 2.0 * s10;
+      Pm[(size_t)c0 * n + r0 + 1] = 2.0 * s10;
+    }
+    if (hr && hc) {
+      Pm[(size_t)(r0 + 1) * n + c0 + 1] = 2.0 * s11;
+      Pm[(size_t)(c0 + 1) * n + r0 + 1] = 2.0 * s11;
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" long long pyseqm_eigh_density_workspace(int nact, int nmax) {
+  if (nact < 0 || nmax < 0) return -1;
+  return 4LL * nact * nmax * nmax;
+}
+
+extern "C" int pyseqm_eigh_density_lds_ok(int norb, int nocc) {
+  if (norb < 1 || norb > kEighMaxN || nocc < 0 || nocc > norb) return 0;
+  const int big = norb * (norb + 1) / 2 > norb * nocc ? norb * (norb + 1) / 2 : norb * nocc;
+  return big + 5 * norb + 16 <= kLdsDoubles;
+}
+
+extern "C" int pyseqm_eigh_density(const pyseqm_layout_t* L, const int* idx, int nact, int nmax, int nocc_max,
+                                   const double* F, double* P, double* ws, int* fail, void* stream) {
+  if (!L || nact < 0 || nmax < 1 || nmax > kEighMaxN || nocc_max < 0 || nocc_max > nmax || !fail)
+    return PYSEQM_EINVAL;
+  if (!pyseqm_eigh_density_lds_ok(nmax, nocc_max)) return PYSEQM_EINVAL;
+  if (nact == 0) return PYSEQM_OK;
+  const int big = nmax * (nmax + 1) / 2 > nmax * nocc_max ? nmax * (nmax + 1) / 2 : nmax * nocc_max;
+  const size_t lds = (size_t)(big + 5 * nmax + 16) * sizeof(double);
+  k_eigh_density<<<nact, kNT, lds, (hipStream_t)stream>>>(*L, idx, F, P, ws, nmax, fail);
+  return hipGetLastError() == hipSuccess ? PYSEQM_OK : PYSEQM_ELAUNCH;
+}

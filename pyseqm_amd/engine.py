@@ -38,6 +38,22 @@ EIGH_MIN_N_BATCH = 16
 MIXED_DIIS_MIN_N = 256
 
 
+# Blocks up to this size (and whose LDS footprint fits, eigh_wg_ok) are diagonalised by the
+# one-workgroup-per-molecule solver of eigh.hip.
+EIGH_WG_MAX_N = 192
+EIGH_WG_LDS_DOUBLES = 20480
+
+
+def eigh_wg_ok(norb, nocc):
+    """Whether the workgroup eigensolver takes blocks of norb orbitals / nocc occupied orbitals:
+    n <= 192 and the packed lower triangle (or the n x nocc vectors) plus 5 n + 16 doubles fit
+    160 KiB of LDS (pyseqm_eigh_density_lds_ok)."""
+    n = np.asarray(norb, dtype=np.int64)
+    o = np.asarray(nocc, dtype=np.int64)
+    big = np.maximum(n * (n + 1) // 2, n * o)
+    return (n >= 1) & (n <= EIGH_WG_MAX_N) & (o <= n) & (big + 5 * n + 16 <= EIGH_WG_LDS_DOUBLES)
+
+
 def size_groups(norb, ratio=0.7, min_group=32, floor=EIGH_MIN_N):
     """Partition molecules (by position) into groups of similar orbital counts: sorted by size,
     a group takes molecules down to `ratio` of its largest; groups smaller than `min_group` join
@@ -105,7 +121,9 @@ class SCFEngine:
         self.coords = None
         self.trace = None  # set to a list to collect per-iteration (dE, diis, dm, dm_max) metrics
         self.diis_log = None  # set to a list to collect (cF, slot, B matrices, coefficients, resets, errors)
-        self.eigh_log = collections.deque(maxlen=256)  # (blocks, padded size) of the recent batched eigensolves
+        self.eigh_log = collections.deque(maxlen=256)
+        # vectors the workgroup eigensolver lost to Gram-Schmidt (stays 0 unless its vectors degenerate)
+        self.eigh_fail = torch.zeros(1, dtype=torch.int32, device=dev)  # (blocks, padded size) of the recent batched eigensolves
         if coordinates is not None:
             self.set_coordinates(coordinates)
 
@@ -182,30 +200,40 @@ class SCFEngine:
 
     # ---------------------------------------------------------- density step
     def density_eigh(self, F, idx, nact, Pout, want_evecs=False):
-        """Batched FP64 eigensolve of the active Fock matrices -> Pout (diag.py:110-241).
+        """FP64 density step by diagonalisation of the active Fock blocks -> Pout (diag.py:110-241).
 
-        The active molecules are grouped by size so that each batched solve pads to its own
-        group's largest block (the reference pads every molecule to the batch's largest, which
-        for an uneven batch multiplies the O(n^3) work)."""
+        Blocks of up to EIGH_WG_MAX_N orbitals go to the one-workgroup-per-molecule solver
+        (pyseqm_eigh_density: Householder in LDS, multisection, inverse iteration, cluster
+        Gram-Schmidt, no padding), launched per size group so that each launch reserves LDS for its own largest
+        block.  Larger blocks, and calls that want the eigenvectors themselves, take the batched
+        library eigensolver on Gershgorin-padded stacks, again per size group (the reference pads
+        every molecule to the batch's largest, which for an uneven batch multiplies the O(n^3) work)."""
         b = self.b
         if not want_evecs:
-            groups = size_groups(b.norb[self._idx_host])
-            if len(groups) > 1:
-                idx_h = np.asarray(self._idx_host)
-                for sel in groups:
-                    sub = torch.as_tensor(idx_h[sel].astype(np.int32), device=self.device)
-                    saved = self._idx_host
-                    self._idx_host = idx_h[sel]
-                    self.density_eigh(F, sub, len(sel), Pout, want_evecs=True)
-                    self._idx_host = saved
+            idx_h = np.asarray(self._idx_host)
+            wg = eigh_wg_ok(b.norb[idx_h], b.nocc[idx_h])
+            if wg.any():
+                ids = idx_h[wg]
+                for sel in size_groups(b.norb[ids], floor=1):
+                    self._eigh_wg(F, ids[sel], Pout)
+            rest = idx_h[~wg]
+            if len(rest) == 0:
                 return None
+            groups = size_groups(b.norb[rest])
+            saved = self._idx_host
+            for sel in groups:
+                sub = torch.as_tensor(rest[sel].astype(np.int32), device=self.device)
+                self._idx_host = rest[sel]
+                self.density_eigh(F, sub, len(sel), Pout, want_evecs=True)
+            self._idx_host = saved
+            return None
         nmax = int(b.norb[self._idx_host].max())
         if nact >= EIGH_MIN_N_BATCH:
             nmax = max(nmax, EIGH_MIN_N)
         Fp = torch.empty(nact, nmax, nmax, dtype=torch.float64, device=self.device)
         nat.call("pyseqm_pad_for_eigh", b.L, nat.ptr(F), nat.ptr(idx), nact, nmax, nat.ptr(Fp), nat.ptr(self.red),
                  nat.red_blocks(nmax), self.stream)
-        self.eigh_log.append((nact, nmax))
+        self.eigh_log.append((nact, nmax, "library"))
         e, V = torch.linalg.eigh(Fp)
         V = V.contiguous()
         nat.call("pyseqm_density_from_evecs", b.L, nat.ptr(V), nmax, nat.ptr(idx), nact, nat.ptr(Pout),
@@ -213,6 +241,17 @@ class SCFEngine:
         if want_evecs:
             return e, V, nmax
         return None
+
+    def _eigh_wg(self, F, ids, Pout):
+        """One launch of the workgroup eigensolver on the molecules `ids` (host int array)."""
+        b = self.b
+        nmax = int(b.norb[ids].max())
+        nocc_max = int(b.nocc[ids].max())
+        ws = torch.empty(max(4 * len(ids) * nmax * nmax, 1), dtype=torch.float64, device=self.device)
+        idx = torch.as_tensor(ids.astype(np.int32), device=self.device)
+        nat.call("pyseqm_eigh_density", b.L, nat.ptr(idx), len(ids), nmax, nocc_max, nat.ptr(F), nat.ptr(Pout),
+                 nat.ptr(ws), nat.ptr(self.eigh_fail), self.stream)
+        self.eigh_log.append((len(ids), nmax, "workgroup"))
 
     def _sp2_workspace(self):
         ws = getattr(self, "_sp2ws", None)

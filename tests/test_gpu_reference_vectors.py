@@ -4,8 +4,8 @@ made by tests/golden/make_reference_vectors.py from lanl/PYSEQM on the CPU).
 Bars: FP64 density (diagonalisation) Hf / Etot / Eelec / Enuc within 1e-6 eV of the reference and the
 same SCF iteration count within one; the mixed-precision SP2 density within 1e-4 eV/atom.  The 128
 alkanes are the first 128 of the 1024-alkane benchmark batch: the test also asserts that this run
-takes the benchmark's density path (more than one size group of batched eigensolves) and that it
-holds atom pairs beyond the 40-bohr overlap cutoff (hcore.py:83).
+takes the benchmark's density path (one-workgroup-per-molecule eigensolves in more than one size
+group) and that it holds atom pairs beyond the 40-bohr overlap cutoff (hcore.py:83).
 """
 import numpy as np
 import pytest
@@ -52,11 +52,13 @@ def test_c2_slice_takes_the_benchmark_path():
         far += int((d > 40.0).sum())
     assert far > 0
     eng, r, en, _, _ = _run(c)
-    groups = E.size_groups(eng.b.norb)
+    groups = E.size_groups(eng.b.norb, floor=1)
     assert len(groups) >= 2
     first = list(eng.eigh_log)[:len(groups)]  # the first SCF iteration solves every group
-    assert sorted(n for _, n in first) == sorted(int(eng.b.norb[g].max()) for g in groups), first
-    assert sum(nb for nb, _ in first) == eng.b.nmol
+    assert all(kind == "workgroup" for _, _, kind in first), first  # unpadded workgroup eigensolves
+    assert sorted(n for _, n, _ in first) == sorted(int(eng.b.norb[g].max()) for g in groups), first
+    assert sum(nb for nb, _, _ in first) == eng.b.nmol
+    assert int(eng.eigh_fail.item()) == 0
     np.testing.assert_allclose(en["Hf"], c["Hf"], rtol=0, atol=1e-6)
 
 

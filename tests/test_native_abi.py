@@ -56,3 +56,12 @@ def test_fock_scratch_size_matches_header():
     src = open(os.path.join(ROOT, "include", "pyseqm_hip.h")).read()
     m = re.search(r"#define PYSEQM_FOCK_SCRATCH_DOUBLES \((\d+)LL \* 1024 \* 1024\)", src)
     assert m and int(m.group(1)) * 1024 * 1024 == _native.FOCK_SCRATCH_DOUBLES
+
+
+def test_workgroup_eigensolver_size_rule_matches_the_library():
+    from pyseqm_amd.engine import eigh_wg_ok
+    lib = _native.lib()
+    for n in list(range(1, 40)) + list(range(150, 200)) + [256, 400]:
+        for nocc in sorted({0, n // 3, n // 2, (2 * n) // 3, n}):
+            assert bool(eigh_wg_ok(n, nocc)) == bool(lib.pyseqm_eigh_density_lds_ok(n, nocc)), (n, nocc)
+    assert lib.pyseqm_eigh_density_workspace(7, 100) == 4 * 7 * 100 * 100
