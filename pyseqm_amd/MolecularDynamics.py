@@ -5,7 +5,9 @@
 ``run(molecule, steps, ..., seed)`` signatures.  The XL-BOMD trajectory batch lives on the device
 (``XLTrajectories``): one step is a fixed sequence of kernel launches with no host round trip.
 Thermodynamic records (T, Ek, Ep per step) are kept in memory in ``self.history``; the HDF5/xyz
-writers and checkpoints of the reference are not part of this path.
+writers of the reference are not part of this path.  ``XL_BOMD.save_checkpoint`` /
+``XL_BOMD.run_from_checkpoint`` (MolecularDynamics.py:1099-1166) write and resume the device-resident
+trajectory state (safetensors, nothing executed on load), continuing a run bit for bit.
 """
 import numpy as np
 import torch
@@ -178,6 +180,62 @@ class XL_BOMD(Molecular_Dynamics_Basic):
         tr.step_count = i
         tr.step()
         self._sync_molecule(molecule)
+
+    # --------------------------------------------------------- checkpoints
+    def save_checkpoint(self, molecule, steps, reuse_P=True, remove_com=None, *, step_done, path):
+        """Write the trajectory batch after `step_done` steps of a `steps`-step run (the reference's
+        save_checkpoint, MolecularDynamics.py:1099-1116): device state, thermo history, parameters."""
+        from .xlbomd import save_state
+        extra = {"MD_type": "XL_BOMD", "seqm_parameters": self.seqm_parameters, "xl_bomd_params": self.xl_bomd_params,
+                 "timestep": self.timestep, "Temp": self.Temp, "steps": int(steps), "step_done": int(step_done),
+                 "reuse_P": bool(reuse_P), "species": molecule.species.detach().cpu().tolist(),
+                 "charges": molecule.tot_charge.detach().cpu().tolist() if torch.is_tensor(molecule.tot_charge) else 0,
+                 "history": {k: [np.asarray(v).tolist() for v in vals] for k, vals in self.history.items()}}
+        save_state(path, self.traj.state_dict(), extra)
+
+    @staticmethod
+    def run_from_checkpoint(path, device=None):
+        """Resume a run written by save_checkpoint and carry it to its `steps` (the reference's
+        run_from_checkpoint, MolecularDynamics.py:1124-1166).  Returns (md, molecule)."""
+        from .Molecule import Molecule
+        from .seqm_functions.constants import Constants
+        from .xlbomd import load_state
+        state, extra = load_state(path)
+        if extra.get("MD_type") != "XL_BOMD":
+            raise RuntimeError(f"unknown MD type {extra.get('MD_type')!r} in checkpoint")
+        dev = torch.device(device) if device is not None else torch.device("cuda")
+        species = torch.as_tensor(extra["species"], dtype=torch.int64, device=dev)
+        const = Constants().to(dev)
+        from .layout import PackedBatch
+        b0 = PackedBatch(species.cpu().numpy(), extra["seqm_parameters"]["method"], "cpu")
+        x = state["tensors"]["x"][:b0.natoms].numpy()
+        coords = np.zeros((species.shape[0], species.shape[1], 3))
+        coords.reshape(-1, 3)[b0.real_flat] = x
+        charges = torch.as_tensor(extra["charges"], dtype=torch.float64, device=dev) \
+            if isinstance(extra["charges"], list) else extra["charges"]
+        molecule = Molecule(const, extra["seqm_parameters"], torch.as_tensor(coords, device=dev), species,
+                            charges=charges)
+        md = XL_BOMD(xl_bomd_params=extra["xl_bomd_params"], seqm_parameters=extra["seqm_parameters"],
+                     timestep=extra["timestep"], Temp=extra["Temp"], step_offset=extra["step_done"])
+        md.set_dof(molecule, 0.0)
+        eng = molecule.engine()
+        bt = eng.b
+        sp2 = extra["seqm_parameters"].get("sp2", [False])
+        t = state["tensors"]
+        md.traj = XLTrajectories(eng, md.k, md.timestep, t["x"].to(dev), t["v"].to(dev), t["P"].to(dev),
+                                 density="sp2" if sp2[0] else "eigh")
+        md.traj.load_state_dict(state)
+        eng.set_coordinates(md.traj.x)
+        molecule.Electronic_entropy = torch.zeros(bt.nmol, dtype=torch.float64, device=dev)
+        md._sync_molecule(molecule)
+        md.history = {k: [np.asarray(v) for v in vals] for k, vals in extra["history"].items()}
+        reuse_P = extra["reuse_P"]
+        for i in range(md.step_offset, extra["steps"]):
+            md._do_integrator_step(i, molecule, dict())
+            if not reuse_P:
+                molecule.dm = None
+            md._record(molecule)
+        return md, molecule
 
     def _sync_molecule(self, molecule):
         """Write the device state back into the reference's padded attributes."""

@@ -268,6 +268,22 @@ class XLService:
         self.latencies.append(dt)
         return dt
 
+    def checkpoint(self, path):
+        """Write every trajectory's device state and the step counter (safetensors)."""
+        from .xlbomd import save_state
+        self.stream.synchronize()
+        save_state(path, self.traj.state_dict(), {"service_steps": self.steps})
+
+    def restore(self, path):
+        """Load a checkpoint into the resident batch in place: the captured graphs keep replaying on
+        the same buffers, and the trajectories continue from the saved step."""
+        from .xlbomd import load_state
+        state, extra = load_state(path)
+        with torch.cuda.stream(self.stream):
+            self.traj.load_state_dict(state)
+        self.stream.synchronize()
+        self.steps = int(extra["service_steps"])
+
     def run_until(self, stop_event):
         while not stop_event.is_set():
             self.step()

@@ -125,6 +125,35 @@ class XLTrajectories:
         self._verlet(1)
         self.step_count += 1
 
+    # ---------------------------------------------------------- checkpoints
+    STATE = ("x", "v", "a", "f", "P", "D", "Pt", "eelec")
+
+    def state_dict(self):
+        """The trajectory batch's device state (positions, velocities, accelerations, forces, the
+        density field P, D, the history ring Pt, the XL energies) and its step counter: what a run
+        needs to continue bit for bit (the integrals and Fock matrices follow from x and P)."""
+        return {"tensors": {k: getattr(self, k) for k in self.STATE},
+                "meta": {"k": self.k, "m": self.m, "dt": self.dt, "step_count": self.step_count,
+                         "density": self.density_kind, "force_h": self.force_h,
+                         "natoms": int(self.eng.b.natoms), "total": int(self.eng.b.total)}}
+
+    def load_state_dict(self, state):
+        """Copy a state_dict into this batch's device tensors, in place (graphs captured on them stay
+        valid); the batch must hold the same molecules, k, time step and density step."""
+        meta, mine = state["meta"], self.state_dict()["meta"]
+        for key in ("k", "m", "natoms", "total", "density"):
+            if meta[key] != mine[key]:
+                raise ValueError(f"checkpoint {key} = {meta[key]!r} does not match this trajectory batch ({mine[key]!r})")
+        if float(meta["dt"]) != self.dt:
+            raise ValueError(f"checkpoint time step {meta['dt']} != {self.dt}")
+        with torch.no_grad():
+            for k in self.STATE:
+                src, dst = state["tensors"][k], getattr(self, k)
+                if tuple(src.shape) != tuple(dst.shape):
+                    raise ValueError(f"checkpoint {k}: shape {tuple(src.shape)} != {tuple(dst.shape)}")
+                dst.copy_(src.to(dst.device, dst.dtype))
+        self.step_count = int(meta["step_count"])
+
     # ------------------------------------------------------------- thermo
     def energies(self):
         eng = self.eng
@@ -197,3 +226,27 @@ class ForceXL(torch.nn.Module):
         force = tr.eng.b.atom_to_padded(tr.f[:tr.eng.b.natoms].clone())
         molecule.Electronic_entropy = EEnt
         return force, D, Hf, Etot, Eelec, Enuc, Eiso, e, e_gap
+
+
+def save_state(path, state, extra=None):
+    """Write a state_dict (plus JSON-serialisable `extra`) to `path` as safetensors, atomically (a
+    temporary file renamed over the target); reading it back executes nothing from the file."""
+    import json
+    import os
+
+    from safetensors.torch import save_file
+    meta = {"meta": json.dumps(state["meta"]), "extra": json.dumps(extra or {})}
+    tmp = f"{path}.tmp{os.getpid()}"
+    save_file({k: v.detach().contiguous().cpu() for k, v in state["tensors"].items()}, tmp, metadata=meta)
+    os.replace(tmp, path)
+
+
+def load_state(path):
+    """(state_dict with host tensors, extra) from a file written by save_state."""
+    import json
+
+    from safetensors import safe_open
+    with safe_open(path, framework="pt", device="cpu") as fh:
+        md = fh.metadata()
+        tensors = {k: fh.get_tensor(k) for k in fh.keys()}
+    return {"tensors": tensors, "meta": json.loads(md["meta"])}, json.loads(md["extra"])

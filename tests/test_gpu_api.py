@@ -138,3 +138,28 @@ def test_per_pair_kbeta_is_refused():
     mol = _molecule(["methane.xyz"], p)
     with pytest.raises(NotImplementedError, match="Kbeta"):
         pyseqm_amd.Electronic_Structure(p)(mol, learned_parameters={"Kbeta": torch.ones(4, 4, device=DEV)})
+
+
+@pytest.mark.parametrize("method", ["MNDO", "AM1", "PM3"])
+def test_reference_style_script_through_seqm_imports(method):
+    """A caller written against the reference (its own smoke test's calls, through the `seqm`
+    module names) runs unchanged and reproduces the reference golden."""
+    import seqm  # noqa: F401
+    from seqm.ElectronicStructure import Electronic_Structure
+    from seqm.Molecule import Molecule
+    from seqm.seqm_functions.constants import Constants
+    from seqm.seqm_functions.read_xyz import read_xyz as seqm_read_xyz
+    from tests.conftest import GOLDEN
+    torch.set_default_dtype(torch.float64)
+    species, coordinates = seqm_read_xyz([f"{GOLDEN}/data/methane.xyz"])
+    species = torch.as_tensor(species, dtype=torch.int64, device=DEV)
+    coordinates = torch.as_tensor(coordinates, dtype=torch.float64, device=DEV)
+    const = Constants().to(DEV)
+    seqm_parameters = {"method": method, "scf_eps": 1.0e-6, "scf_converger": [1]}
+    molecule = Molecule(const, seqm_parameters, coordinates, species).to(DEV)
+    esdriver = Electronic_Structure(seqm_parameters).to(DEV)
+    esdriver(molecule)
+    g = golden(f"smoke_single_point_{method}")
+    for k in ("Etot", "Eelec", "Enuc"):
+        assert abs(getattr(molecule, k).item() - g[k]) < 1e-9, (k, getattr(molecule, k).item(), g[k])
+    np.testing.assert_allclose(molecule.force.cpu().numpy().reshape(-1), np.array(g["force"]).reshape(-1), atol=1e-6)
