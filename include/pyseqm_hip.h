@@ -66,6 +66,21 @@ typedef struct {
                                 doubles, valid lanes packed in lane order */
 } pyseqm_layout_t;
 
+/* Builds the pair tiles of a packed batch on the device: the pair enumeration of the reference
+ * Parser (seqm/basics.py:219-403: pairs i < j of each molecule) grouped into 8 x 8 atom tiles of
+ * the three kinds, as the tiles / tile_woff / csr_off / csr_idx fields of pyseqm_layout_t hold them.
+ *   mol_atom_off [nmol+1], mol_nheavy [nmol]: atom offsets and heavy-atom counts (atoms of a
+ *     molecule stored heavy first);
+ *   mol_tile_off [3*(nmol+1)], ntile [3] (host array): each molecule's first tile per kind and the
+ *     tile counts (closed forms the caller computes in O(nmol));
+ *   outputs tiles [4*ntiles], tile_woff [ntiles], w_size [3] (doubles of each kind's integral
+ *     buffer), csr_off [natoms+1], csr_idx [slots];
+ *   scratch: ntiles + natoms/2 + 1 int64.
+ * Records and slot lists come out in the host enumeration's order. */
+int pyseqm_build_tiles(int nmol, int natoms, const int* mol_atom_off, const int* mol_nheavy, const int* mol_tile_off,
+                       const int* ntile, int* tiles, long long* tile_woff, long long* w_size, int* csr_off,
+                       int* csr_idx, long long* scratch, void* stream);
+
 /* Derived per-atom quantities: dd, qq (charge separations), rho0/1/2 (additive terms),
  * tore, qn.  Replaces two_elec_two_center_int.py:117-147 + cal_par.py:11-280. */
 int pyseqm_atom_setup(const pyseqm_layout_t* L, double* atom_par, void* stream);

@@ -80,6 +80,8 @@ _RAW_SIGS = {
                                     c_void_p],
     "pyseqm_md_verlet": [c_int, c_int, c_double, c_double, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                          c_void_p],
+    "pyseqm_build_tiles": [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                           c_void_p, c_void_p, c_void_p, c_void_p],
 }
 _lib = None
 

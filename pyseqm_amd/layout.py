@@ -7,6 +7,8 @@ follows the reference Parser (seqm/basics.py:219-403): atoms are the non-zero en
 row in order, pairs are i < j within a molecule in row-major order (the reference pair index
 is kept in ``pair_ref`` for outputs in reference order), nocc = sum(tore)/2 - charge/2.
 """
+import ctypes
+
 import numpy as np
 import torch
 
@@ -140,10 +142,91 @@ class PackedBatch:
         self.npairs_mol = self.nat * (self.nat - 1) // 2
         self.pair_base = np.concatenate([[0], np.cumsum(self.npairs_mol)]).astype(np.int64)
         self.npairs = int(self.pair_base[-1])
-        self._build_tiles()
-        self._to_device()
+        if self.device.type == "cuda":
+            self._tile_counts()
+            self._to_device(device_tiles=True)
+        else:
+            self._build_tiles()
+            self._to_device()
 
     # ------------------------------------------------------------------ tiles
+    def _tile_counts(self):
+        """Per-molecule tile and slot counts in closed form (O(nmol)); the tiles themselves are built
+        on the device (pyseqm_build_tiles).  A triangle region of n atoms has nb (nb + 1) / 2 tiles of
+        nb = ceil(n / 8) per side, less a last diagonal tile holding one atom, and (nb + 1) n partial
+        slots (less that tile's 2); a rectangle nr x nc has nbr nbc tiles and nbc nr + nbr nc slots."""
+        nh, nH = self.nheavy.astype(np.int64), self.nhydro.astype(np.int64)
+        nbx, nbh = -(-nh // TILE), -(-nH // TILE)
+        skip_x, skip_h = (nh % TILE == 1).astype(np.int64), (nH % TILE == 1).astype(np.int64)
+        counts = [nbx * (nbx + 1) // 2 - skip_x, nbx * nbh, nbh * (nbh + 1) // 2 - skip_h]
+        self.ntile = [int(c.sum()) for c in counts]
+        base = np.cumsum([0] + self.ntile)
+        self.tile_base = base[:3]
+        self.mol_tile_off = np.concatenate([np.concatenate([[0], np.cumsum(c)]) + b
+                                            for c, b in zip(counts, base[:3])]).astype(np.int64)
+        self.tile_kind = np.concatenate([np.full(n, k) for k, n in enumerate(self.ntile)]).astype(np.int64)
+        slots = ((nbx + 1) * nh - 2 * skip_x) + (nbh * nh + nbx * nH) + ((nbh + 1) * nH - 2 * skip_h)
+        self.nslots = int(slots.sum())
+        self._host_tiles = None
+        self._lanes = None
+
+    def _fetch_tiles(self):
+        """Host copies of the device-built tile records, integral offsets and slot lists."""
+        if self._host_tiles is None:
+            nt = sum(self.ntile)
+            rec = self.d["tiles"][:4 * nt].cpu().numpy().astype(np.int64).reshape(-1, 4)
+            T = np.zeros((nt, 6), dtype=np.int64)
+            T[:, :3] = rec[:, :3]
+            T[:, 3] = rec[:, 3] & 0xff
+            T[:, 4] = (rec[:, 3] >> 8) & 0xff
+            T[:, 5] = (rec[:, 3] >> 16) & 1
+            nrow, ncol, diag = T[:, 3], T[:, 4], T[:, 5] != 0
+            nvalid = np.where(diag, nrow * (nrow - 1) // 2, nrow * ncol).astype(np.int64)
+            self._host_tiles = (T, self.d["tile_woff"][:nt].cpu().numpy().astype(np.int64), (nvalid + 15) // 16 * 16,
+                                self.d["csr_off"].cpu().numpy().astype(np.int64),
+                                self.d["csr_idx"][:self.nslots].cpu().numpy().astype(np.int64))
+        return self._host_tiles
+
+    @property
+    def tiles_host(self):
+        return self._tiles_host if hasattr(self, "_tiles_host") else self._fetch_tiles()[0]
+
+    @tiles_host.setter
+    def tiles_host(self, v):
+        self._tiles_host = v
+
+    @property
+    def tile_woff_host(self):
+        return self._tile_woff_host if hasattr(self, "_tile_woff_host") else self._fetch_tiles()[1]
+
+    @tile_woff_host.setter
+    def tile_woff_host(self, v):
+        self._tile_woff_host = v
+
+    @property
+    def tile_wstride(self):
+        return self._tile_wstride if hasattr(self, "_tile_wstride") else self._fetch_tiles()[2]
+
+    @tile_wstride.setter
+    def tile_wstride(self, v):
+        self._tile_wstride = v
+
+    @property
+    def csr_off_host(self):
+        return self._csr_off_host if hasattr(self, "_csr_off_host") else self._fetch_tiles()[3]
+
+    @csr_off_host.setter
+    def csr_off_host(self, v):
+        self._csr_off_host = v
+
+    @property
+    def csr_idx_host(self):
+        return self._csr_idx_host if hasattr(self, "_csr_idx_host") else self._fetch_tiles()[4]
+
+    @csr_idx_host.setter
+    def csr_idx_host(self, v):
+        self._csr_idx_host = v
+
     def _build_tiles(self):
         kinds = []
         a0 = self.mol_atom_off[:-1].astype(np.int64)
@@ -238,25 +321,44 @@ class PackedBatch:
         return self.d["pair_ref"]
 
     # ----------------------------------------------------------------- device
-    def _to_device(self):
+    def _to_device(self, device_tiles=False):
         dev = self.device
         i32 = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=torch.int32, device=dev)
         self.d = {
             "mol_atom_off": i32(self.mol_atom_off), "mol_norb": i32(self.norb), "mol_nocc": i32(self.nocc),
             "mol_mat_off": torch.as_tensor(self.mol_mat_off, dtype=torch.int64, device=dev),
             "mol_tile_off": i32(self.mol_tile_off), "atom_mol": i32(self.atom_mol), "atom_ao": i32(self.atom_ao),
-            "atom_Z": i32(self.Z), "csr_off": i32(self.csr_off_host), "csr_idx": i32(self.csr_idx_host),
-            "tile_woff": torch.as_tensor(self.tile_woff_host if len(self.tile_woff_host) else np.zeros(1, np.int64),
-                                         dtype=torch.int64, device=dev),
+            "atom_Z": i32(self.Z),
         }
-        T = self.tiles_host
-        packed = np.zeros((len(T), 4), dtype=np.int64)
-        if len(T):
-            packed[:, 0] = T[:, 0]
-            packed[:, 1] = T[:, 1]
-            packed[:, 2] = T[:, 2]
-            packed[:, 3] = T[:, 3] | (T[:, 4] << 8) | (T[:, 5] << 16)
-        self.d["tiles"] = i32(packed.reshape(-1)) if len(T) else torch.zeros(4, dtype=torch.int32, device=dev)
+        if device_tiles:
+            nt = sum(self.ntile)
+            self.d["tiles"] = torch.zeros(max(4 * nt, 4), dtype=torch.int32, device=dev)
+            self.d["tile_woff"] = torch.zeros(max(nt, 1), dtype=torch.int64, device=dev)
+            self.d["csr_off"] = torch.zeros(self.natoms + 1, dtype=torch.int32, device=dev)
+            self.d["csr_idx"] = torch.zeros(max(self.nslots, 1), dtype=torch.int32, device=dev)
+            w_size = torch.zeros(3, dtype=torch.int64, device=dev)
+            scratch = torch.empty(nt + (self.natoms + 1) // 2 + 1, dtype=torch.int64, device=dev)
+            ntile = (ctypes.c_int * 3)(*self.ntile)
+            nheavy = i32(self.nheavy)
+            _native.call_raw("pyseqm_build_tiles", self.nmol, self.natoms, _native.ptr(self.d["mol_atom_off"]),
+                             _native.ptr(nheavy), _native.ptr(self.d["mol_tile_off"]), ntile,
+                             _native.ptr(self.d["tiles"]), _native.ptr(self.d["tile_woff"]), _native.ptr(w_size),
+                             _native.ptr(self.d["csr_off"]), _native.ptr(self.d["csr_idx"]), _native.ptr(scratch),
+                             _native.stream_ptr(dev))
+            self.w_size = [int(v) for v in w_size.cpu()]  # the one host read of the build
+        else:
+            self.d["csr_off"] = i32(self.csr_off_host)
+            self.d["csr_idx"] = i32(self.csr_idx_host)
+            self.d["tile_woff"] = torch.as_tensor(self.tile_woff_host if len(self.tile_woff_host) else np.zeros(1, np.int64),
+                                                  dtype=torch.int64, device=dev)
+            T = self.tiles_host
+            packed = np.zeros((len(T), 4), dtype=np.int64)
+            if len(T):
+                packed[:, 0] = T[:, 0]
+                packed[:, 1] = T[:, 1]
+                packed[:, 2] = T[:, 2]
+                packed[:, 3] = T[:, 3] | (T[:, 4] << 8) | (T[:, 5] << 16)
+            self.d["tiles"] = i32(packed.reshape(-1)) if len(T) else torch.zeros(4, dtype=torch.int32, device=dev)
         L = _native.Layout()
         L.nmol, L.natoms, L.npairs = self.nmol, self.natoms, self.npairs
         for k in range(3):

@@ -62,6 +62,13 @@ class Request:
         self.npairs = nat * (nat - 1) // 2
         self.bytes = 0
 
+    @classmethod
+    def from_records(cls, buf, first_rid=0, t_arrival=None):
+        """Requests from a stream of binary request records (pyseqm_amd.wire)."""
+        from . import wire
+        return [cls(first_rid + k, Z, X, method, charge, t_arrival)
+                for k, (Z, X, method, charge) in enumerate(wire.decode(buf))]
+
     @property
     def latency(self):
         return None if self.t_done is None else self.t_done - self.t_arrival

@@ -59,3 +59,19 @@ def test_xl_graph_replay_matches_eager():
         dx = (g.traj.x - e.traj.x).abs().max().item()
         assert dx < 1e-10, (step, dx)
     np.testing.assert_allclose(g.energies(), e.energies(), atol=1e-9, rtol=0)
+
+
+def test_wire_records_served_like_arrays():
+    """Requests arriving as binary records (pyseqm_amd.wire) give the results of the same molecules
+    submitted as arrays, with the batch layout built on the device."""
+    from pyseqm_amd import wire
+    sp, x = W.alkanes(6, seed=9, nmax=12)
+    buf = b"".join(wire.encode(sp[m], x[m], "AM1") for m in range(6))
+    srv = SCFServer(DEV, lanes=1, density="eigh", eps=1e-7, max_batch=16)
+    reqs = [srv.submit(r) for r in Request.from_records(buf)]
+    srv.close()
+    e = SCFEngine(sp, x, "AM1", DEV)
+    hf = e.energies(e.scf(eps=1e-7, converger=(2,)).P)["Hf"].cpu().numpy()
+    for m, r in enumerate(reqs):
+        assert r.result["converged"]
+        assert abs(r.result["Hf"] - hf[m]) < 1e-5, (m, r.result["Hf"], hf[m])

@@ -154,3 +154,30 @@ def test_sp2_launch_order_is_balanced_largest_first():
             assert np.all(np.diff(norb[r]) <= 0)
             runs.append(len(r))
         assert max(runs) - min(runs) <= 1
+
+
+def test_request_records_round_trip():
+    from pyseqm_amd import wire
+    from pyseqm_amd import workloads as W
+    sp, x = W.alkanes(3, seed=1, nmax=5)
+    ws, wx = W.water_clusters(4, 1, seed=2)
+    rng = np.random.default_rng(0)
+    recs = []
+    buf = b""
+    for m in range(3):
+        n = int((sp[m] > 0).sum())
+        perm = rng.permutation(n)  # any atom order on the wire
+        buf += wire.encode(sp[m, :n][perm], x[m, :n][perm], "AM1", 0)
+    buf += wire.encode(ws[0], wx[0], "PM3", -2)
+    recs = wire.decode(buf)
+    assert [r[2] for r in recs] == ["AM1"] * 3 + ["PM3"] and recs[3][3] == -2
+    for m in range(3):
+        n = int((sp[m] > 0).sum())
+        np.testing.assert_array_equal(recs[m][0], sp[m, :n])  # decreasing Z, stable
+        assert {tuple(v) for v in recs[m][1]} == {tuple(v) for v in x[m, :n]}
+    s2, x2, ch = wire.batch(recs[:3])
+    assert s2.shape[0] == 3 and ch.tolist() == [0, 0, 0]
+    with pytest.raises(ValueError, match="truncated"):
+        wire.decode(buf[:-5])
+    with pytest.raises(ValueError, match="request record"):
+        wire.decode(b"XXXX" + buf[4:])
