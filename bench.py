@@ -463,7 +463,7 @@ def main():
             "workload": f"{args.scale_config}: {e4.b.nmol} x {int(e4.b.nmax)} orbitals, {e4.b.natoms} atoms, "
                         f"mixed SP2 SCF {t4 * 1e3:.1f} ms, {r4.iterations} SCF iterations, converged "
                         f"{int((~r4.notconverged).sum())}/{e4.b.nmol}",
-            "fock": fock_roofline(e4, r4.P, args.fock_reps),
+            "fock": dict(fock_roofline(e4, r4.P, args.fock_reps), traffic=measured_traffic(args.scale_config)),
             "sp2": sp2_roofline(e4, step4)}
         del e4
     if args.serve_window > 0:
