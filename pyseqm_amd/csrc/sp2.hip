@@ -2,7 +2,7 @@
 //
 // Trace-correcting SP2 (seqm/seqm_functions/SP2.py:9-85): X0 = (hN I - F)/(hN - h1) from the
 // Gershgorin bounds, then X <- X^2 if |tr X^2 - N| < |2 tr X - tr X^2 - N| else 2X - X^2, until the
-// trace error stops improving.  Here
+// idempotency error stops falling (k_sp2_decide).  Here
 //   * X is kept in FP32 in a zero-padded per-molecule block (leading dimension ld = n rounded up
 //     to 32, so K never needs a tail and rows are 16-byte aligned);
 //   * X^2 runs on v_mfma_f32_32x32x16_bf16 as the 3-term split hi*hi + hi*lo + lo*hi
@@ -26,8 +26,11 @@ typedef double doublex4 __attribute__((ext_vector_type(4)));
 
 namespace {
 
-// per-molecule SP2 state, doubles
-enum { S_TR = 0, S_FRO, S_E0, S_E1, S_E2, S_BRANCH, S_DONE, S_IT, S_NST };
+// per-molecule SP2 state, doubles: trace and squared Frobenius norm of the current X, its
+// idempotency error tr X - ||X||_F^2 and those of the two previous iterates, the polynomial of the
+// next step (1: X^2, 0: 2X - X^2), stop flag, iteration count, and the polynomials of the last two
+// steps taken (-1 before the first)
+enum { S_TR = 0, S_FRO, S_E0, S_E1, S_E2, S_BRANCH, S_DONE, S_IT, S_B1, S_B2, S_NST };
 
 __device__ inline double wsum(double v) {
   for (int o = 1; o < 64; o <<= 1) v += __shfl_xor(v, o);
@@ -142,10 +145,11 @@ __global__ void k_sp2_state0(const int* __restrict__ idx, int nact, int nb, cons
   const double N = nocc[m];
   s[S_TR] = t;
   s[S_FRO] = f;
-  s[S_E0] = s[S_E1] = s[S_E2] = fabs(t - N);
+  s[S_E0] = s[S_E1] = s[S_E2] = fabs(t - f);
   s[S_BRANCH] = fabs(f - N) < fabs(2.0 * t - f - N) ? 1.0 : 0.0;
   s[S_DONE] = 0.0;
   s[S_IT] = 0.0;
+  s[S_B1] = s[S_B2] = -1.0;
 }
 
 // Zero-padded (nmax x nmax) copy of F with the padded diagonal above the Gershgorin range
@@ -773,9 +777,16 @@ __global__ __launch_bounds__(256) void k_gemm_bf16x3_f64(const int* __restrict__
       }
 }
 
-// Sum the tile partials (fixed order), update the trace errors, decide the next branch and
-// whether the molecule stops: err < eps and no improvement over two iterations ago (the
-// reference's reduced-precision criterion, SP2.py:75-79), or iteration cap.
+// Sum the tile partials (fixed order), update the idempotency errors, decide the next branch and
+// whether the molecule stops.  Stop when the trace error is below eps (SP2.py:75-79) and the
+// idempotency error e = tr X - ||X||_F^2 = sum_i x_i (1 - x_i) has stopped falling: e_k >= e_{k-2}
+// after two steps with different polynomials, with e_k < 3/16 (the parameterless criterion of
+// Kruchinina, Rudberg and Rubensson, J. Chem. Theory Comput. 12, 5788 (2016)).  With e < 3/16 every
+// eigenvalue lies within 1/4 of 0 or 1, where the step pair (X^2, 2X - X^2) in either order
+// shrinks every x (1 - x) in exact arithmetic, so a non-decrease means rounding dominates.  The
+// reference's reduced-precision rule (trace error not falling) is not used: the deviations of
+// occupied (below 1) and virtual (above 0) eigenvalues cancel in the trace, so its error can dip
+// mid-run and stop the expansion while eigenvalues are still ~1e-2 from 0 or 1.  Or iteration cap.
 __global__ __launch_bounds__(64) void k_sp2_decide(const int* __restrict__ idx, int nact,
                                                    const int* __restrict__ norb, const int* __restrict__ nocc,
                                                    double* __restrict__ st, const double* __restrict__ part,
@@ -804,10 +815,13 @@ __global__ __launch_bounds__(64) void k_sp2_decide(const int* __restrict__ idx, 
   s[S_FRO] = fro;
   s[S_E2] = s[S_E1];
   s[S_E1] = s[S_E0];
-  s[S_E0] = fabs(tr - N);
+  s[S_E0] = fabs(tr - fro);
+  s[S_B2] = s[S_B1];
+  s[S_B1] = s[S_BRANCH];  // the polynomial of the step that made this X
   s[S_IT] += 1.0;
   s[S_BRANCH] = fabs(fro - N) < fabs(2.0 * tr - fro - N) ? 1.0 : 0.0;
-  const bool stop = (s[S_E0] < eps && s[S_E0] >= s[S_E2] && s[S_IT] >= 3.0) || s[S_IT] >= max_iter;
+  const bool settled = fabs(tr - N) < eps && s[S_B1] != s[S_B2] && s[S_E0] < 0.1875 && s[S_E0] >= s[S_E2];
+  const bool stop = (settled && s[S_IT] >= 3.0) || s[S_IT] >= max_iter;
   if (stop) {
     s[S_DONE] = 1.0;
     atomicAdd(ndone, 1);

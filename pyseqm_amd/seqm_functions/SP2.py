@@ -2,9 +2,11 @@
 (seqm/seqm_functions/SP2.py:9-85) on the mixed-precision MFMA kernels.
 
 Every D x D block is one system, zero rows included, exactly as the reference treats its input.
-X runs in FP32 with X^2 on bf16x3 MFMA (FP32 accumulation); the stop rule is the reference's
-reduced-precision one (trace error below max(eps, 1e-2) and no longer improving, SP2.py:21-28,
-74-78); the returned density gets one McWeeny step to FP64 accuracy, factor * (3 X^2 - 2 X^3)."""
+X runs in FP32 with X^2 on bf16x3 MFMA (FP32 accumulation); a block stops when its trace error is
+below max(eps, 1e-2) (SP2.py:21-28) and its idempotency error tr X - tr X^2 no longer falls over
+two steps with different polynomials (in place of the reference's 'trace error no longer
+improving', SP2.py:74-78, which the cancelling eigenvalue deviations can fool); the returned
+density gets one McWeeny step to FP64 accuracy, factor * (3 X^2 - 2 X^3)."""
 import numpy as np
 import torch
 

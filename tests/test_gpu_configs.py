@@ -26,10 +26,9 @@ DEV = "cuda"
 
 
 def _check_density(eng, P):
-    """Trace 2 nocc to 1e-8 relative and max |P^2 - 2P| <= 1e-5.  SP2 stops on the reference's
-    reduced-precision rule (trace error below 1e-2 and no longer improving, SP2.py:21-28), so a
-    few eigenvalues can sit ~1e-3 from 0 / 1 before the McWeeny step, which leaves ~3e-6 of them:
-    measured up to 1.3e-9 relative on the trace of the 1536-orbital blocks."""
+    """Trace 2 nocc to 1e-8 relative and max |P^2 - 2P| <= 1e-5.  SP2 stops once its idempotency
+    error stops falling (FP32 rounding level, ~1e-6 per orbital), and the McWeeny step squares the
+    remaining eigenvalue deviations."""
     b = eng.b
     for m in range(b.nmol):
         n, o = int(b.norb[m]), int(b.mol_mat_off[m])

@@ -225,3 +225,33 @@ def test_mcweeny_matches_fp64_closed_form(sizes):
         assert torch.equal(got, got.T), m
         assert (got - 2.0 * X2).abs().max().item() > 1e-9, m
         assert abs(got.trace().item() - 2 * b.nocc[m]) < 1e-6, m
+
+
+def test_sp2_stop_not_fooled_by_trace_cancellation():
+    """Alkanes 717, 333 and 767 of the 1024-molecule AM1 batch (seed 0): near the SCF's fixed
+    point their SP2 trace error dips to ~1e-3 at iteration 12 (occupied and virtual eigenvalue
+    deviations cancel in the trace), jumps, and is ~8e-3 at iteration 14, under 1e-2 and not below
+    its value two iterations earlier, while tr X - tr X^2 is still ~1e-2.  A stop on the trace
+    error there left tr P short of 2 nocc by ~1e-4 and Hf off by 3.4e-5 eV/atom.  The idempotency
+    criterion runs on to the rounding floor: every final density has trace 2 nocc to 1e-8 and Hf
+    within 1e-6 eV/atom of the FP64 diagonalisation SCF (molecule 5, never affected, is the
+    control)."""
+    sp, x = W.alkanes(1024, seed=0)
+    sel = [717, 333, 767, 5]
+    sp, x = sp[sel], x[sel]
+    k = int((sp > 0).sum(1).max())
+    sp, x = sp[:, :k], x[:, :k]
+    nat_ = (sp > 0).sum(1)
+    e1 = SCFEngine(sp, x, "AM1", DEV)
+    r1 = e1.scf(eps=1e-8, converger=(2,))
+    e2 = SCFEngine(sp, x, "AM1", DEV)
+    r2 = e2.scf(eps=1e-5, converger=(2,), density="sp2")
+    assert not r1.notconverged.any() and not r2.notconverged.any()
+    d = (e2.energies(r2.P)["Hf"] - e1.energies(r1.P)["Hf"]).cpu().numpy() / nat_
+    print("dHf per atom", d, flush=True)
+    assert np.abs(d).max() < 1e-6, d
+    b = e2.b
+    for m in range(b.nmol):
+        n, o = int(b.norb[m]), int(b.mol_mat_off[m])
+        tr = r2.P[o:o + n * n].view(n, n).trace().item()
+        assert abs(tr - 2 * b.nocc[m]) < 1e-8, (m, tr)
