@@ -256,10 +256,10 @@ int pyseqm_diis_combine(const pyseqm_layout_t* L, const int* idx, int nact, int 
 /* ------------------------------------------------------------------ SP2 density step
  * Trace-correcting SP2 (SP2.py:9-85) in mixed precision: X (FP32, per-molecule zero-padded
  * blocks at xoff[m] with leading dimension xld[m], a multiple of 32), X^2 on bf16x3 MFMA with FP32
- * accumulation, per-molecule state[10*m..] = (tr X, tr X^2, idempotency errors tr X - tr X^2 of
- * this and the two previous iterates, next branch, done, iters, branches of the last two steps)
- * on the device.  A molecule stops when its trace error is below eps and its idempotency error
- * (below 3/16) has not fallen over two steps with different polynomials, or at max_iter. */
+ * accumulation, per-molecule state[8*m..] = (tr X, tr X^2, idempotency errors e = tr X - tr X^2
+ * of this and the two previous iterates, next branch, done, iters) on the device.  A molecule
+ * stops when its trace error is below eps and e_k >= 8 e_{k-2}^2 with e_{k-2} < 3/16 (the
+ * quadratic decrease of exact arithmetic has ended), or at max_iter. */
 int pyseqm_sp2_init(const pyseqm_layout_t* L, const double* F, const int* idx, int nact,
                     const long long* xoff, const int* xld, float* X, double* state, double* red, int nb,
                     void* stream);

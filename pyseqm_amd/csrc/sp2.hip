@@ -28,9 +28,8 @@ namespace {
 
 // per-molecule SP2 state, doubles: trace and squared Frobenius norm of the current X, its
 // idempotency error tr X - ||X||_F^2 and those of the two previous iterates, the polynomial of the
-// next step (1: X^2, 0: 2X - X^2), stop flag, iteration count, and the polynomials of the last two
-// steps taken (-1 before the first)
-enum { S_TR = 0, S_FRO, S_E0, S_E1, S_E2, S_BRANCH, S_DONE, S_IT, S_B1, S_B2, S_NST };
+// next step (1: X^2, 0: 2X - X^2), stop flag, iteration count
+enum { S_TR = 0, S_FRO, S_E0, S_E1, S_E2, S_BRANCH, S_DONE, S_IT, S_NST };
 
 __device__ inline double wsum(double v) {
   for (int o = 1; o < 64; o <<= 1) v += __shfl_xor(v, o);
@@ -149,7 +148,6 @@ __global__ void k_sp2_state0(const int* __restrict__ idx, int nact, int nb, cons
   s[S_BRANCH] = fabs(f - N) < fabs(2.0 * t - f - N) ? 1.0 : 0.0;
   s[S_DONE] = 0.0;
   s[S_IT] = 0.0;
-  s[S_B1] = s[S_B2] = -1.0;
 }
 
 // Zero-padded (nmax x nmax) copy of F with the padded diagonal above the Gershgorin range
@@ -778,15 +776,21 @@ __global__ __launch_bounds__(256) void k_gemm_bf16x3_f64(const int* __restrict__
 }
 
 // Sum the tile partials (fixed order), update the idempotency errors, decide the next branch and
-// whether the molecule stops.  Stop when the trace error is below eps (SP2.py:75-79) and the
-// idempotency error e = tr X - ||X||_F^2 = sum_i x_i (1 - x_i) has stopped falling: e_k >= e_{k-2}
-// after two steps with different polynomials, with e_k < 3/16 (the parameterless criterion of
-// Kruchinina, Rudberg and Rubensson, J. Chem. Theory Comput. 12, 5788 (2016)).  With e < 3/16 every
-// eigenvalue lies within 1/4 of 0 or 1, where the step pair (X^2, 2X - X^2) in either order
-// shrinks every x (1 - x) in exact arithmetic, so a non-decrease means rounding dominates.  The
-// reference's reduced-precision rule (trace error not falling) is not used: the deviations of
-// occupied (below 1) and virtual (above 0) eigenvalues cancel in the trace, so its error can dip
-// mid-run and stop the expansion while eigenvalues are still ~1e-2 from 0 or 1.  Or iteration cap.
+// whether the molecule stops.  e = tr X - ||X||_F^2 = sum_i x_i (1 - x_i) over the eigenvalues x_i
+// of X; a molecule stops at iteration k when its trace error is below eps (SP2.py:75-79) and
+// e_k >= 8 e_{k-2}^2 with e_{k-2} < 3/16, or at the iteration cap.  In exact arithmetic, once
+// e_{k-2} < 3/16 every eigenvalue lies within 1/4 of 0 or 1, and a pair of steps with different
+// polynomials maps each deviation d to at most 4 d^2, so e_k <= 4 e_{k-2}^2 (up to the
+// d (1 - d) vs d factor, within the margin of 8); the trace test only repeats a polynomial when
+// the deviations on the side it grows are below the square of the other side's, where e_k stays
+// below that bound too.  So e_k >= 8 e_{k-2}^2 means FP32 rounding has taken over: the expansion
+// stops at its first iterate on the rounding floor (the idea of stopping when the error no longer
+// follows the exact-arithmetic decrease is that of Kruchinina, Rudberg and Rubensson, J. Chem.
+// Theory Comput. 12, 5788 (2016)).  Further FP32 iterations do not help: the trace test then
+// chooses on rounding noise and the density error grows again.  The reference's reduced-precision
+// rule (trace error not falling, SP2.py:74-78) is not used: occupied deviations (below 1) and
+// virtual ones (above 0) cancel in the trace, so the trace error can dip mid-run and stop the
+// expansion while eigenvalues are still ~1e-2 from 0 or 1.
 __global__ __launch_bounds__(64) void k_sp2_decide(const int* __restrict__ idx, int nact,
                                                    const int* __restrict__ norb, const int* __restrict__ nocc,
                                                    double* __restrict__ st, const double* __restrict__ part,
@@ -816,11 +820,9 @@ __global__ __launch_bounds__(64) void k_sp2_decide(const int* __restrict__ idx, 
   s[S_E2] = s[S_E1];
   s[S_E1] = s[S_E0];
   s[S_E0] = fabs(tr - fro);
-  s[S_B2] = s[S_B1];
-  s[S_B1] = s[S_BRANCH];  // the polynomial of the step that made this X
   s[S_IT] += 1.0;
   s[S_BRANCH] = fabs(fro - N) < fabs(2.0 * tr - fro - N) ? 1.0 : 0.0;
-  const bool settled = fabs(tr - N) < eps && s[S_B1] != s[S_B2] && s[S_E0] < 0.1875 && s[S_E0] >= s[S_E2];
+  const bool settled = fabs(tr - N) < eps && s[S_E2] < 0.1875 && s[S_E0] >= 8.0 * s[S_E2] * s[S_E2];
   const bool stop = (settled && s[S_IT] >= 3.0) || s[S_IT] >= max_iter;
   if (stop) {
     s[S_DONE] = 1.0;

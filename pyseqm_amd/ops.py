@@ -162,7 +162,7 @@ def density_sp2(layout: int, F: torch.Tensor, idx: torch.Tensor, nmax: int, xoff
     """SP2 (SP2.py:9-85) in mixed precision on the listed blocks, then the McWeeny step (FP64 accuracy) into the packed
     density P.  `iters` iterations are launched (molecules that stop earlier make the rest no-ops), so the
     call makes no host synchronisation.  Buffers as pyseqm_amd.sp2.SP2Workspace lays them out: X blocks
-    (FP32, Xa / Xb) at xoff with leading dimension xld, state (nmol, 10), part (nmol, part_stride), Y2 (FP64,
+    (FP32, Xa / Xb) at xoff with leading dimension xld, state (nmol, 8), part (nmol, part_stride), Y2 (FP64,
     X layout), ndone (1 int)."""
     L = _L(layout)
     nact = int(idx.numel())

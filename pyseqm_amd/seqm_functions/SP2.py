@@ -3,9 +3,9 @@
 
 Every D x D block is one system, zero rows included, exactly as the reference treats its input.
 X runs in FP32 with X^2 on bf16x3 MFMA (FP32 accumulation); a block stops when its trace error is
-below max(eps, 1e-2) (SP2.py:21-28) and its idempotency error tr X - tr X^2 no longer falls over
-two steps with different polynomials (in place of the reference's 'trace error no longer
-improving', SP2.py:74-78, which the cancelling eigenvalue deviations can fool); the returned
+below max(eps, 1e-2) (SP2.py:21-28) and its idempotency error e = tr X - tr X^2 no longer falls
+quadratically (e_k >= 8 e_{k-2}^2 once e_{k-2} < 3/16), in place of the reference's 'trace error
+no longer improving' (SP2.py:74-78), which the cancelling eigenvalue deviations can fool; the returned
 density gets one McWeeny step to FP64 accuracy, factor * (3 X^2 - 2 X^3)."""
 import numpy as np
 import torch

@@ -5,17 +5,17 @@ SP2.py:9-85): trace-correcting SP2 with X in FP32 and X^2 on bf16x3 MFMA (FP32 a
 branch / stop decided on the device, then one McWeeny step P = 2 (3 X^2 - 2 X^3) to FP64
 accuracy: X^2 on the FP64 MFMA GEMM, X^3 = X^2 + X (X^2 - X) with the small correction product on
 bf16x3 MFMA.  Stop rule: trace error below eps (SP2.py:75-79) and the idempotency error
-tr X - ||X||_F^2 no longer falling over two steps with different polynomials (Kruchinina, Rudberg
-and Rubensson, J. Chem. Theory Comput. 12, 5788 (2016)); the reference's own reduced-precision
-rule watches the trace error, in which occupied and virtual deviations cancel, and can stop
-mid-run with eigenvalues still ~1e-2 from 0 or 1.
+e = tr X - ||X||_F^2 no longer falling quadratically (e_k >= 8 e_{k-2}^2 once e_{k-2} < 3/16: the
+first iterate on the FP32 rounding floor); the reference's own reduced-precision rule watches the
+trace error, in which occupied and virtual deviations cancel, and can stop mid-run with
+eigenvalues still ~1e-2 from 0 or 1.
 """
 import numpy as np
 import torch
 
 from . import _native as nat
 
-NSTATE = 10
+NSTATE = 8
 SP2_EPS = 1.0e-2      # SP2.py:4 (reduced-precision threshold)
 SP2_MAX_ITER = 100
 POLL = 4              # iterations between host checks of the stop counter

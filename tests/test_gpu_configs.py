@@ -26,9 +26,9 @@ DEV = "cuda"
 
 
 def _check_density(eng, P):
-    """Trace 2 nocc to 1e-8 relative and max |P^2 - 2P| <= 1e-5.  SP2 stops once its idempotency
-    error stops falling (FP32 rounding level, ~1e-6 per orbital), and the McWeeny step squares the
-    remaining eigenvalue deviations."""
+    """Trace 2 nocc to 1e-8 relative and max |P^2 - 2P| <= 1e-5.  SP2 stops at its first iterate
+    on the FP32 rounding floor (idempotency error ~1e-6 per orbital), and the McWeeny step squares
+    the remaining eigenvalue deviations."""
     b = eng.b
     for m in range(b.nmol):
         n, o = int(b.norb[m]), int(b.mol_mat_off[m])
