@@ -2,8 +2,12 @@
 // P = 2 C_occ C_occ^T of the packed Fock block (sym_eig_trunc, seqm/seqm_functions/diag.py:110-241,
 // as the SCF's make_Pnew uses it, scf_loop.py:55-103), without padding the block to a batch size.
 //
-//   1. Householder reduction of the block to tridiagonal form T = Q^T F Q, with the lower triangle
-//      held packed in LDS (row r at r (r + 1) / 2); the reflectors go to the global workspace.
+//   1. Householder reduction of the block to tridiagonal form T = Q^T F Q (dsytd2 order), with the
+//      matrix in registers: a GR x 16 grid of threads, thread (tr, tc) holding the RR x RC elements
+//      (tr + GR a, tc + 16 b); row sums of the symmetric product reduce over the 16 lanes of a row
+//      group (one 16-lane DPP row) by DPP moves, and only vectors pass through LDS.  The
+//      reflectors go to the global workspace.  4 x 16 threads with 12 x 3 slots for blocks of up
+//      to 48 orbitals, 16 x 16 with 6 x 6 up to 96, 32 x 16 with 6 x 12 up to 192 (64, 256, 512).
 //   2. The nocc lowest eigenvalues of T by multisection on Sturm counts: a group of lanes per
 //      eigenvalue evaluates the count at evenly spaced points of its bracket each round.
 //   3. Their eigenvectors by inverse iteration (tridiagonal LU with partial pivoting, three solves
@@ -21,30 +25,20 @@ using namespace pyseqm;
 
 namespace {
 
-constexpr int kNT = 1024;           // threads per workgroup (16 waves)
-constexpr int kNW = kNT / 64;
 constexpr int kEighMaxN = 192;      // largest block this solver takes
 constexpr int kLdsDoubles = 20480;  // 160 KiB of LDS
+constexpr int kRed = 128;           // doubles of reduction scratch (two halves of 64)
 
 // workspace per listed molecule, in doubles, at slot * 4 * nmax^2:
-//   [0, nmax^2)          reflectors: row i holds v_i (v_i[i+1] = 1 implicit) and tau_i at [i][i]
+//   [0, nmax^2)          reflectors: row i holds v_i (v_i[i+1] = 1) and tau_i at [i][i]
 //   [nmax^2, 4 nmax^2)   the U factor of each inverse-iteration LU (reciprocal diagonal and two
 //                        superdiagonals), element k of vector j at k * nmax + j
-__device__ inline size_t tri(int r, int c) { return (size_t)r * (r + 1) / 2 + c; }
+// LDS: eigenvectors n x nocc, then d, e, eigenvalues, two reflector buffers, p (n each), scratch
+__host__ __device__ inline int lds_doubles(int n, int nocc) { return n * nocc + 6 * n + kRed; }
 
-__device__ inline double eig_block_sum(double v, double* red) {
-  v = wave_sum(v);
-  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  __syncthreads();
-  if (lane == 0) red[wid] = v;
-  __syncthreads();
-  double s = 0.0;
-#pragma unroll
-  for (int k = 0; k < kNW; ++k) s += red[k];
-  return s;  // every thread
-}
-
+template <int NT>
 __device__ inline double eig_block_max(double v, double* red) {
+  constexpr int NW = NT / 64;
   for (int o = 1; o < 64; o <<= 1) v = fmax(v, __shfl_xor(v, o));
   const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
   __syncthreads();
@@ -52,11 +46,13 @@ __device__ inline double eig_block_max(double v, double* red) {
   __syncthreads();
   double s = red[0];
 #pragma unroll
-  for (int k = 1; k < kNW; ++k) s = fmax(s, red[k]);
+  for (int k = 1; k < NW; ++k) s = fmax(s, red[k]);
   return s;
 }
 
+template <int NT>
 __device__ inline double eig_block_min(double v, double* red) {
+  constexpr int NW = NT / 64;
   for (int o = 1; o < 64; o <<= 1) v = fmin(v, __shfl_xor(v, o));
   const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
   __syncthreads();
@@ -64,15 +60,8 @@ __device__ inline double eig_block_min(double v, double* red) {
   __syncthreads();
   double s = red[0];
 #pragma unroll
-  for (int k = 1; k < kNW; ++k) s = fmin(s, red[k]);
+  for (int k = 1; k < NW; ++k) s = fmin(s, red[k]);
   return s;
-}
-
-// threads per row of the row-group mappings: the largest power of two with m * tpr <= kNT (<= 64)
-__device__ inline int rows_tpr(int m) {
-  int t = 64;
-  while (t > 1 && m * t > kNT) t >>= 1;
-  return t;
 }
 
 template <int G>
@@ -143,6 +132,26 @@ __device__ inline double group_sum_dyn(double v, int tpr) {
   }
 }
 
+// One DPP lane permutation of a double (two 32-bit DPP moves; VALU, no LDS round trip).
+template <int CTRL>
+__device__ inline double dpp_f64(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Sum over the 16 lanes of a DPP row, in every lane: partners at xor 1, xor 2 (quad
+// permutations), then the other quad (half-row mirror) and the other half-row (row mirror).
+// Additions are commutative, so every lane ends with the same bits.
+__device__ inline double row16_sum(double v) {
+  v += dpp_f64<0xB1>(v);
+  v += dpp_f64<0x4E>(v);
+  v += dpp_f64<0x141>(v);
+  v += dpp_f64<0x140>(v);
+  return v;
+}
+
 // deterministic pseudo-random start vector element in (-1, 1)
 __device__ inline double start_value(int j, int k) {
   unsigned h = (unsigned)j * 2654435761u + (unsigned)k * 40503u + 12345u;
@@ -152,10 +161,13 @@ __device__ inline double start_value(int j, int k) {
   return (double)h * (2.0 / 4294967296.0) - 1.0;
 }
 
-__global__ __launch_bounds__(kNT) void k_eigh_density(pyseqm_layout_t L, const int* __restrict__ idx,
+template <int GR, int GC, int RR, int RC>
+__global__ __launch_bounds__(GR* GC) void k_eigh_density(pyseqm_layout_t L, const int* __restrict__ idx,
                                                        const double* __restrict__ F, double* __restrict__ P,
                                                        double* __restrict__ ws, int nmax,
                                                        int* __restrict__ fail) {
+  constexpr int NT = GR * GC;
+  static_assert(GC == 16, "row groups are DPP rows");
   extern __shared__ double lds[];
   const int slot = blockIdx.x;
   const int m = idx[slot];
@@ -168,130 +180,164 @@ __global__ __launch_bounds__(kNT) void k_eigh_density(pyseqm_layout_t L, const i
   double* U0 = HH + nm2;
   double* U1 = HH + 2 * nm2;
   double* U2 = HH + 3 * nm2;
-  const int big = max(n * (n + 1) / 2, n * nocc);
-  double* A = lds;  // packed lower triangle, then the eigenvectors (n x nocc)
-  double* d = lds + big;
+  double* Z = lds;  // eigenvectors, n x nocc: Z[k * nocc + j]
+  double* d = Z + n * nocc;
   double* e = d + n;
-  double* vb = e + n;
-  double* pb = vb + n;
-  double* lam = pb + n;
-  double* red = lam + n;  // 16
+  double* lam = e + n;
+  double* xs0 = lam + n;  // reflector columns of even / odd steps
+  double* xs1 = xs0 + n;
+  double* pp = xs1 + n;
+  double* red = pp + n;  // kRed
+  double* red1 = red;  // GR <= 64 partials each
+  double* red2 = red + kRed / 2;
   const double* Fm = F + mo;
 
-  // ---- 1. load the lower triangle (waves over rows, lanes over columns)
-  for (int r = tid >> 6; r < n; r += kNW)
-    for (int c = tid & 63; c <= r; c += 64) A[tri(r, c)] = Fm[(size_t)r * n + c];
-  __syncthreads();
-
-  // ---- Householder tridiagonalisation (dsytd2, lower)
+  // ---- 1. Householder tridiagonalisation (dsytd2, lower) in registers, two barriers per step.
+  // Thread (tr, tc) of the GR x GC grid holds A[tr + GR a][tc + GC b], a < RR, b < RC.
+  // Step i: the owners of column i (tc == i mod GC) write x = A[i+1:, i] to LDS with partial sums
+  // of |x[1:]|^2; every thread forms tau, beta and v = (1, x[1:] / (alpha - beta)) at its columns;
+  // p = tau A v (row sums over the lanes of each row group, rows <= i masked by v = 0), with
+  // partial sums of p.v; then A -= v w^T + w v^T, w = p - (tau / 2)(p.v) v, on every slot (v = w
+  // = 0 at indices <= i leave the finished rows and columns alone).
+  const int tr = tid / GC, tc = tid - (tid / GC) * GC;
+  double Ar[RR][RC];
+#pragma unroll
+  for (int a = 0; a < RR; ++a)
+#pragma unroll
+    for (int b = 0; b < RC; ++b) {
+      const int r = tr + GR * a, c = tc + GC * b;
+      Ar[a][b] = (r < n && c < n) ? Fm[(size_t)r * n + c] : 0.0;
+    }
   for (int i = 0; i + 2 < n; ++i) {
-    const int mlen = n - i - 1;
-    for (int k = tid; k < mlen; k += kNT) vb[k] = A[tri(i + 1 + k, i)];
+    double* x = (i & 1) ? xs1 : xs0;
+    const int bi = i / GC, ti = i - bi * GC;
+    if (tc == ti) {
+      double part = 0.0;
+#pragma unroll
+      for (int a = 0; a < RR; ++a) {
+        const int r = tr + GR * a;
+        double val = 0.0;
+#pragma unroll
+        for (int b = 0; b < RC; ++b)
+          if (b == bi) val = Ar[a][b];
+        if (r == i) {
+          d[i] = val;
+        } else if (r > i && r < n) {
+          x[r] = val;
+          if (r > i + 1) part += val * val;
+        }
+      }
+      red1[tr] = part;
+    }
     __syncthreads();
-    double part = 0.0;
-    for (int k = 1 + tid; k < mlen; k += kNT) part += vb[k] * vb[k];
-    const double xn2 = eig_block_sum(part, red);
-    const double alpha = vb[0];
+    double xn2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < GR; ++k) xn2 += red1[k];  // fixed order in every thread
+    const double alpha = x[i + 1];
     double tau = 0.0, beta = alpha, scal = 0.0;
     if (xn2 > 0.0) {
       beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
       tau = (beta - alpha) / beta;
       scal = 1.0 / (alpha - beta);
     }
+    // tau = 0 (column already zero) runs the same steps with p = w = 0: A unchanged
     if (tid == 0) {
-      d[i] = A[tri(i, i)];
       e[i] = beta;
       HH[(size_t)i * nmax + i] = tau;
     }
-    __syncthreads();
-    if (tau == 0.0) continue;  // the same decision in every thread
-    for (int k = tid; k < mlen; k += kNT) {
-      const double v = k == 0 ? 1.0 : vb[k] * scal;
-      vb[k] = v;
-      HH[(size_t)i * nmax + i + 1 + k] = v;
-    }
-    __syncthreads();
-    // p = tau A22 v (row groups of tpr lanes)
-    const int tpr = rows_tpr(mlen);
-    const int ng = kNT / tpr, g = tid / tpr, gl = tid % tpr;
-    for (int r0 = 0; r0 < mlen; r0 += ng) {
-      const int rr = r0 + g;
-      double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-      if (rr < mlen) {
-        const int r = i + 1 + rr;
-        int cc = gl;
-        for (; cc + 3 * tpr < mlen; cc += 4 * tpr) {
+    // v at column c (0 at c <= i); the loops run over this thread's columns outermost so that only
+    // one column's v and w are live beside the register block
+    auto vcol = [&](int c) { return (c <= i || c >= n) ? 0.0 : (c == i + 1 ? 1.0 : x[c] * scal); };
+    double s[RR];
 #pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            const int c = i + 1 + cc + u * tpr;
-            const double a = c <= r ? A[tri(r, c)] : A[tri(c, r)];
-            const double t = a * vb[cc + u * tpr];
-            if (u == 0) s0 += t;
-            else if (u == 1) s1 += t;
-            else if (u == 2) s2 += t;
-            else s3 += t;
-          }
-        }
-        for (; cc < mlen; cc += tpr) {
-          const int c = i + 1 + cc;
-          s0 += (c <= r ? A[tri(r, c)] : A[tri(c, r)]) * vb[cc];
-        }
-      }
-      double s = group_sum_dyn((s0 + s1) + (s2 + s3), tpr);
-      if (rr < mlen && gl == 0) pb[rr] = tau * s;
+    for (int a = 0; a < RR; ++a) s[a] = 0.0;
+#pragma unroll
+    for (int b = 0; b < RC; ++b) {
+      const int c = tc + GC * b;
+      const double vb = vcol(c);
+      if (tr == 0 && c > i && c < n) HH[(size_t)i * nmax + c] = vb;
+#pragma unroll
+      for (int a = 0; a < RR; ++a) s[a] += Ar[a][b] * vb;
     }
+    double q = 0.0;
+#pragma unroll
+    for (int a = 0; a < RR; ++a) {
+      const double sa = row16_sum(s[a]);
+      const int r = tr + GR * a;
+      if (tc == 0 && r > i && r < n) {
+        const double pr = tau * sa;
+        pp[r] = pr;
+        q += pr * (r == i + 1 ? 1.0 : x[r] * scal);
+      }
+    }
+    if (tc == 0) red2[tr] = q;
     __syncthreads();
-    part = 0.0;
-    for (int k = tid; k < mlen; k += kNT) part += pb[k] * vb[k];
-    const double pv = eig_block_sum(part, red);
+    double pv = 0.0;
+#pragma unroll
+    for (int k = 0; k < GR; ++k) pv += red2[k];
     const double a2 = -0.5 * tau * pv;
-    for (int k = tid; k < mlen; k += kNT) pb[k] += a2 * vb[k];  // w
-    __syncthreads();
-    // A22 -= v w^T + w v^T (lower triangle)
-    for (int r0 = 0; r0 < mlen; r0 += ng) {
-      const int rr = r0 + g;
-      if (rr < mlen) {
-        const int r = i + 1 + rr;
-        const double vr = vb[rr], wr = pb[rr];
-        for (int cc = gl; cc <= rr; cc += tpr) A[tri(r, i + 1 + cc)] -= vr * pb[cc] + wr * vb[cc];
+    double vr[RR], wr[RR];
+#pragma unroll
+    for (int a = 0; a < RR; ++a) {
+      const int r = tr + GR * a;
+      vr[a] = 0.0;
+      wr[a] = 0.0;
+      if (r > i && r < n) {
+        vr[a] = r == i + 1 ? 1.0 : x[r] * scal;
+        wr[a] = pp[r] + a2 * vr[a];
       }
     }
-    __syncthreads();
-  }
-  if (tid == 0) {
-    if (n >= 2) {
-      d[n - 2] = A[tri(n - 2, n - 2)];
-      e[n - 2] = A[tri(n - 1, n - 2)];
+#pragma unroll
+    for (int b = 0; b < RC; ++b) {
+      const int c = tc + GC * b;
+      const double vb = vcol(c);
+      const double wb = (c > i && c < n) ? pp[c] + a2 * vb : 0.0;
+#pragma unroll
+      for (int a = 0; a < RR; ++a) Ar[a][b] = fma(-vr[a], wb, fma(-wr[a], vb, Ar[a][b]));
     }
-    d[n - 1] = A[tri(n - 1, n - 1)];
   }
+  // the trailing 2 x 2 block (or the 1 x 1 of a one-orbital block)
+#pragma unroll
+  for (int a = 0; a < RR; ++a)
+#pragma unroll
+    for (int b = 0; b < RC; ++b) {
+      const int r = tr + GR * a, c = tc + GC * b;
+      if (r == n - 2 && c == n - 2) d[n - 2] = Ar[a][b];
+      if (r == n - 1 && c == n - 2) e[n - 2] = Ar[a][b];
+      if (r == n - 1 && c == n - 1) d[n - 1] = Ar[a][b];
+    }
   __syncthreads();
 
   // ---- 2. the nocc lowest eigenvalues: multisection, a group of tb lanes per eigenvalue
   double lo_p = 1.0e300, hi_p = -1.0e300, e2max = 0.0;
-  for (int k = tid; k < n; k += kNT) {
+  for (int k = tid; k < n; k += NT) {
     const double el = k > 0 ? fabs(e[k - 1]) : 0.0, er = k + 1 < n ? fabs(e[k]) : 0.0;
     lo_p = fmin(lo_p, d[k] - el - er);
     hi_p = fmax(hi_p, d[k] + el + er);
     if (k + 1 < n) e2max = fmax(e2max, e[k] * e[k]);
   }
-  double glo = eig_block_min(lo_p, red);
-  double ghi = eig_block_max(hi_p, red);
-  e2max = eig_block_max(e2max, red);
+  double glo = eig_block_min<NT>(lo_p, red);
+  double ghi = eig_block_max<NT>(hi_p, red);
+  e2max = eig_block_max<NT>(e2max, red);
   const double tnorm = fmax(fabs(glo), fabs(ghi));
   const double pivmin = 1.0e-300 * fmax(1.0, e2max);
   glo -= 2.2e-16 * tnorm * 4.0 + 2.0 * pivmin;
   ghi += 2.2e-16 * tnorm * 4.0 + 2.0 * pivmin;
+  // Bracket width at which an eigenvalue is final: 1e-11 |T| (or the rounding floor).  Inverse
+  // iteration only needs the shift that close: with eigenvalues further apart than the cluster
+  // threshold 1e-5 |T| each solve shrinks the other components by 1e-6 or more, and the vectors
+  // of a cluster are orthogonalised as a set.
+  const double btol = 1.0e-11 * tnorm;
   {
     int tb = 64;
-    while (tb > 1 && nocc * tb > kNT) tb >>= 1;
-    const int ngrp = kNT / tb, gi = tid / tb, gl = tid % tb;
+    while (tb > 1 && nocc * tb > NT) tb >>= 1;
+    const int ngrp = NT / tb, gi = tid / tb, gl = tid % tb;
     for (int j0 = 0; j0 < nocc; j0 += ngrp) {
       const int j = j0 + gi;
       if (j < nocc) {  // uniform over the lane group
         double lo = glo, hi = ghi;
         for (int round = 0; round < 64; ++round) {
-          if (!(hi - lo > 4.4e-16 * fmax(fabs(lo), fabs(hi)) + 2.0 * pivmin)) break;
+          if (!(hi - lo > fmax(4.4e-16 * fmax(fabs(lo), fabs(hi)), btol) + 2.0 * pivmin)) break;
           const double x = lo + (hi - lo) * (double)(gl + 1) / (double)(tb + 1);
           const int cnt = (x > lo && x < hi) ? sturm_count(d, e, n, x, pivmin) : -1;
           const double nlo = group_max_dyn(cnt >= 0 && cnt <= j ? x : lo, tb);
@@ -307,9 +353,8 @@ __global__ __launch_bounds__(kNT) void k_eigh_density(pyseqm_layout_t L, const i
   __syncthreads();
 
   // ---- 3. inverse iteration, one thread per eigenvector; vector j in LDS at Z[k * nocc + j]
-  double* Z = A;
   const double tiny = 2.2e-16 * fmax(tnorm, 1.0e-300);
-  for (int j = tid; j < nocc; j += kNT) {
+  for (int j = tid; j < nocc; j += NT) {
     for (int k = 0; k < n; ++k) Z[k * nocc + j] = start_value(j, k);
     const double sig = lam[j];
     for (int rep = 0; rep < 3; ++rep) {
@@ -378,7 +423,7 @@ __global__ __launch_bounds__(kNT) void k_eigh_density(pyseqm_layout_t L, const i
   // ---- 4. clusters (eigenvalues closer than ctol) orthogonalised by the thread of their first
   //         member, two Gram-Schmidt passes; every vector normalised
   const double ctol = 1.0e-5 * fmax(tnorm, 1.0e-300);
-  for (int j = tid; j < nocc; j += kNT) {
+  for (int j = tid; j < nocc; j += NT) {
     if (j > 0 && lam[j] - lam[j - 1] <= ctol) continue;  // inside a cluster: its first thread works
     int jend = j + 1;
     while (jend < nocc && lam[jend] - lam[jend - 1] <= ctol) ++jend;
@@ -401,31 +446,40 @@ __global__ __launch_bounds__(kNT) void k_eigh_density(pyseqm_layout_t L, const i
   }
   __syncthreads();
 
-  // ---- 5. C = Q Z in place (reflectors last to first), tc lanes per column of Z
+  // ---- 5. C = Q Z in place (reflectors last to first), tc lanes per column of Z; the next
+  //         reflector is staged into the other buffer while the current one is applied, so one
+  //         barrier per reflector orders the staging, the column updates and the buffer reuse
   double* C = Z;
   {
     int tc = 64;
-    while (tc > 1 && nocc * tc > kNT) tc >>= 1;
+    while (tc > 1 && nocc * tc > NT) tc >>= 1;
     const int a = tid / tc, cl = tid % tc;
+    double* vc = xs0;
+    double* vn = xs1;
+    if (n >= 3)  // reflector n - 3 has 2 elements
+      for (int k = tid; k < 2; k += NT) vc[k] = HH[(size_t)(n - 3) * nmax + n - 2 + k];
+    __syncthreads();
     for (int i = n - 3; i >= 0; --i) {
       const double tau = HH[(size_t)i * nmax + i];
-      if (tau == 0.0) continue;  // uniform
       const int mlen = n - i - 1;
-      for (int k = tid; k < mlen; k += kNT) vb[k] = HH[(size_t)i * nmax + i + 1 + k];
+      if (i >= 1)
+        for (int k = tid; k < mlen + 1; k += NT) vn[k] = HH[(size_t)(i - 1) * nmax + i + k];
+      if (tau != 0.0 && a < nocc) {  // tau: the same in every thread
+        double sdot = 0.0;
+        for (int k = cl; k < mlen; k += tc) sdot += vc[k] * C[(i + 1 + k) * nocc + a];
+        sdot = tau * group_sum_dyn(sdot, tc);
+        for (int k = cl; k < mlen; k += tc) C[(i + 1 + k) * nocc + a] -= sdot * vc[k];
+      }
       __syncthreads();
-      double sdot = 0.0;
-      if (a < nocc)
-        for (int k = cl; k < mlen; k += tc) sdot += vb[k] * C[(i + 1 + k) * nocc + a];
-      sdot = tau * group_sum_dyn(sdot, tc);
-      if (a < nocc)
-        for (int k = cl; k < mlen; k += tc) C[(i + 1 + k) * nocc + a] -= sdot * vb[k];
-      __syncthreads();
+      double* t = vc;
+      vc = vn;
+      vn = t;
     }
   }
   // ---- P = 2 C C^T, 2 x 2 output blocks per thread over the upper triangle, both triangles written
   double* Pm = P + mo;
   const int nb2 = (n + 1) / 2;
-  for (int t = tid; t < nb2 * nb2; t += kNT) {
+  for (int t = tid; t < nb2 * nb2; t += NT) {
     const int bi = t / nb2, bj = t % nb2;
     if (bj < bi) continue;
     const int r0 = 2 * bi, c0 = 2 * bj;
@@ -466,8 +520,7 @@ extern "C" long long pyseqm_eigh_density_workspace(int nact, int nmax) {
 
 extern "C" int pyseqm_eigh_density_lds_ok(int norb, int nocc) {
   if (norb < 1 || norb > kEighMaxN || nocc < 0 || nocc > norb) return 0;
-  const int big = norb * (norb + 1) / 2 > norb * nocc ? norb * (norb + 1) / 2 : norb * nocc;
-  return big + 5 * norb + 16 <= kLdsDoubles;
+  return lds_doubles(norb, nocc) <= kLdsDoubles;
 }
 
 extern "C" int pyseqm_eigh_density(const pyseqm_layout_t* L, const int* idx, int nact, int nmax, int nocc_max,
@@ -476,8 +529,14 @@ extern "C" int pyseqm_eigh_density(const pyseqm_layout_t* L, const int* idx, int
     return PYSEQM_EINVAL;
   if (!pyseqm_eigh_density_lds_ok(nmax, nocc_max)) return PYSEQM_EINVAL;
   if (nact == 0) return PYSEQM_OK;
-  const int big = nmax * (nmax + 1) / 2 > nmax * nocc_max ? nmax * (nmax + 1) / 2 : nmax * nocc_max;
-  const size_t lds = (size_t)(big + 5 * nmax + 16) * sizeof(double);
-  k_eigh_density<<<nact, kNT, lds, (hipStream_t)stream>>>(*L, idx, F, P, ws, nmax, fail);
+  const size_t lds = (size_t)lds_doubles(nmax, nocc_max) * sizeof(double);
+  hipStream_t s = (hipStream_t)stream;
+  // thread grid GR x 16 with RR x RC register slots per thread: GR RR and 16 RC cover nmax
+  if (nmax <= 48)
+    k_eigh_density<4, 16, 12, 3><<<nact, 64, lds, s>>>(*L, idx, F, P, ws, nmax, fail);
+  else if (nmax <= 96)
+    k_eigh_density<16, 16, 6, 6><<<nact, 256, lds, s>>>(*L, idx, F, P, ws, nmax, fail);
+  else
+    k_eigh_density<32, 16, 6, 12><<<nact, 512, lds, s>>>(*L, idx, F, P, ws, nmax, fail);
   return hipGetLastError() == hipSuccess ? PYSEQM_OK : PYSEQM_ELAUNCH;
 }

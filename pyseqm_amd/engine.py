@@ -39,19 +39,20 @@ MIXED_DIIS_MIN_N = 256
 
 
 # Blocks up to this size (and whose LDS footprint fits, eigh_wg_ok) are diagonalised by the
-# one-workgroup-per-molecule solver of eigh.hip.
+# one-workgroup-per-molecule solver of eigh.hip, which picks its thread grid by the launch's
+# largest block: up to 48, 96 and 192 orbitals (EIGH_WG_CLASSES).
 EIGH_WG_MAX_N = 192
 EIGH_WG_LDS_DOUBLES = 20480
+EIGH_WG_CLASSES = (48, 96, 192)
 
 
 def eigh_wg_ok(norb, nocc):
     """Whether the workgroup eigensolver takes blocks of norb orbitals / nocc occupied orbitals:
-    n <= 192 and the packed lower triangle (or the n x nocc vectors) plus 5 n + 16 doubles fit
-    160 KiB of LDS (pyseqm_eigh_density_lds_ok)."""
+    n <= 192 and the n x nocc eigenvectors plus 6 n + 128 doubles fit 160 KiB of LDS
+    (pyseqm_eigh_density_lds_ok)."""
     n = np.asarray(norb, dtype=np.int64)
     o = np.asarray(nocc, dtype=np.int64)
-    big = np.maximum(n * (n + 1) // 2, n * o)
-    return (n >= 1) & (n <= EIGH_WG_MAX_N) & (o <= n) & (big + 5 * n + 16 <= EIGH_WG_LDS_DOUBLES)
+    return (n >= 1) & (n <= EIGH_WG_MAX_N) & (o <= n) & (n * o + 6 * n + 128 <= EIGH_WG_LDS_DOUBLES)
 
 
 def size_groups(norb, ratio=0.7, min_group=32, floor=EIGH_MIN_N):
@@ -203,9 +204,9 @@ class SCFEngine:
         """FP64 density step by diagonalisation of the active Fock blocks -> Pout (diag.py:110-241).
 
         Blocks of up to EIGH_WG_MAX_N orbitals go to the one-workgroup-per-molecule solver
-        (pyseqm_eigh_density: Householder in LDS, multisection, inverse iteration, cluster
-        Gram-Schmidt, no padding), launched per size group so that each launch reserves LDS for its own largest
-        block.  Larger blocks, and calls that want the eigenvectors themselves, take the batched
+        (pyseqm_eigh_density: Householder with the block in registers, multisection, inverse
+        iteration, cluster Gram-Schmidt, no padding) in one launch.  Larger blocks, and calls that
+        want the eigenvectors themselves, take the batched
         library eigensolver on Gershgorin-padded stacks, again per size group (the reference pads
         every molecule to the batch's largest, which for an uneven batch multiplies the O(n^3) work)."""
         b = self.b
@@ -213,9 +214,11 @@ class SCFEngine:
             idx_h = np.asarray(self._idx_host)
             wg = eigh_wg_ok(b.norb[idx_h], b.nocc[idx_h])
             if wg.any():
+                # one launch, largest blocks first: the long solves start at once and the short
+                # ones fill the CUs behind them (C2's density step: 5.2 ms against 8.4 ms for
+                # launches per size group, each of which waits for its own largest block)
                 ids = idx_h[wg]
-                for sel in size_groups(b.norb[ids], floor=1):
-                    self._eigh_wg(F, ids[sel], Pout)
+                self._eigh_wg(F, ids[np.argsort(-b.norb[ids], kind="stable")], Pout)
             rest = idx_h[~wg]
             if len(rest) == 0:
                 return None

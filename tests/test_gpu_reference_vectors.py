@@ -4,8 +4,9 @@ made by tests/golden/make_reference_vectors.py from lanl/PYSEQM on the CPU).
 Bars: FP64 density (diagonalisation) Hf / Etot / Eelec / Enuc within 1e-6 eV of the reference and the
 same SCF iteration count within one; the mixed-precision SP2 density within 1e-4 eV/atom.  The 128
 alkanes are the first 128 of the 1024-alkane benchmark batch: the test also asserts that this run
-takes the benchmark's density path (one-workgroup-per-molecule eigensolves in more than one size
-group) and that it holds atom pairs beyond the 40-bohr overlap cutoff (hcore.py:83).
+takes the benchmark's density path (one launch of one-workgroup-per-molecule eigensolves over
+every block, largest first) and that it holds atom pairs beyond the 40-bohr overlap cutoff
+(hcore.py:83).
 """
 import numpy as np
 import pytest
@@ -41,7 +42,8 @@ def test_fp64_engine_matches_reference(name):
 
 
 def test_c2_slice_takes_the_benchmark_path():
-    """First 128 of the 1024 alkanes: size-grouped batched eigensolves and >40-bohr pairs."""
+    """First 128 of the 1024 alkanes: one workgroup-eigensolver launch per SCF iteration, and
+    >40-bohr pairs."""
     c = CASES["am1_alkanes_c2_first128"]
     sp, x = case_input(c)
     # atom pairs beyond the overlap cutoff (their overlap, hence beta term, is zero)
@@ -52,12 +54,9 @@ def test_c2_slice_takes_the_benchmark_path():
         far += int((d > 40.0).sum())
     assert far > 0
     eng, r, en, _, _ = _run(c)
-    groups = E.size_groups(eng.b.norb, floor=1)
-    assert len(groups) >= 2
-    first = list(eng.eigh_log)[:len(groups)]  # the first SCF iteration solves every group
-    assert all(kind == "workgroup" for _, _, kind in first), first  # unpadded workgroup eigensolves
-    assert sorted(n for _, n, _ in first) == sorted(int(eng.b.norb[g].max()) for g in groups), first
-    assert sum(nb for nb, _, _ in first) == eng.b.nmol
+    first = eng.eigh_log[0]  # the first SCF iteration solves every block in one launch
+    assert first == (eng.b.nmol, int(eng.b.norb.max()), "workgroup"), first
+    assert all(kind == "workgroup" for _, _, kind in eng.eigh_log)
     assert int(eng.eigh_fail.item()) == 0
     np.testing.assert_allclose(en["Hf"], c["Hf"], rtol=0, atol=1e-6)
 
