@@ -64,72 +64,28 @@ __device__ inline double eig_block_min(double v, double* red) {
   return s;
 }
 
-template <int G>
-__device__ inline double group_max_g(double v) {
-#pragma unroll
-  for (int o = 1; o < G; o <<= 1) v = fmax(v, __shfl_xor(v, o));
-  return v;
-}
-template <int G>
-__device__ inline double group_min_g(double v) {
-#pragma unroll
-  for (int o = 1; o < G; o <<= 1) v = fmin(v, __shfl_xor(v, o));
-  return v;
-}
-__device__ inline double group_max_dyn(double v, int t) {
-  switch (t) {
-    case 64: return group_max_g<64>(v);
-    case 32: return group_max_g<32>(v);
-    case 16: return group_max_g<16>(v);
-    case 8: return group_max_g<8>(v);
-    case 4: return group_max_g<4>(v);
-    case 2: return group_max_g<2>(v);
-    default: return v;
-  }
-}
-__device__ inline double group_min_dyn(double v, int t) {
-  switch (t) {
-    case 64: return group_min_g<64>(v);
-    case 32: return group_min_g<32>(v);
-    case 16: return group_min_g<16>(v);
-    case 8: return group_min_g<8>(v);
-    case 4: return group_min_g<4>(v);
-    case 2: return group_min_g<2>(v);
-    default: return v;
-  }
+// 1 / q to full FP64 accuracy without the scaled division sequence: v_rcp_f64 and two Newton
+// steps (q is kept away from zero and overflow by the caller's pivmin)
+__device__ inline double rcp_f64(double q) {
+  double r = __builtin_amdgcn_rcp(q);
+  r = fma(fma(-q, r, 1.0), r, r);
+  r = fma(fma(-q, r, 1.0), r, r);
+  return r;
 }
 
-// number of eigenvalues of the tridiagonal (d, e) below x (Sturm count of the LDL^T pivots)
-__device__ inline int sturm_count(const double* d, const double* e, int n, double x, double pivmin) {
+// number of eigenvalues of the tridiagonal (d, e) below x: the negative pivots of the LDL^T
+// factorisation of T - x I (e2[k] = e[k]^2)
+__device__ inline int sturm_count(const double* d, const double* e2, int n, double x, double pivmin) {
   int cnt = 0;
   double q = d[0] - x;
   if (fabs(q) < pivmin) q = -pivmin;
   cnt += q < 0.0;
   for (int k = 1; k < n; ++k) {
-    q = (d[k] - x) - (e[k - 1] * e[k - 1]) / q;
+    q = (d[k] - x) - e2[k - 1] * rcp_f64(q);
     if (fabs(q) < pivmin) q = -pivmin;
     cnt += q < 0.0;
   }
   return cnt;
-}
-
-template <int G>
-__device__ inline double group_reduce(double v) {
-#pragma unroll
-  for (int o = 1; o < G; o <<= 1) v += __shfl_xor(v, o);
-  return v;
-}
-
-__device__ inline double group_sum_dyn(double v, int tpr) {
-  switch (tpr) {
-    case 64: return group_reduce<64>(v);
-    case 32: return group_reduce<32>(v);
-    case 16: return group_reduce<16>(v);
-    case 8: return group_reduce<8>(v);
-    case 4: return group_reduce<4>(v);
-    case 2: return group_reduce<2>(v);
-    default: return v;
-  }
 }
 
 // One DPP lane permutation of a double (two 32-bit DPP moves; VALU, no LDS round trip).
@@ -149,6 +105,36 @@ __device__ inline double row16_sum(double v) {
   v += dpp_f64<0x4E>(v);
   v += dpp_f64<0x141>(v);
   v += dpp_f64<0x140>(v);
+  return v;
+}
+
+// Sum / max / min over aligned groups of t lanes (t a power of two, the same in every lane):
+// DPP within a 16-lane row, shuffles across rows.
+__device__ inline double group_sum_t(double v, int t) {
+  if (t >= 2) v += dpp_f64<0xB1>(v);
+  if (t >= 4) v += dpp_f64<0x4E>(v);
+  if (t >= 8) v += dpp_f64<0x141>(v);
+  if (t >= 16) v += dpp_f64<0x140>(v);
+  if (t >= 32) v += __shfl_xor(v, 16);
+  if (t >= 64) v += __shfl_xor(v, 32);
+  return v;
+}
+__device__ inline double group_max_t(double v, int t) {
+  if (t >= 2) v = fmax(v, dpp_f64<0xB1>(v));
+  if (t >= 4) v = fmax(v, dpp_f64<0x4E>(v));
+  if (t >= 8) v = fmax(v, dpp_f64<0x141>(v));
+  if (t >= 16) v = fmax(v, dpp_f64<0x140>(v));
+  if (t >= 32) v = fmax(v, __shfl_xor(v, 16));
+  if (t >= 64) v = fmax(v, __shfl_xor(v, 32));
+  return v;
+}
+__device__ inline double group_min_t(double v, int t) {
+  if (t >= 2) v = fmin(v, dpp_f64<0xB1>(v));
+  if (t >= 4) v = fmin(v, dpp_f64<0x4E>(v));
+  if (t >= 8) v = fmin(v, dpp_f64<0x141>(v));
+  if (t >= 16) v = fmin(v, dpp_f64<0x140>(v));
+  if (t >= 32) v = fmin(v, __shfl_xor(v, 16));
+  if (t >= 64) v = fmin(v, __shfl_xor(v, 32));
   return v;
 }
 
@@ -314,7 +300,10 @@ __global__ __launch_bounds__(GR* GC) void k_eigh_density(pyseqm_layout_t L, cons
     const double el = k > 0 ? fabs(e[k - 1]) : 0.0, er = k + 1 < n ? fabs(e[k]) : 0.0;
     lo_p = fmin(lo_p, d[k] - el - er);
     hi_p = fmax(hi_p, d[k] + el + er);
-    if (k + 1 < n) e2max = fmax(e2max, e[k] * e[k]);
+    if (k + 1 < n) {
+      pp[k] = e[k] * e[k];  // e^2 for the Sturm counts (pp is free after the reduction)
+      e2max = fmax(e2max, pp[k]);
+    }
   }
   double glo = eig_block_min<NT>(lo_p, red);
   double ghi = eig_block_max<NT>(hi_p, red);
@@ -339,9 +328,9 @@ __global__ __launch_bounds__(GR* GC) void k_eigh_density(pyseqm_layout_t L, cons
         for (int round = 0; round < 64; ++round) {
           if (!(hi - lo > fmax(4.4e-16 * fmax(fabs(lo), fabs(hi)), btol) + 2.0 * pivmin)) break;
           const double x = lo + (hi - lo) * (double)(gl + 1) / (double)(tb + 1);
-          const int cnt = (x > lo && x < hi) ? sturm_count(d, e, n, x, pivmin) : -1;
-          const double nlo = group_max_dyn(cnt >= 0 && cnt <= j ? x : lo, tb);
-          const double nhi = group_min_dyn(cnt > j ? x : hi, tb);
+          const int cnt = (x > lo && x < hi) ? sturm_count(d, pp, n, x, pivmin) : -1;
+          const double nlo = group_max_t(cnt >= 0 && cnt <= j ? x : lo, tb);
+          const double nhi = group_min_t(cnt > j ? x : hi, tb);
           if (nlo == lo && nhi == hi) break;  // no representable interior point left
           lo = nlo;
           hi = nhi;
@@ -368,7 +357,7 @@ __global__ __launch_bounds__(GR* GC) void k_eigh_density(pyseqm_layout_t L, cons
         const size_t o = (size_t)k * nmax + j;
         if (fabs(diag) >= fabs(sub)) {
           const double piv = fabs(diag) < tiny ? copysign(tiny, diag) : diag;
-          const double rp = 1.0 / piv;
+          const double rp = rcp_f64(piv);
           const double mlt = sub * rp;
           U0[o] = rp;
           U1[o] = sup;
@@ -378,7 +367,7 @@ __global__ __launch_bounds__(GR* GC) void k_eigh_density(pyseqm_layout_t L, cons
           diag = nd - mlt * sup;
           sup = ns;
         } else {
-          const double rp = 1.0 / sub;
+          const double rp = rcp_f64(sub);
           const double mlt = diag * rp;
           U0[o] = rp;
           U1[o] = nd;
@@ -392,27 +381,45 @@ __global__ __launch_bounds__(GR* GC) void k_eigh_density(pyseqm_layout_t L, cons
       const double rlast = 1.0 / (fabs(diag) < tiny ? copysign(tiny, diag) : diag);
       double x1 = yk * rlast, x2 = 0.0, xmax = fabs(x1);
       Z[(n - 1) * nocc + j] = x1;
-      // back substitution; U of row k - 1 is loaded while row k is solved
-      double u0n = 0.0, u1n = 0.0, u2n = 0.0;
-      if (n >= 2) {
-        const size_t o = (size_t)(n - 2) * nmax + j;
-        u0n = U0[o];
-        u1n = U1[o];
-        u2n = U2[o];
+      // back substitution in blocks of kPF rows: the U factors of the next block are loaded while
+      // this block is solved (the loads come from L2; one row ahead left each row waiting on one)
+      constexpr int kPF = 8;
+      double c0[kPF], c1[kPF], c2[kPF];
+#pragma unroll
+      for (int t = 0; t < kPF; ++t) {
+        const int kk = n - 2 - t;
+        const size_t o = (size_t)(kk >= 0 ? kk : 0) * nmax + j;
+        c0[t] = U0[o];
+        c1[t] = U1[o];
+        c2[t] = U2[o];
       }
-      for (int k = n - 2; k >= 0; --k) {
-        const double u0 = u0n, u1 = u1n, u2 = u2n;
-        if (k >= 1) {
-          const size_t o = (size_t)(k - 1) * nmax + j;
-          u0n = U0[o];
-          u1n = U1[o];
-          u2n = U2[o];
+      for (int k0 = n - 2; k0 >= 0; k0 -= kPF) {
+        double n0[kPF], n1[kPF], n2[kPF];
+#pragma unroll
+        for (int t = 0; t < kPF; ++t) {
+          const int kk = k0 - kPF - t;
+          const size_t o = (size_t)(kk >= 0 ? kk : 0) * nmax + j;
+          n0[t] = U0[o];
+          n1[t] = U1[o];
+          n2[t] = U2[o];
         }
-        const double x = (Z[k * nocc + j] - u1 * x1 - u2 * x2) * u0;
-        Z[k * nocc + j] = x;
-        xmax = fmax(xmax, fabs(x));
-        x2 = x1;
-        x1 = x;
+#pragma unroll
+        for (int t = 0; t < kPF; ++t) {
+          const int k = k0 - t;
+          if (k >= 0) {
+            const double x = (Z[k * nocc + j] - c1[t] * x1 - c2[t] * x2) * c0[t];
+            Z[k * nocc + j] = x;
+            xmax = fmax(xmax, fabs(x));
+            x2 = x1;
+            x1 = x;
+          }
+        }
+#pragma unroll
+        for (int t = 0; t < kPF; ++t) {
+          c0[t] = n0[t];
+          c1[t] = n1[t];
+          c2[t] = n2[t];
+        }
       }
       const double sc = xmax > 0.0 ? 1.0 / xmax : 1.0;
       for (int k = 0; k < n; ++k) Z[k * nocc + j] *= sc;
@@ -467,7 +474,7 @@ __global__ __launch_bounds__(GR* GC) void k_eigh_density(pyseqm_layout_t L, cons
       if (tau != 0.0 && a < nocc) {  // tau: the same in every thread
         double sdot = 0.0;
         for (int k = cl; k < mlen; k += tc) sdot += vc[k] * C[(i + 1 + k) * nocc + a];
-        sdot = tau * group_sum_dyn(sdot, tc);
+        sdot = tau * group_sum_t(sdot, tc);
         for (int k = cl; k < mlen; k += tc) C[(i + 1 + k) * nocc + a] -= sdot * vc[k];
       }
       __syncthreads();
