@@ -239,11 +239,13 @@ __global__ __launch_bounds__(GR* GC) void k_eigh_density(pyseqm_layout_t L, cons
     for (int a = 0; a < RR; ++a) s[a] = 0.0;
 #pragma unroll
     for (int b = 0; b < RC; ++b) {
+      if (GC * b + GC - 1 <= i) continue;  // every column of the slot is finished: v = 0 (uniform)
       const int c = tc + GC * b;
       const double vb = vcol(c);
       if (tr == 0 && c > i && c < n) HH[(size_t)i * nmax + c] = vb;
 #pragma unroll
-      for (int a = 0; a < RR; ++a) s[a] += Ar[a][b] * vb;
+      for (int a = 0; a < RR; ++a)
+        if (GR * a + GR - 1 > i) s[a] += Ar[a][b] * vb;  // rows of finished slots are not needed
     }
     double q = 0.0;
 #pragma unroll
@@ -275,11 +277,13 @@ __global__ __launch_bounds__(GR* GC) void k_eigh_density(pyseqm_layout_t L, cons
     }
 #pragma unroll
     for (int b = 0; b < RC; ++b) {
+      if (GC * b + GC - 1 <= i) continue;  // v = w = 0 on every column of the slot
       const int c = tc + GC * b;
       const double vb = vcol(c);
       const double wb = (c > i && c < n) ? pp[c] + a2 * vb : 0.0;
 #pragma unroll
-      for (int a = 0; a < RR; ++a) Ar[a][b] = fma(-vr[a], wb, fma(-wr[a], vb, Ar[a][b]));
+      for (int a = 0; a < RR; ++a)
+        if (GR * a + GR - 1 > i) Ar[a][b] = fma(-vr[a], wb, fma(-wr[a], vb, Ar[a][b]));
     }
   }
   // the trailing 2 x 2 block (or the 1 x 1 of a one-orbital block)
@@ -453,35 +457,59 @@ __global__ __launch_bounds__(GR* GC) void k_eigh_density(pyseqm_layout_t L, cons
   }
   __syncthreads();
 
-  // ---- 5. C = Q Z in place (reflectors last to first), tc lanes per column of Z; the next
-  //         reflector is staged into the other buffer while the current one is applied, so one
-  //         barrier per reflector orders the staging, the column updates and the buffer reuse
+  // ---- 5. C = Q Z in registers: thread (tr, tc) holds C[tc + 16 b][tr + GR c] (b < RC rows
+  //         over the lanes of its DPP row, c < RR columns), so each column's product v^T C reduces
+  //         inside one DPP row and the reflectors (last to first) need no barrier; each thread
+  //         reads its RC elements of the next reflector from the workspace while applying one.
   double* C = Z;
   {
-    int tc = 64;
-    while (tc > 1 && nocc * tc > NT) tc >>= 1;
-    const int a = tid / tc, cl = tid % tc;
-    double* vc = xs0;
-    double* vn = xs1;
-    if (n >= 3)  // reflector n - 3 has 2 elements
-      for (int k = tid; k < 2; k += NT) vc[k] = HH[(size_t)(n - 3) * nmax + n - 2 + k];
-    __syncthreads();
-    for (int i = n - 3; i >= 0; --i) {
-      const double tau = HH[(size_t)i * nmax + i];
-      const int mlen = n - i - 1;
-      if (i >= 1)
-        for (int k = tid; k < mlen + 1; k += NT) vn[k] = HH[(size_t)(i - 1) * nmax + i + k];
-      if (tau != 0.0 && a < nocc) {  // tau: the same in every thread
-        double sdot = 0.0;
-        for (int k = cl; k < mlen; k += tc) sdot += vc[k] * C[(i + 1 + k) * nocc + a];
-        sdot = tau * group_sum_t(sdot, tc);
-        for (int k = cl; k < mlen; k += tc) C[(i + 1 + k) * nocc + a] -= sdot * vc[k];
+#pragma unroll
+    for (int c = 0; c < RR; ++c)
+#pragma unroll
+      for (int b = 0; b < RC; ++b) {
+        const int k = tc + 16 * b, a = tr + GR * c;
+        Ar[c][b] = (k < n && a < nocc) ? Z[k * nocc + a] : 0.0;
       }
-      __syncthreads();
-      double* t = vc;
-      vc = vn;
-      vn = t;
+    double vcur[RC], vnext[RC];
+    double tcur = 0.0, tnext = 0.0;
+    auto load_v = [&](int i, double* v, double& t) {
+      t = HH[(size_t)i * nmax + i];
+#pragma unroll
+      for (int b = 0; b < RC; ++b) {
+        const int k = tc + 16 * b;
+        v[b] = (k > i && k < n) ? HH[(size_t)i * nmax + k] : 0.0;
+      }
+    };
+    if (n >= 3) load_v(n - 3, vcur, tcur);
+    for (int i = n - 3; i >= 0; --i) {
+      if (i >= 1) load_v(i - 1, vnext, tnext);
+      if (tcur != 0.0) {  // the same in every thread
+#pragma unroll
+        for (int c = 0; c < RR; ++c) {
+          if (GR * c >= nocc) continue;  // no column of this slot exists (uniform)
+          double sd = 0.0;
+#pragma unroll
+          for (int b = 0; b < RC; ++b)
+            if (16 * b + 15 > i) sd += Ar[c][b] * vcur[b];  // slots wholly at rows <= i have v = 0
+          sd = tcur * row16_sum(sd);
+#pragma unroll
+          for (int b = 0; b < RC; ++b)
+            if (16 * b + 15 > i) Ar[c][b] = fma(-vcur[b], sd, Ar[c][b]);
+        }
+      }
+#pragma unroll
+      for (int b = 0; b < RC; ++b) vcur[b] = vnext[b];
+      tcur = tnext;
     }
+    __syncthreads();  // every thread has read Z
+#pragma unroll
+    for (int c = 0; c < RR; ++c)
+#pragma unroll
+      for (int b = 0; b < RC; ++b) {
+        const int k = tc + 16 * b, a = tr + GR * c;
+        if (k < n && a < nocc) C[k * nocc + a] = Ar[c][b];
+      }
+    __syncthreads();
   }
   // ---- P = 2 C C^T, 2 x 2 output blocks per thread over the upper triangle, both triangles written
   double* Pm = P + mo;
