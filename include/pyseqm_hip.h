@@ -287,6 +287,44 @@ int pyseqm_sp2_mcweeny(const pyseqm_layout_t* L, const int* idx, int nact, int n
                        const int* xld, const double* state, float* Xa, float* Xb, double* Y2, double* P,
                        void* stream);
 
+/* ------------------------------------------------------------------ thermal density / KSA-XL-BOMD
+ * Every eigenpair of the listed blocks (the eigen-decomposition of Fermi_Q, fermi_q.py:27-30, via
+ * sym_eig_trunc(eig_only=True), diag.py:110-241): Q in the packed layout (column j = eigenvector
+ * j, ascending eigenvalues), evals[orb_off[m] + j].  The workgroup eigensolver with all n vectors:
+ * nmax <= 192 and pyseqm_eigh_density_lds_ok(nmax, nmax); ws as for pyseqm_eigh_density. */
+int pyseqm_eigh_vectors(const pyseqm_layout_t* L, const int* idx, int nact, int nmax, const double* F,
+                        const long long* orb_off, double* evals, double* Q, double* ws, int* fail, void* stream);
+/* Fermi_Q (fermi_q.py:8-72) after the eigen-decomposition: the chemical potential by Newton steps
+ * from the HOMO/LUMO midpoint, stopping at the first step at which every listed molecule has
+ * |nocc - sum f| <= 1e-9 (at most 64), occupations occ[orb_off[m] + j] = 1 / (1 + exp((e - mu) / kB T)),
+ * mu[m], D = 2 Q diag(occ) Q^T (packed) and the entropy S[m] = -kB sum [f ln f + (1-f) ln(1-f)]
+ * over 1e-14 < f < 1 - 1e-14.  Requires 1 <= nocc < norb <= 256 per molecule; work holds
+ * pyseqm_fermi_workspace(nact) bytes. */
+long long pyseqm_fermi_workspace(int nact);
+int pyseqm_fermi_density(const pyseqm_layout_t* L, const int* idx, int nact, int nmax, const long long* orb_off,
+                         const double* evals, const double* Q, double T, double kB, double* occ, double* mu,
+                         double* D, double* S, void* work, void* stream);
+/* C = op(A) op(B) on every listed molecule's packed n x n block (ta / tb: transpose), FP64: the
+ * eigenbasis transforms Q^T X Q and Q X Q^T of Canon_DM_PRT (canon_dm_prt.py:17, 37). */
+int pyseqm_block_gemm(const pyseqm_layout_t* L, const int* idx, int nact, int nmax, int ta, int tb,
+                      const double* A, const double* B, double* C, void* stream);
+/* The canonical density-matrix perturbation recursion of Canon_DM_PRT (canon_dm_prt.py:18-35) in
+ * place on Y = Q^T FO1 Q (eigenbasis; eigenvalues evals, chemical potential mu[m], `steps`
+ * recursion steps), then the trace-keeping chemical-potential shift. */
+int pyseqm_canon_prt(const pyseqm_layout_t* L, const int* idx, int nact, int nmax, const long long* orb_off,
+                     const double* evals, const double* mu, double T, double kB, int steps, double* Y, void* stream);
+/* The Krylov fit of EnergyXL's max_rank branch (xlbomd.py:260-339): out[m] = sum of A * B over the
+ * block (Frobenius product); Y <- Y + s c_m X (mode 0), Y / sqrt(c_m) (1), s (Y - X) (2), s c_m X (3),
+ * Y + s X (4) with c_m = c[m * cstride]; alpha = O^{-1} rhs per molecule (O at m * R^2, rhs and
+ * alpha at m * R, R = pyseqm_ksa_max_rank(), r <= R; partial pivoting). */
+int pyseqm_block_dot(const pyseqm_layout_t* L, const int* idx, int nact, const double* A, const double* B,
+                     double* out, void* stream);
+int pyseqm_block_axpy(const pyseqm_layout_t* L, const int* idx, int nact, int nmax, int mode, double s,
+                      const double* c, int cstride, const double* X, double* Y, void* stream);
+int pyseqm_ksa_max_rank(void);
+int pyseqm_small_solve(const int* idx, int nact, int r, const double* O, const double* rhs, double* alpha,
+                       void* stream);
+
 /* ------------------------------------------------------------------ XL-BOMD / MD
  * Density propagation of XL_BOMD._propagate_P (seqm/MolecularDynamics.py:1418-1427) plus the
  * ring update Pt[m-1-cindx] = P (1461) over `total` packed elements:

@@ -1143,3 +1143,135 @@ def xl_propagate(D, P, Pt, k, step, c=0.95):
         Pn = Pn + cs[i] * hist[i]
     Pt[m - 1 - cindx] = Pn
     return Pn, Pt
+
+
+# ------------------------------------------------ thermal density / KSA-XL-BOMD
+KB_EV = 8.61739e-5          # Boltzmann constant, eV/K (seqm/dynamics/xlbomd.py:214)
+CANON_DM_PRT_ITER = 10      # recursion steps of Canon_DM_PRT in EnergyXL (seqm/dynamics/xlbomd.py:55)
+FERMI_MAX_ITER = 64         # fermi_q.py:12
+FERMI_OCC_TOL = 1.0e-9      # fermi_q.py:14 (float64)
+FERMI_ENTROPY_EPS = 1.0e-14
+FERMI_TINY = 1.0e-30
+
+
+def _sigmoid(x):
+    with np.errstate(over="ignore"):
+        return 1.0 / (1.0 + np.exp(-x))
+
+
+def fermi_q(b, F, T, kB=KB_EV):
+    """Fermi_Q (seqm/seqm_functions/fermi_q.py:8-72) on every molecule's block of real orbitals:
+    all eigenpairs (e ascending, Q columns), the chemical potential by Newton steps from the
+    HOMO/LUMO midpoint with the reference's batch-wide stop (every |nocc - sum f| <= 1e-9, at most
+    64 passes; sum f counts occupations per orbital, f = sigmoid(-(e - mu) / kB T)), the density
+    D = 2 Q diag(f) Q^T (padded) and the entropy S = -kB sum [f ln f + (1-f) ln(1-f)] over
+    1e-14 < f < 1 - 1e-14.  Returns D, S (nmol,), and per molecule lists e, Q, f, mu."""
+    beta = 1.0 / (kB * T)
+    es, Qs, idxs = [], [], []
+    for mo in range(b.nmol):
+        idx = orbital_index(b, mo)
+        w, V = np.linalg.eigh(F[mo][np.ix_(idx, idx)])
+        es.append(w)
+        Qs.append(V)
+        idxs.append(idx)
+    N = b.nocc.astype(np.float64)
+    mu = np.array([0.5 * (es[m][int(N[m]) - 1] + es[m][int(N[m])]) for m in range(b.nmol)])
+    for _ in range(FERMI_MAX_ITER):
+        fs = [_sigmoid(-beta * (es[m] - mu[m])) for m in range(b.nmol)]
+        occ = np.array([f.sum() for f in fs])
+        docc = np.array([max((beta * f * (1.0 - f)).sum(), FERMI_TINY) for f in fs])
+        err = np.abs(N - occ)
+        if (err <= FERMI_OCC_TOL).all():
+            break
+        mu = mu + (N - occ) / docc
+    D = np.zeros_like(F)
+    S = np.zeros(b.nmol)
+    for m in range(b.nmol):
+        f, V, idx = fs[m], Qs[m], idxs[m]
+        D[m][np.ix_(idx, idx)] = 2.0 * (V * f) @ V.T
+        ok = (f > FERMI_ENTROPY_EPS) & (1.0 - f > FERMI_ENTROPY_EPS)
+        p = np.where(ok, f, 0.5)
+        S[m] = (-kB * (p * np.log(p) + (1.0 - p) * np.log(1.0 - p)) * ok).sum()
+    return D, S, es, Qs, fs, mu
+
+
+def canon_dm_prt(FO1, T, Q, e, mu0, m=CANON_DM_PRT_ITER, kB=KB_EV):
+    """Canon_DM_PRT (seqm/seqm_functions/canon_dm_prt.py:6-39) on one block of real orbitals: the
+    first-order density response to the perturbation FO1 by the recursive Fermi-operator
+    expansion in the eigenbasis of the unperturbed Fock matrix (e, Q; chemical potential mu0),
+    m recursion steps, with the chemical-potential shift that keeps the trace.  Returns the block
+    P1 = Q P1' Q^T (per orbital occupation, without the factor 2 of the density)."""
+    beta = 1.0 / (kB * T)
+    cnst = 2.0 ** (-2 - m) * beta
+    X = Q.T @ FO1 @ Q
+    p0 = 0.5 - cnst * (e - mu0)
+    P1 = -cnst * X
+    for _ in range(m):
+        p02 = p0 * p0
+        dX1 = p0[:, None] * P1 + P1 * p0[None, :]
+        iD0 = 1.0 / (2.0 * (p02 - p0) + 1.0)
+        p0 = iD0 * p02
+        P1 = iD0[:, None] * (dX1 + 2.0 * (P1 - dX1) * p0[None, :])
+    dpdmu = beta * p0 * (1.0 - p0)
+    dmu1 = -np.trace(P1) / dpdmu.sum()
+    P1 = P1 + np.diag(dpdmu) * dmu1
+    return Q @ P1 @ Q.T
+
+
+def g_matrix(b, V, T, G1=None):
+    """G(V): the two-electron part of the Fock matrix of a density V, no core Hamiltonian
+    (seqm/seqm_functions/G_XL_LR.py:7-908): fock with H = 0."""
+    return fock(b, V, np.zeros_like(V), T, G1)
+
+
+def ksa_rank_kernel(b, D, P, Qs, es, mus, T_el, Tt, rank, err_threshold, kB=KB_EV, K0=1.0):
+    """The rank-m Krylov approximation of the XL-BOMD kernel in EnergyXL's max_rank branch
+    (seqm/dynamics/xlbomd.py:243-339): Krylov vectors V_k (modified Gram-Schmidt in the Frobenius
+    product, normalised), their responses W_k = K0 (P1(G(V_k)) - V_k) with P1 from canon_dm_prt,
+    the least-squares fit of K0 (D - P) by the W_k (batch-wide loop while k < rank - 1 and the
+    largest relative residual exceeds err_threshold).  Returns dP2dt2 = -sum_k alpha_k V_k (padded),
+    the residuals and the rank used."""
+    fro = lambda A, B: np.einsum("nij,nij->n", A, B)
+    dDS = K0 * (D - P)
+    dDS_norm = np.sqrt(fro(dDS, dDS))
+    Vs, Ws = [], []
+    dW = dDS
+    k = -1
+    Error = np.full(b.nmol, 10.0)
+    idxs = [orbital_index(b, mo) for mo in range(b.nmol)]
+
+    def fit(r):
+        alpha = np.zeros((b.nmol, r))
+        for mo in range(b.nmol):
+            O = np.array([[np.sum(Ws[i][mo] * Ws[j][mo]) for j in range(r)] for i in range(r)])
+            rhs = np.array([np.sum(Ws[i][mo] * dDS[mo]) for i in range(r)])
+            alpha[mo] = np.linalg.solve(O, rhs)
+        return alpha
+
+    while k < rank - 1 and Error.max() > err_threshold:
+        k += 1
+        Vk = dW.copy()
+        for j in range(k):
+            Vk = Vk - fro(Vk, Vs[j])[:, None, None] * Vs[j]
+        Vk = Vk / np.sqrt(fro(Vk, Vk))[:, None, None]
+        Vs.append(Vk)
+        FO1 = g_matrix(b, Vk, Tt)
+        PO1 = np.zeros_like(Vk)
+        for mo in range(b.nmol):
+            ix = np.ix_(idxs[mo], idxs[mo])
+            PO1[mo][ix] = canon_dm_prt(FO1[mo][ix], T_el, Qs[mo], es[mo], mus[mo], kB=kB)
+        Wk = K0 * (PO1 - Vk)
+        Ws.append(Wk)
+        dW = Wk
+        alpha = fit(k + 1)
+        res = np.einsum("nr,rnij->nij", alpha, np.array(Ws)) - dDS
+        Error = np.sqrt(fro(res, res)) / dDS_norm
+    alpha = fit(k + 1)
+    dP2dt2 = -np.einsum("nr,rnij->nij", alpha, np.array(Vs))
+    return dP2dt2, Error, k + 1
+
+
+def ksa_xl_propagate(dP2dt2, P, Pt, k, step):
+    """KSA_XL_BOMD._propagate_P (seqm/MolecularDynamics.py:1615-1619): P(n+1) = kappa (dP2dt2 + P)
+    + the same history terms as XL_BOMD: xl_propagate with D' = dP2dt2 + P and no mixing."""
+    return xl_propagate(dP2dt2 + P, P, Pt, k, step, c=1.0)

@@ -149,11 +149,11 @@ class Molecular_Dynamics_Basic(torch.nn.Module):
 class XL_BOMD(Molecular_Dynamics_Basic):
     """XL-BOMD (MolecularDynamics.py:1373-1605) with the trajectory batch resident on the device."""
 
+    KSA_PROPAGATION = False  # KSA_XL_BOMD propagates P with the Krylov kernel dP2dt2
+
     def __init__(self, damp=None, xl_bomd_params=dict(), *args, **kwargs):
         if damp is not None:
             raise NotImplementedError("the Langevin-damped XL-BOMD variant is not part of this path")
-        if "max_rank" in xl_bomd_params:
-            raise NotImplementedError("KSA-XL-BOMD (max_rank) is not part of this path")
         self.k = int(xl_bomd_params["k"])
         self.xl_bomd_params = xl_bomd_params
         self.damp = damp
@@ -172,8 +172,22 @@ class XL_BOMD(Molecular_Dynamics_Basic):
         sp2 = self.seqm_parameters.get("sp2", [False])
         self.traj = XLTrajectories(eng, self.k, self.timestep, b.atoms_from_padded(molecule.coordinates.detach()),
                                    b.atoms_from_padded(molecule.velocities.detach()), b.from_padded(molecule.dm),
-                                   density="sp2" if sp2[0] else "eigh")
+                                   density="sp2" if sp2[0] else "eigh", ksa=self._ksa_params())
         self.traj.set_forces(b.atoms_from_padded(molecule.force.detach()))
+
+    def _ksa_params(self):
+        """The max_rank branch of EnergyXL (xlbomd.py:201-341): the Fermi density at T_el and the
+        rank-m Krylov kernel; KSA_XL_BOMD also propagates with it."""
+        p = self.xl_bomd_params
+        if "max_rank" not in p:
+            if self.KSA_PROPAGATION:
+                raise ValueError("KSA_XL_BOMD needs xl_bomd_params['max_rank'] (and 'T_el', 'err_threshold')")
+            return None
+        if self.seqm_parameters.get("sp2", [False])[0]:
+            raise NotImplementedError("the Fermi density of the max_rank branch comes from the eigen-decomposition; "
+                                      "sp2 is not used with it")
+        return {"max_rank": int(p["max_rank"]), "err_threshold": float(p.get("err_threshold", 0.0)),
+                "T_el": float(p["T_el"]), "propagate": bool(self.KSA_PROPAGATION)}
 
     def _do_integrator_step(self, i, molecule, learned_parameters, **kwargs):
         tr = self.traj
@@ -186,7 +200,8 @@ class XL_BOMD(Molecular_Dynamics_Basic):
         """Write the trajectory batch after `step_done` steps of a `steps`-step run (the reference's
         save_checkpoint, MolecularDynamics.py:1099-1116): device state, thermo history, parameters."""
         from .xlbomd import save_state
-        extra = {"MD_type": "XL_BOMD", "seqm_parameters": self.seqm_parameters, "xl_bomd_params": self.xl_bomd_params,
+        extra = {"MD_type": type(self).__name__, "seqm_parameters": self.seqm_parameters,
+                 "xl_bomd_params": self.xl_bomd_params,
                  "timestep": self.timestep, "Temp": self.Temp, "steps": int(steps), "step_done": int(step_done),
                  "reuse_P": bool(reuse_P), "species": molecule.species.detach().cpu().tolist(),
                  "charges": molecule.tot_charge.detach().cpu().tolist() if torch.is_tensor(molecule.tot_charge) else 0,
@@ -201,7 +216,8 @@ class XL_BOMD(Molecular_Dynamics_Basic):
         from .seqm_functions.constants import Constants
         from .xlbomd import load_state
         state, extra = load_state(path)
-        if extra.get("MD_type") != "XL_BOMD":
+        cls = {"XL_BOMD": XL_BOMD, "KSA_XL_BOMD": KSA_XL_BOMD}.get(extra.get("MD_type"))
+        if cls is None:
             raise RuntimeError(f"unknown MD type {extra.get('MD_type')!r} in checkpoint")
         dev = torch.device(device) if device is not None else torch.device("cuda")
         species = torch.as_tensor(extra["species"], dtype=torch.int64, device=dev)
@@ -215,15 +231,15 @@ class XL_BOMD(Molecular_Dynamics_Basic):
             if isinstance(extra["charges"], list) else extra["charges"]
         molecule = Molecule(const, extra["seqm_parameters"], torch.as_tensor(coords, device=dev), species,
                             charges=charges)
-        md = XL_BOMD(xl_bomd_params=extra["xl_bomd_params"], seqm_parameters=extra["seqm_parameters"],
-                     timestep=extra["timestep"], Temp=extra["Temp"], step_offset=extra["step_done"])
+        md = cls(xl_bomd_params=extra["xl_bomd_params"], seqm_parameters=extra["seqm_parameters"],
+                 timestep=extra["timestep"], Temp=extra["Temp"], step_offset=extra["step_done"])
         md.set_dof(molecule, 0.0)
         eng = molecule.engine()
         bt = eng.b
         sp2 = extra["seqm_parameters"].get("sp2", [False])
         t = state["tensors"]
         md.traj = XLTrajectories(eng, md.k, md.timestep, t["x"].to(dev), t["v"].to(dev), t["P"].to(dev),
-                                 density="sp2" if sp2[0] else "eigh")
+                                 density="sp2" if sp2[0] else "eigh", ksa=md._ksa_params())
         md.traj.load_state_dict(state)
         eng.set_coordinates(md.traj.x)
         molecule.Electronic_entropy = torch.zeros(bt.nmol, dtype=torch.float64, device=dev)
@@ -251,3 +267,15 @@ class XL_BOMD(Molecular_Dynamics_Basic):
             molecule.Etot, molecule.Eelec, molecule.Enuc = en["Etot"], en["Eelec"], en["Enuc"]
             molecule.Hf, molecule.Eiso = en["Hf"], en["Eiso"]
             molecule.dm = b.to_padded(tr.D)
+            molecule.Electronic_entropy = tr.EEnt.clone()
+            if tr.ksa:
+                molecule.dP2dt2 = b.to_padded(tr.dP2dt2)
+
+
+class KSA_XL_BOMD(XL_BOMD):
+    """Krylov-subspace XL-BOMD (MolecularDynamics.py:1608-1619): the XL step with the Fermi density
+    at T_el and P propagated with the rank-m Krylov kernel, P(n+1) = kappa (dP2dt2 + P(n)) + the
+    history terms (xl_bomd_params: k, max_rank, err_threshold, T_el).  The potential energy of the
+    thermo output includes the electronic entropy term -2 T_el S."""
+
+    KSA_PROPAGATION = True

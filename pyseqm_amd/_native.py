@@ -40,6 +40,15 @@ _SIGS = {
                             c_void_p, c_void_p],
     "pyseqm_pad_for_eigh": [c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_int, c_void_p],
     "pyseqm_eigh_density": [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
+    "pyseqm_eigh_vectors": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                            c_void_p],
+    "pyseqm_fermi_density": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_double, c_double, c_void_p,
+                             c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
+    "pyseqm_block_gemm": [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p],
+    "pyseqm_canon_prt": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_double, c_double, c_int, c_void_p,
+                         c_void_p],
+    "pyseqm_block_dot": [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p],
+    "pyseqm_block_axpy": [c_void_p, c_int, c_int, c_int, c_double, c_void_p, c_int, c_void_p, c_void_p, c_void_p],
     "pyseqm_density_from_evecs": [c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p],
     "pyseqm_mix": [c_void_p, c_int, c_int, c_double, c_void_p, c_void_p, c_void_p, c_void_p],
     "pyseqm_copy_active": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p],
@@ -82,6 +91,7 @@ _RAW_SIGS = {
                          c_void_p],
     "pyseqm_build_tiles": [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                            c_void_p, c_void_p, c_void_p, c_void_p],
+    "pyseqm_small_solve": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p],
 }
 _lib = None
 
@@ -112,13 +122,18 @@ def lib():
         L.pyseqm_eigh_density_workspace.restype = c_longlong
         L.pyseqm_eigh_density_lds_ok.argtypes = [c_int, c_int]
         L.pyseqm_eigh_density_lds_ok.restype = c_int
+        L.pyseqm_fermi_workspace.argtypes = [c_int]
+        L.pyseqm_fermi_workspace.restype = c_longlong
+        L.pyseqm_ksa_max_rank.argtypes = []
+        L.pyseqm_ksa_max_rank.restype = c_int
         _lib = L
     return _lib
 
 
 def exported_symbols():
     return list(_SIGS) + list(_RAW_SIGS) + ["pyseqm_fock_scratch_doubles", "pyseqm_eigh_density_workspace",
-                                            "pyseqm_eigh_density_lds_ok"]
+                                            "pyseqm_eigh_density_lds_ok", "pyseqm_fermi_workspace",
+                                            "pyseqm_ksa_max_rank"]
 
 
 def stream_ptr(device=None):

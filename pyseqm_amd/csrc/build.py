@@ -9,7 +9,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.dirname(HERE)
 ROOT = os.path.dirname(PKG)
-SOURCES = ["pairs.hip", "fock.hip", "scf.hip", "sp2.hip", "xl.hip", "eigh.hip", "layout.hip"]
+SOURCES = ["pairs.hip", "fock.hip", "scf.hip", "sp2.hip", "xl.hip", "eigh.hip", "layout.hip", "ksa.hip"]
 OUT = os.path.join(PKG, "libpyseqm_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-shared",

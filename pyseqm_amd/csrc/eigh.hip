@@ -147,18 +147,23 @@ __device__ inline double start_value(int j, int k) {
   return (double)h * (2.0 / 4294967296.0) - 1.0;
 }
 
-template <int GR, int GC, int RR, int RC>
+// EVEC = false: the density P = 2 C_occ C_occ^T of the nv lowest vectors.  EVEC = true: every
+// eigenpair (nv = n vectors), written as Qout (packed layout, column j = eigenvector j) and evals
+// (orb_off[m] + j, the Rayleigh quotients of the final vectors: exact to rounding whatever the
+// multisection bracket width).
+template <int GR, int GC, int RR, int RC, bool EVEC>
 __global__ __launch_bounds__(GR* GC) void k_eigh_density(pyseqm_layout_t L, const int* __restrict__ idx,
                                                        const double* __restrict__ F, double* __restrict__ P,
                                                        double* __restrict__ ws, int nmax,
-                                                       int* __restrict__ fail) {
+                                                       int* __restrict__ fail, const long long* __restrict__ orb_off,
+                                                       double* __restrict__ evals) {
   constexpr int NT = GR * GC;
   static_assert(GC == 16, "row groups are DPP rows");
   extern __shared__ double lds[];
   const int slot = blockIdx.x;
   const int m = idx[slot];
   const int n = L.mol_norb[m];
-  const int nocc = L.mol_nocc[m];
+  const int nv = EVEC ? n : L.mol_nocc[m];  // vectors computed
   const size_t mo = L.mol_mat_off[m];
   const int tid = threadIdx.x;
   const size_t nm2 = (size_t)nmax * nmax;
@@ -166,8 +171,8 @@ __global__ __launch_bounds__(GR* GC) void k_eigh_density(pyseqm_layout_t L, cons
   double* U0 = HH + nm2;
   double* U1 = HH + 2 * nm2;
   double* U2 = HH + 3 * nm2;
-  double* Z = lds;  // eigenvectors, n x nocc: Z[k * nocc + j]
-  double* d = Z + n * nocc;
+  double* Z = lds;  // eigenvectors, n x nv: Z[k * nv + j]
+  double* d = Z + n * nv;
   double* e = d + n;
   double* lam = e + n;
   double* xs0 = lam + n;  // reflector columns of even / odd steps
@@ -298,7 +303,7 @@ __global__ __launch_bounds__(GR* GC) void k_eigh_density(pyseqm_layout_t L, cons
     }
   __syncthreads();
 
-  // ---- 2. the nocc lowest eigenvalues: multisection, a group of tb lanes per eigenvalue
+  // ---- 2. the nv lowest eigenvalues: multisection, a group of tb lanes per eigenvalue
   double lo_p = 1.0e300, hi_p = -1.0e300, e2max = 0.0;
   for (int k = tid; k < n; k += NT) {
     const double el = k > 0 ? fabs(e[k - 1]) : 0.0, er = k + 1 < n ? fabs(e[k]) : 0.0;
@@ -323,11 +328,11 @@ __global__ __launch_bounds__(GR* GC) void k_eigh_density(pyseqm_layout_t L, cons
   const double btol = 1.0e-11 * tnorm;
   {
     int tb = 64;
-    while (tb > 1 && nocc * tb > NT) tb >>= 1;
+    while (tb > 1 && nv * tb > NT) tb >>= 1;
     const int ngrp = NT / tb, gi = tid / tb, gl = tid % tb;
-    for (int j0 = 0; j0 < nocc; j0 += ngrp) {
+    for (int j0 = 0; j0 < nv; j0 += ngrp) {
       const int j = j0 + gi;
-      if (j < nocc) {  // uniform over the lane group
+      if (j < nv) {  // uniform over the lane group
         double lo = glo, hi = ghi;
         for (int round = 0; round < 64; ++round) {
           if (!(hi - lo > fmax(4.4e-16 * fmax(fabs(lo), fabs(hi)), btol) + 2.0 * pivmin)) break;
@@ -345,10 +350,10 @@ __global__ __launch_bounds__(GR* GC) void k_eigh_density(pyseqm_layout_t L, cons
   }
   __syncthreads();
 
-  // ---- 3. inverse iteration, one thread per eigenvector; vector j in LDS at Z[k * nocc + j]
+  // ---- 3. inverse iteration, one thread per eigenvector; vector j in LDS at Z[k * nv + j]
   const double tiny = 2.2e-16 * fmax(tnorm, 1.0e-300);
-  for (int j = tid; j < nocc; j += NT) {
-    for (int k = 0; k < n; ++k) Z[k * nocc + j] = start_value(j, k);
+  for (int j = tid; j < nv; j += NT) {
+    for (int k = 0; k < n; ++k) Z[k * nv + j] = start_value(j, k);
     const double sig = lam[j];
     for (int rep = 0; rep < 3; ++rep) {
       // LU of T - sig I with partial pivoting, the right-hand side eliminated alongside; the
@@ -357,7 +362,7 @@ __global__ __launch_bounds__(GR* GC) void k_eigh_density(pyseqm_layout_t L, cons
       double yk = Z[j];
       for (int k = 0; k + 1 < n; ++k) {
         const double sub = e[k], nd = d[k + 1] - sig, ns = k + 2 < n ? e[k + 1] : 0.0;
-        const double yk1 = Z[(k + 1) * nocc + j];
+        const double yk1 = Z[(k + 1) * nv + j];
         const size_t o = (size_t)k * nmax + j;
         if (fabs(diag) >= fabs(sub)) {
           const double piv = fabs(diag) < tiny ? copysign(tiny, diag) : diag;
@@ -366,7 +371,7 @@ __global__ __launch_bounds__(GR* GC) void k_eigh_density(pyseqm_layout_t L, cons
           U0[o] = rp;
           U1[o] = sup;
           U2[o] = 0.0;
-          Z[k * nocc + j] = yk;
+          Z[k * nv + j] = yk;
           yk = yk1 - mlt * yk;
           diag = nd - mlt * sup;
           sup = ns;
@@ -376,7 +381,7 @@ __global__ __launch_bounds__(GR* GC) void k_eigh_density(pyseqm_layout_t L, cons
           U0[o] = rp;
           U1[o] = nd;
           U2[o] = ns;
-          Z[k * nocc + j] = yk1;
+          Z[k * nv + j] = yk1;
           yk = yk - mlt * yk1;
           diag = sup - mlt * nd;
           sup = -mlt * ns;
@@ -384,7 +389,7 @@ __global__ __launch_bounds__(GR* GC) void k_eigh_density(pyseqm_layout_t L, cons
       }
       const double rlast = 1.0 / (fabs(diag) < tiny ? copysign(tiny, diag) : diag);
       double x1 = yk * rlast, x2 = 0.0, xmax = fabs(x1);
-      Z[(n - 1) * nocc + j] = x1;
+      Z[(n - 1) * nv + j] = x1;
       // back substitution in blocks of kPF rows: the U factors of the next block are loaded while
       // this block is solved (the loads come from L2; one row ahead left each row waiting on one)
       constexpr int kPF = 8;
@@ -411,8 +416,8 @@ __global__ __launch_bounds__(GR* GC) void k_eigh_density(pyseqm_layout_t L, cons
         for (int t = 0; t < kPF; ++t) {
           const int k = k0 - t;
           if (k >= 0) {
-            const double x = (Z[k * nocc + j] - c1[t] * x1 - c2[t] * x2) * c0[t];
-            Z[k * nocc + j] = x;
+            const double x = (Z[k * nv + j] - c1[t] * x1 - c2[t] * x2) * c0[t];
+            Z[k * nv + j] = x;
             xmax = fmax(xmax, fabs(x));
             x2 = x1;
             x1 = x;
@@ -426,7 +431,7 @@ __global__ __launch_bounds__(GR* GC) void k_eigh_density(pyseqm_layout_t L, cons
         }
       }
       const double sc = xmax > 0.0 ? 1.0 / xmax : 1.0;
-      for (int k = 0; k < n; ++k) Z[k * nocc + j] *= sc;
+      for (int k = 0; k < n; ++k) Z[k * nv + j] *= sc;
     }
   }
   __syncthreads();
@@ -434,29 +439,40 @@ __global__ __launch_bounds__(GR* GC) void k_eigh_density(pyseqm_layout_t L, cons
   // ---- 4. clusters (eigenvalues closer than ctol) orthogonalised by the thread of their first
   //         member, two Gram-Schmidt passes; every vector normalised
   const double ctol = 1.0e-5 * fmax(tnorm, 1.0e-300);
-  for (int j = tid; j < nocc; j += NT) {
+  for (int j = tid; j < nv; j += NT) {
     if (j > 0 && lam[j] - lam[j - 1] <= ctol) continue;  // inside a cluster: its first thread works
     int jend = j + 1;
-    while (jend < nocc && lam[jend] - lam[jend - 1] <= ctol) ++jend;
+    while (jend < nv && lam[jend] - lam[jend - 1] <= ctol) ++jend;
     for (int q = j; q < jend; ++q) {
       for (int pass = 0; pass < 2; ++pass)
         for (int p = j; p < q; ++p) {
           double dot = 0.0;
-          for (int k = 0; k < n; ++k) dot += Z[k * nocc + p] * Z[k * nocc + q];
-          for (int k = 0; k < n; ++k) Z[k * nocc + q] -= dot * Z[k * nocc + p];
+          for (int k = 0; k < n; ++k) dot += Z[k * nv + p] * Z[k * nv + q];
+          for (int k = 0; k < n; ++k) Z[k * nv + q] -= dot * Z[k * nv + p];
         }
       double nr = 0.0;
-      for (int k = 0; k < n; ++k) nr += Z[k * nocc + q] * Z[k * nocc + q];
+      for (int k = 0; k < n; ++k) nr += Z[k * nv + q] * Z[k * nv + q];
       if (!(nr > 1.0e-200)) {  // a vector lost to its cluster's span: flagged, left unscaled
         atomicAdd(fail, 1);
         continue;
       }
       const double sc = 1.0 / sqrt(nr);
-      for (int k = 0; k < n; ++k) Z[k * nocc + q] *= sc;
+      for (int k = 0; k < n; ++k) Z[k * nv + q] *= sc;
     }
   }
   __syncthreads();
 
+  if (EVEC) {  // Rayleigh quotients z^T T z of the normalised vectors (ascending to rounding)
+    for (int j = tid; j < nv; j += NT) {
+      double rq = d[0] * Z[j] * Z[j];
+      for (int k = 1; k < n; ++k) {
+        const double zk = Z[k * nv + j], zk1 = Z[(k - 1) * nv + j];
+        rq += d[k] * zk * zk + 2.0 * e[k - 1] * zk * zk1;
+      }
+      lam[j] = rq;
+    }
+    __syncthreads();
+  }
   // ---- 5. C = Q Z in registers: thread (tr, tc) holds C[tc + 16 b][tr + GR c] (b < RC rows
   //         over the lanes of its DPP row, c < RR columns), so each column's product v^T C reduces
   //         inside one DPP row and the reflectors (last to first) need no barrier; each thread
@@ -468,7 +484,7 @@ __global__ __launch_bounds__(GR* GC) void k_eigh_density(pyseqm_layout_t L, cons
 #pragma unroll
       for (int b = 0; b < RC; ++b) {
         const int k = tc + 16 * b, a = tr + GR * c;
-        Ar[c][b] = (k < n && a < nocc) ? Z[k * nocc + a] : 0.0;
+        Ar[c][b] = (k < n && a < nv) ? Z[k * nv + a] : 0.0;
       }
     double vcur[RC], vnext[RC];
     double tcur = 0.0, tnext = 0.0;
@@ -486,7 +502,7 @@ __global__ __launch_bounds__(GR* GC) void k_eigh_density(pyseqm_layout_t L, cons
       if (tcur != 0.0) {  // the same in every thread
 #pragma unroll
         for (int c = 0; c < RR; ++c) {
-          if (GR * c >= nocc) continue;  // no column of this slot exists (uniform)
+          if (GR * c >= nv) continue;  // no column of this slot exists (uniform)
           double sd = 0.0;
 #pragma unroll
           for (int b = 0; b < RC; ++b)
@@ -507,9 +523,15 @@ __global__ __launch_bounds__(GR* GC) void k_eigh_density(pyseqm_layout_t L, cons
 #pragma unroll
       for (int b = 0; b < RC; ++b) {
         const int k = tc + 16 * b, a = tr + GR * c;
-        if (k < n && a < nocc) C[k * nocc + a] = Ar[c][b];
+        if (k < n && a < nv) C[k * nv + a] = Ar[c][b];
       }
     __syncthreads();
+  }
+  if (EVEC) {
+    double* Qm = P + mo;
+    for (int t = tid; t < n * n; t += NT) Qm[t] = C[t];  // C is n x n, row k = orbital, column j = vector
+    for (int j = tid; j < n; j += NT) evals[orb_off[m] + j] = lam[j];
+    return;
   }
   // ---- P = 2 C C^T, 2 x 2 output blocks per thread over the upper triangle, both triangles written
   double* Pm = P + mo;
@@ -520,9 +542,9 @@ __global__ __launch_bounds__(GR* GC) void k_eigh_density(pyseqm_layout_t L, cons
     const int r0 = 2 * bi, c0 = 2 * bj;
     const int r1 = min(r0 + 1, n - 1), c1 = min(c0 + 1, n - 1);
     double s00 = 0.0, s01 = 0.0, s10 = 0.0, s11 = 0.0;
-    for (int a = 0; a < nocc; ++a) {
-      const double x0 = C[(size_t)r0 * nocc + a], x1 = C[(size_t)r1 * nocc + a];
-      const double y0 = C[(size_t)c0 * nocc + a], y1 = C[(size_t)c1 * nocc + a];
+    for (int a = 0; a < nv; ++a) {
+      const double x0 = C[(size_t)r0 * nv + a], x1 = C[(size_t)r1 * nv + a];
+      const double y0 = C[(size_t)c0 * nv + a], y1 = C[(size_t)c1 * nv + a];
       s00 += x0 * y0;
       s01 += x0 * y1;
       s10 += x1 * y0;
@@ -553,6 +575,18 @@ extern "C" long long pyseqm_eigh_density_workspace(int nact, int nmax) {
   return 4LL * nact * nmax * nmax;
 }
 
+// thread grid GR x 16 with RR x RC register slots per thread: GR RR and 16 RC cover nmax
+template <bool EVEC>
+void launch_eigh(const pyseqm_layout_t* L, const int* idx, int nact, int nmax, const double* F, double* out,
+                 double* ws, int* fail, const long long* orb_off, double* evals, size_t lds, hipStream_t s) {
+  if (nmax <= 48)
+    k_eigh_density<4, 16, 12, 3, EVEC><<<nact, 64, lds, s>>>(*L, idx, F, out, ws, nmax, fail, orb_off, evals);
+  else if (nmax <= 96)
+    k_eigh_density<16, 16, 6, 6, EVEC><<<nact, 256, lds, s>>>(*L, idx, F, out, ws, nmax, fail, orb_off, evals);
+  else
+    k_eigh_density<32, 16, 6, 12, EVEC><<<nact, 512, lds, s>>>(*L, idx, F, out, ws, nmax, fail, orb_off, evals);
+}
+
 extern "C" int pyseqm_eigh_density_lds_ok(int norb, int nocc) {
   if (norb < 1 || norb > kEighMaxN || nocc < 0 || nocc > norb) return 0;
   return lds_doubles(norb, nocc) <= kLdsDoubles;
@@ -566,12 +600,17 @@ extern "C" int pyseqm_eigh_density(const pyseqm_layout_t* L, const int* idx, int
   if (nact == 0) return PYSEQM_OK;
   const size_t lds = (size_t)lds_doubles(nmax, nocc_max) * sizeof(double);
   hipStream_t s = (hipStream_t)stream;
-  // thread grid GR x 16 with RR x RC register slots per thread: GR RR and 16 RC cover nmax
-  if (nmax <= 48)
-    k_eigh_density<4, 16, 12, 3><<<nact, 64, lds, s>>>(*L, idx, F, P, ws, nmax, fail);
-  else if (nmax <= 96)
-    k_eigh_density<16, 16, 6, 6><<<nact, 256, lds, s>>>(*L, idx, F, P, ws, nmax, fail);
-  else
-    k_eigh_density<32, 16, 6, 12><<<nact, 512, lds, s>>>(*L, idx, F, P, ws, nmax, fail);
+  launch_eigh<false>(L, idx, nact, nmax, F, P, ws, fail, nullptr, nullptr, lds, s);
+  return hipGetLastError() == hipSuccess ? PYSEQM_OK : PYSEQM_ELAUNCH;
+}
+
+extern "C" int pyseqm_eigh_vectors(const pyseqm_layout_t* L, const int* idx, int nact, int nmax, const double* F,
+                                   const long long* orb_off, double* evals, double* Q, double* ws, int* fail,
+                                   void* stream) {
+  if (!L || nact < 0 || nmax < 1 || nmax > kEighMaxN || !fail || !orb_off || !evals) return PYSEQM_EINVAL;
+  if (!pyseqm_eigh_density_lds_ok(nmax, nmax)) return PYSEQM_EINVAL;
+  if (nact == 0) return PYSEQM_OK;
+  const size_t lds = (size_t)lds_doubles(nmax, nmax) * sizeof(double);
+  launch_eigh<true>(L, idx, nact, nmax, F, Q, ws, fail, orb_off, evals, lds, (hipStream_t)stream);
   return hipGetLastError() == hipSuccess ? PYSEQM_OK : PYSEQM_ELAUNCH;
 }

@@ -59,7 +59,11 @@ class XLTrajectories:
     Pt (m, total) history ring.  ``P0`` is the converged SCF density that seeds P, Pt and D
     (MolecularDynamics.py:1590-1605)."""
 
-    def __init__(self, eng, k, dt, x, v, P0, density="eigh", force_h=1.0e-3):
+    def __init__(self, eng, k, dt, x, v, P0, density="eigh", force_h=1.0e-3, ksa=None):
+        """ksa: None, or the max_rank branch of EnergyXL (xlbomd.py:201-339) as a dict with
+        "max_rank", "err_threshold", "T_el" and "propagate" (True: P is propagated with the Krylov
+        kernel as KSA_XL_BOMD does, MolecularDynamics.py:1615-1619; False: with the density as XL_BOMD
+        does).  D is then the Fermi density of F(P) at T_el."""
         self.eng = eng
         b = eng.b
         dev = eng.device
@@ -86,6 +90,14 @@ class XLTrajectories:
         self.mol_of_atom = torch.as_tensor(b.atom_mol, device=dev).long()
         self.idx_all = torch.arange(b.nmol, dtype=torch.int32, device=dev)
         self.step_count = 0
+        self.ksa = dict(ksa) if ksa else None
+        self.EEnt = torch.zeros(b.nmol, **f64)  # -2 T_el S, the XL potential's entropy term
+        if self.ksa:
+            from .ksa import KSAWorkspace
+            self.ks = KSAWorkspace(eng, int(self.ksa["max_rank"]))
+            self.dP2dt2 = torch.zeros(total, **f64)
+            self.tmp = torch.zeros(total, **f64)
+            self.STATE = XLTrajectories.STATE + ("dP2dt2", "EEnt")
 
     # ----------------------------------------------------------------- pieces
     def set_forces(self, f):
@@ -101,17 +113,31 @@ class XLTrajectories:
     def _propagate(self):
         cindx = self.step_count % self.m
         coef = (ctypes.c_double * self.m)(*self.coeff[cindx:cindx + self.m])
-        nat.call_raw("pyseqm_xl_propagate", self.eng.b.total, nat.ptr(self.D), nat.ptr(self.P), nat.ptr(self.Pt),
-                     self.Pt.shape[1], self.m, coef, self.coeff_D, MIX_C, self.m - 1 - cindx, self.eng.stream)
+        D, c = self.D, MIX_C
+        if self.ksa and self.ksa["propagate"]:
+            # KSA_XL_BOMD._propagate_P: kappa (dP2dt2 + P) + the history terms
+            from .ksa import _axpy
+            self.tmp.copy_(self.P)
+            _axpy(self.eng, self.ks, 4, 1.0, None, 0, self.dP2dt2, self.tmp)
+            D, c = self.tmp, 1.0
+        nat.call_raw("pyseqm_xl_propagate", self.eng.b.total, nat.ptr(D), nat.ptr(self.P), nat.ptr(self.Pt),
+                     self.Pt.shape[1], self.m, coef, self.coeff_D, c, self.m - 1 - cindx, self.eng.stream)
 
     def energy_and_forces(self):
         """At the current x and P: F(P), D = density(F), the XL energy and forces at fixed D, P."""
         eng, b = self.eng, self.eng.b
         eng.set_coordinates(self.x)
         eng.fock(self.P, self.F)
-        eng.density_kind = self.density_kind
-        eng._idx_host = np.arange(b.nmol)
-        eng.density(self.F, self.idx_all, b.nmol, self.D)
+        if self.ksa:
+            from . import ksa as K
+            T = float(self.ksa["T_el"])
+            K.fermi_density(eng, self.ks, self.F, T, self.D)
+            torch.mul(self.ks.S, -2.0 * T, out=self.EEnt)
+            K.rank_kernel(eng, self.ks, self.D, self.P, T, float(self.ksa.get("err_threshold", 0.0)), self.dP2dt2)
+        else:
+            eng.density_kind = self.density_kind
+            eng._idx_host = np.arange(b.nmol)
+            eng.density(self.F, self.idx_all, b.nmol, self.D)
         nat.call("pyseqm_xl_energy", b.L, nat.ptr(self.D), nat.ptr(self.P), nat.ptr(self.F), nat.ptr(eng.H),
                  nat.ptr(self.eelec), nat.ptr(eng.red), nat.red_blocks(b.nmax), eng.stream)
         nat.call("pyseqm_pair_forces_xl", b.L, nat.METHODS[eng.method], nat.ptr(self.x), nat.ptr(eng.atom_par),
@@ -134,14 +160,14 @@ class XLTrajectories:
         needs to continue bit for bit (the integrals and Fock matrices follow from x and P)."""
         return {"tensors": {k: getattr(self, k) for k in self.STATE},
                 "meta": {"k": self.k, "m": self.m, "dt": self.dt, "step_count": self.step_count,
-                         "density": self.density_kind, "force_h": self.force_h,
+                         "density": self.density_kind, "force_h": self.force_h, "ksa": self.ksa,
                          "natoms": int(self.eng.b.natoms), "total": int(self.eng.b.total)}}
 
     def load_state_dict(self, state):
         """Copy a state_dict into this batch's device tensors, in place (graphs captured on them stay
         valid); the batch must hold the same molecules, k, time step and density step."""
         meta, mine = state["meta"], self.state_dict()["meta"]
-        for key in ("k", "m", "natoms", "total", "density"):
+        for key in ("k", "m", "natoms", "total", "density", "ksa"):
             if meta[key] != mine[key]:
                 raise ValueError(f"checkpoint {key} = {meta[key]!r} does not match this trajectory batch ({mine[key]!r})")
         if float(meta["dt"]) != self.dt:
@@ -169,24 +195,30 @@ class XLTrajectories:
 
 
 class EnergyXL(torch.nn.Module):
-    """Energies of the XL Lagrangian at a given density field P (xlbomd.py:73-508, ground state,
-    no KSA rank): returns (Hf, Etot, Eelec, EEnt, Enuc, Eiso, EnucAB, D, dP2dt2, Error, e_gap, e,
-    Fe_occ) with D the density of F(P)."""
+    """Energies of the XL Lagrangian at a given density field P (xlbomd.py:73-508, ground state):
+    returns (Hf, Etot, Eelec, EEnt, Enuc, Eiso, EnucAB, D, dP2dt2, Error, e_gap, e, Fe_occ) with D the
+    density of F(P), or with xl_bomd_params["max_rank"] the Fermi density at T_el, its entropy term
+    EEnt = -2 T_el S and the rank-m Krylov kernel dP2dt2 (xlbomd.py:201-341)."""
 
     def __init__(self, seqm_parameters):
         super().__init__()
         self.seqm_parameters = seqm_parameters
         self.sp2 = seqm_parameters.get("sp2", [False])
 
-    def _run(self, molecule, P):
+    def _run(self, molecule, P, xl_bomd_params):
         eng = molecule.engine()
         b = eng.b
+        ksa = None
+        if "max_rank" in xl_bomd_params:
+            ksa = {"max_rank": int(xl_bomd_params["max_rank"]),
+                   "err_threshold": float(xl_bomd_params.get("err_threshold", 0.0)),
+                   "T_el": float(xl_bomd_params["T_el"]), "propagate": False}
         tr = getattr(molecule, "_xl", None)
         Pp = b.from_padded(P.detach().to(torch.float64))
         x = b.atoms_from_padded(molecule.coordinates.detach())
-        if tr is None:
+        if tr is None or tr.ksa != ksa:
             tr = XLTrajectories(eng, 3, 0.0, x, torch.zeros_like(x), Pp,
-                                density="sp2" if self.sp2[0] else "eigh")
+                                density="sp2" if self.sp2[0] else "eigh", ksa=ksa)
             molecule._xl = tr
         tr.x.copy_(b.atoms_from_padded(x))
         tr.P.copy_(Pp)
@@ -195,19 +227,31 @@ class EnergyXL(torch.nn.Module):
 
     def forward(self, molecule, P, cis_amp=None, learned_parameters=dict(), xl_bomd_params=dict(), all_terms=False,
                 *args, **kwargs):
-        if "max_rank" in xl_bomd_params:
-            raise NotImplementedError("KSA-XL-BOMD (max_rank) is outside the ground-state XL-BOMD path")
-        eng, tr = self._run(molecule, P)
+        eng, tr = self._run(molecule, P, xl_bomd_params)
         en = tr.energies()
         D = eng.b.to_padded(tr.D)
-        e, _ = eng.eig(tr.F)
+        if tr.ksa:
+            b, ks = eng.b, tr.ks
+            e = torch.zeros(b.nmol, int(b.norb.max()), dtype=torch.float64, device=eng.device)
+            occ = torch.zeros_like(e)
+            offs = np.concatenate([[0], np.cumsum(b.norb)[:-1]])
+            for m in range(b.nmol):  # eigenvalues / occupations, one padded row per molecule
+                o, n = int(offs[m]), int(b.norb[m])
+                e[m, :n] = ks.evals[o:o + n]
+                occ[m, :n] = ks.occ[o:o + n]
+            dP2dt2 = b.to_padded(tr.dP2dt2)
+            err = ks.last_error
+            fe_occ = occ
+        else:
+            e, _ = eng.eig(tr.F)
+            dP2dt2 = err = fe_occ = None
         lumo = molecule.nocc.reshape(-1, 1)
         e_gap = (e.gather(1, lumo) - e.gather(1, lumo - 1)).reshape(-1)
-        ent = torch.zeros_like(en["Etot"])
+        ent = tr.EEnt.clone()
         if not all_terms:
             return en["Eelec"], None, D, e_gap, e
-        return (en["Hf"], en["Etot"], en["Eelec"], ent, en["Enuc"], en["Eiso"], None, D, None, None, e_gap, e,
-                None)
+        return (en["Hf"], en["Etot"], en["Eelec"], ent, en["Enuc"], en["Eiso"], None, D, dP2dt2, err, e_gap, e,
+                fe_occ)
 
 
 class ForceXL(torch.nn.Module):

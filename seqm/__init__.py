@@ -7,7 +7,8 @@ Langevin and nonadiabatic dynamics, geometry optimisation, PM6) do not exist her
 import importlib
 import sys
 
-from pyseqm_amd import Constants, Electronic_Structure, Molecular_Dynamics_Basic, Molecule, XL_BOMD  # noqa: F401
+from pyseqm_amd import (Constants, Electronic_Structure, KSA_XL_BOMD, Molecular_Dynamics_Basic,  # noqa: F401
+                        Molecule, XL_BOMD)
 from pyseqm_amd.seqm_functions.read_xyz import read_xyz  # noqa: F401
 from pyseqm_amd.xlbomd import EnergyXL, ForceXL  # noqa: F401
 
@@ -40,5 +41,5 @@ sys.modules.setdefault("seqm.dynamics", sys.modules["seqm.dynamics.xlbomd"])
 sys.modules.setdefault("seqm.io", sys.modules["seqm.io.xyz"])
 seqm_functions = sys.modules["seqm.seqm_functions"]
 
-__all__ = ["Constants", "Electronic_Structure", "EnergyXL", "ForceXL", "Molecular_Dynamics_Basic", "Molecule",
-           "XL_BOMD", "read_xyz"]
+__all__ = ["Constants", "Electronic_Structure", "EnergyXL", "ForceXL", "KSA_XL_BOMD", "Molecular_Dynamics_Basic",
+           "Molecule", "XL_BOMD", "read_xyz"]

@@ -14,7 +14,11 @@ pytestmark = pytest.mark.gpu
 DEV = "cuda"
 
 
-def test_xl_bomd_run_from_checkpoint_continues_the_run(tmp_path):
+@pytest.mark.parametrize("cls,params", [("XL_BOMD", {"k": 6}),
+                                        ("KSA_XL_BOMD", {"k": 6, "max_rank": 3, "err_threshold": 0.0, "T_el": 1500})])
+def test_xl_bomd_run_from_checkpoint_continues_the_run(tmp_path, cls, params):
+    """Also for KSA_XL_BOMD (the reference's test_md_checkpoint_resume.py:200-230 covers both): the
+    Krylov kernel dP2dt2 and the entropy term travel in the checkpoint."""
     sp, c = read_xyz(["methanal.1.xyz", "methane.xyz"])
     p = {"method": "AM1", "scf_eps": 1.0e-7, "scf_converger": [1]}
 
@@ -22,7 +26,7 @@ def test_xl_bomd_run_from_checkpoint_continues_the_run(tmp_path):
         torch.manual_seed(0)
         const = pyseqm_amd.Constants().to(DEV)
         mol = pyseqm_amd.Molecule(const, dict(p), torch.as_tensor(c, device=DEV), torch.as_tensor(sp, device=DEV))
-        md = pyseqm_amd.XL_BOMD(xl_bomd_params={"k": 6}, seqm_parameters=dict(p), timestep=0.5, Temp=300.0)
+        md = getattr(pyseqm_amd, cls)(xl_bomd_params=dict(params), seqm_parameters=dict(p), timestep=0.5, Temp=300.0)
         return md, mol
 
     md, mol = md_and_mol()
@@ -34,7 +38,8 @@ def test_xl_bomd_run_from_checkpoint_continues_the_run(tmp_path):
     md1.run(mol1, steps=5, seed=0)
     path = str(tmp_path / "xl.safetensors")
     md1.save_checkpoint(mol1, 8, step_done=5, path=path)
-    md2, mol2 = pyseqm_amd.XL_BOMD.run_from_checkpoint(path, device=DEV)
+    md2, mol2 = getattr(pyseqm_amd, cls).run_from_checkpoint(path, device=DEV)
+    assert type(md2).__name__ == cls
     T2, Ek2, Ep2, Etot2 = md2.thermo()
     assert torch.equal(mol2.coordinates.detach(), x_full)
     np.testing.assert_allclose(Etot2, Etot, atol=1e-12, rtol=0)

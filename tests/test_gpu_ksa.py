@@ -1,0 +1,207 @@
+"""Thermal density and KSA-XL-BOMD on the GPU (pyseqm_amd/ksa.py, ksa.hip, the eigenpair mode of
+eigh.hip) against the oracle, and the drop-in KSA_XL_BOMD driver against the reference's MD goldens.
+
+Bars: eigenpairs reconstruct F to 1e-10 relative with orthonormal columns, eigenvalues within 1e-10
+of numpy's; Fermi density, occupations, mu and entropy within 1e-10 of the oracle (at 1500 K and
+at 15000 K, where occupations are fractional); the canonical DM perturbation response within 1e-10
+relative; the rank-3 Krylov kernel dP2dt2 within 1e-9 relative; trajectories: Ek(t) and the
+potential (Etot + the entropy term) of device and oracle within 1e-6 eV step by step from the same
+velocities; the reference's 31-point KSA series within its own 1e-2 (step 0 within 1e-8), drift
+within its 5e-2.
+"""
+import numpy as np
+import pytest
+import torch
+
+import pyseqm_amd
+from oracle import seqm_ref as R
+from pyseqm_amd import ksa as K
+from pyseqm_amd.engine import SCFEngine
+from pyseqm_amd.xlbomd import ACC_SCALE, KINETIC_ENERGY_SCALE
+from tests.conftest import golden, read_xyz
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda"
+
+
+def _engine(names=("methane.xyz", "benzene.xyz"), method="AM1"):
+    sp, c = read_xyz(list(names))
+    eng = SCFEngine(sp, c, method, DEV)
+    r = eng.scf(eps=1e-8, converger=(1,))
+    F = torch.empty_like(r.P)
+    eng.fock(r.P, F)
+    ob = R.parse(sp, c, method)
+    H, Tt, _, _, _ = R.hcore(ob)
+    return eng, r.P, F, ob, H, Tt, sp, c
+
+
+def _blocks(eng, X):
+    b = eng.b
+    return [X[int(b.mol_mat_off[m]):int(b.mol_mat_off[m]) + int(b.norb[m]) ** 2].view(int(b.norb[m]), -1).cpu().numpy()
+            for m in range(b.nmol)]
+
+
+def test_eigenpairs_reconstruct_the_blocks():
+    eng, P, F, *_ = _engine()
+    ks = K.KSAWorkspace(eng, 3)
+    K.fermi_density(eng, ks, F, 1500.0, torch.zeros_like(F))
+    torch.cuda.synchronize()
+    offs = ks.orb_off.cpu().numpy()
+    for m, (f, q) in enumerate(zip(_blocks(eng, F), _blocks(eng, ks.Q))):
+        n = f.shape[0]
+        e = ks.evals[int(offs[m]):int(offs[m]) + n].cpu().numpy()
+        scale = np.abs(f).max()
+        np.testing.assert_allclose(e, np.linalg.eigvalsh(f), atol=1e-10 * scale, rtol=0)
+        np.testing.assert_allclose(q.T @ q, np.eye(n), atol=1e-12)
+        np.testing.assert_allclose((q * e) @ q.T, f, atol=1e-10 * scale, rtol=0)
+    assert int(ks.fail.item()) == 0
+
+
+@pytest.mark.parametrize("T", [1500.0, 15000.0])
+def test_fermi_density_matches_oracle(T):
+    eng, P, F, ob, H, Tt, *_ = _engine()
+    b = eng.b
+    ks = K.KSAWorkspace(eng, 3)
+    D = torch.zeros_like(F)
+    K.fermi_density(eng, ks, F, T, D)
+    Fo = b.to_padded(F).cpu().numpy()
+    Do, So, es, Qs, fs, mus = R.fermi_q(ob, Fo, T)
+    np.testing.assert_allclose(b.to_padded(D).cpu().numpy(), Do, atol=1e-10, rtol=0)
+    np.testing.assert_allclose(ks.S.cpu().numpy(), So, atol=1e-12, rtol=1e-10)
+    offs = ks.orb_off.cpu().numpy()
+    for m in range(b.nmol):
+        n = int(b.norb[m])
+        np.testing.assert_allclose(ks.occ[int(offs[m]):int(offs[m]) + n].cpu().numpy(), fs[m], atol=1e-10, rtol=0)
+        assert abs(float(ks.mu[m]) - mus[m]) < 1e-9
+    if T > 10000:
+        assert So[1] > 1e-6  # benzene's occupations are fractional at this temperature
+
+
+def test_canon_response_matches_oracle():
+    eng, P, F, ob, H, Tt, *_ = _engine()
+    b = eng.b
+    T = 15000.0
+    ks = K.KSAWorkspace(eng, 3)
+    K.fermi_density(eng, ks, F, T, torch.zeros_like(F))
+    rng = np.random.default_rng(2)
+    X = torch.zeros_like(F)
+    xs = []
+    for m in range(b.nmol):
+        n, o = int(b.norb[m]), int(b.mol_mat_off[m])
+        x = rng.normal(size=(n, n))
+        x = x + x.T
+        xs.append(x)
+        X[o:o + n * n] = torch.as_tensor(x.ravel(), device=DEV)
+    out = torch.zeros_like(F)
+    K.canon_response(eng, ks, X, out, T)
+    Do, So, es, Qs, fs, mus = R.fermi_q(ob, b.to_padded(F).cpu().numpy(), T)
+    for m, got in enumerate(_blocks(eng, out)):
+        ref = R.canon_dm_prt(xs[m], T, Qs[m], es[m], mus[m])
+        np.testing.assert_allclose(got, ref, atol=1e-10 * np.abs(ref).max(), rtol=0)
+
+
+@pytest.mark.parametrize("err_threshold", [0.0, 1e-3])
+def test_rank_kernel_matches_oracle(err_threshold):
+    eng, P, F, ob, H, Tt, *_ = _engine()
+    b = eng.b
+    T = 15000.0
+    # a density field away from self-consistency: the converged density plus a symmetric kick
+    rng = np.random.default_rng(5)
+    Pp = b.to_padded(P).cpu().numpy()
+    kick = rng.normal(scale=0.02, size=Pp.shape)
+    kick = (kick + kick.transpose(0, 2, 1)) * (Pp != 0)
+    Pk = b.from_padded(torch.as_tensor(Pp + kick, device=DEV))
+    Fk = torch.empty_like(Pk)
+    eng.fock(Pk, Fk)
+    ks = K.KSAWorkspace(eng, 3)
+    D = torch.zeros_like(Pk)
+    K.fermi_density(eng, ks, Fk, T, D)
+    dP = torch.zeros_like(Pk)
+    K.rank_kernel(eng, ks, D, Pk, T, err_threshold, dP)
+    Fo = R.fock(ob, Pp + kick, H, Tt)
+    Do, So, es, Qs, fs, mus = R.fermi_q(ob, Fo, T)
+    dPo, Err, rank = R.ksa_rank_kernel(ob, Do, Pp + kick, Qs, es, mus, T, Tt, 3, err_threshold)
+    assert ks.last_rank == rank
+    got = b.to_padded(dP).cpu().numpy()
+    np.testing.assert_allclose(got, dPo, atol=1e-9 * np.abs(dPo).max(), rtol=0)
+
+
+def _oracle_ksa(sp, x, v, method, k, nsteps, T_el, rank=3, eps=1e-7, dt=0.5):
+    """KSA-XL-BOMD on the oracle (MolecularDynamics.py:1437-1519, 1608-1619 with EnergyXL's
+    max_rank branch): SCF seed density, then per step the KSA propagation, the Fermi density of
+    F(P), the Krylov kernel, the XL energy and finite-difference XL forces."""
+    real = (sp > 0)[..., None]
+    mass = np.where(real, R.MASS[sp][..., None], 1.0)  # padded atoms: no force, no velocity
+    v = v * real
+    r = R.single_point(sp, x, method, eps=eps, converger=(1,))
+    P = r["P"].copy()
+    a = R.forces_fd(sp, x, method, P) / mass * ACC_SCALE
+    Pt = np.repeat(P[None], k + 1, axis=0)
+    dP2dt2 = np.zeros_like(P)
+    ek = lambda v: (0.5 * mass * v * v).sum(axis=(1, 2)) * KINETIC_ENERGY_SCALE
+    out = [np.stack([ek(v), r["Etot"] + ek(v)], 1)]
+    for step in range(nsteps):
+        v = v + 0.5 * a * dt
+        x = x + v * dt
+        P, Pt = R.ksa_xl_propagate(dP2dt2, P, Pt, k, step)
+        b = R.parse(sp, x, method)
+        H, Tt, _, _, _ = R.hcore(b)
+        D, S, es, Qs, fs, mus = R.fermi_q(b, R.fock(b, P, H, Tt), T_el)
+        dP2dt2, _, _ = R.ksa_rank_kernel(b, D, P, Qs, es, mus, T_el, Tt, rank, 0.0)
+        E = R.energy_xl_fixed(sp, x, method, D, P) - 2.0 * T_el * S
+        a = R.forces_fd(sp, x, method, P, D=D) / mass * ACC_SCALE
+        v = v + 0.5 * a * dt
+        out.append(np.stack([ek(v), E + ek(v)], 1))
+    return np.array(out)
+
+
+@pytest.mark.parametrize("names,k,T_el,nsteps", [(("methane.xyz",), 6, 1500.0, 6),
+                                                  (("methane.xyz", "benzene.xyz"), 4, 15000.0, 4)])
+def test_ksa_trajectory_matches_oracle(names, k, T_el, nsteps):
+    """Same initial velocities on both sides: Ek(t) and Ek + Etot + EEnt of the device KSA-XL-BOMD
+    trajectory equal the oracle's step by step (1e-6 eV)."""
+    sp, c = read_xyz(list(names))
+    rng = np.random.default_rng(5)
+    v0 = rng.normal(scale=0.01, size=c.shape)
+    v0[sp == 0] = 0.0
+    p = {"method": "AM1", "scf_eps": 1.0e-7, "scf_converger": [1]}
+    const = pyseqm_amd.Constants().to(DEV)
+    mol = pyseqm_amd.Molecule(const, p, torch.as_tensor(c, device=DEV), torch.as_tensor(sp, device=DEV))
+    md = pyseqm_amd.KSA_XL_BOMD(xl_bomd_params={"k": k, "max_rank": 3, "err_threshold": 0.0, "T_el": T_el},
+                                damp=None, seqm_parameters=p, timestep=0.5, Temp=300.0)
+    md.set_dof(mol)
+    mol.velocities = torch.as_tensor(v0, device=DEV)
+    md._zero_com(mol, translate_to_origin=False, restore_kinetic_energy=False)
+    v_start = mol.velocities.cpu().numpy().copy()
+    md.run(mol, steps=nsteps, reuse_P=True, remove_com=None)
+    T, Ek, Ep, Etot = md.thermo()
+    ref = _oracle_ksa(sp, c, v_start, "AM1", k, nsteps, T_el)
+    np.testing.assert_allclose(Ek, ref[:, :, 0], atol=1e-6, rtol=0)
+    np.testing.assert_allclose(Etot, ref[:, :, 1], atol=1e-6, rtol=0)
+    if T_el > 10000:
+        assert np.abs(mol.Electronic_entropy.cpu().numpy()).max() > 1e-6  # the entropy term is exercised
+
+
+@pytest.mark.parametrize("names,k,gname", [(("methane.xyz",), 6, "md_ksa_xl_bomd_methane"),
+                                           (("methane.xyz",), 4, "md_ksa_xl_bomd_methane_k4"),
+                                           (("methane.xyz", "benzene.xyz"), 6, "md_ksa_xl_bomd_batch_mixed")])
+def test_ksa_xl_bomd_trajectory_matches_golden(names, k, gname):
+    """The reference's own KSA MD tests (tests/unit/test_md_suite.py:261-327): 30 steps of
+    KSA_XL_BOMD (max_rank 3, err_threshold 0, T_el 1500 K) from Maxwell-Boltzmann velocities drawn
+    on the device with seed 0; Etot(t) within the reference's 1e-2, drift within 5e-2.  Step 0
+    (SCF energy + the kinetic energy fixed by T = 300 K) is compared tightly."""
+    torch.manual_seed(0)
+    sp, c = read_xyz(list(names))
+    p = {"method": "AM1", "scf_eps": 1.0e-7, "scf_converger": [1]}
+    const = pyseqm_amd.Constants().to(DEV)
+    mol = pyseqm_amd.Molecule(const, p, torch.as_tensor(c, device=DEV), torch.as_tensor(sp, device=DEV))
+    md = pyseqm_amd.KSA_XL_BOMD(xl_bomd_params={"k": k, "max_rank": 3, "err_threshold": 0.0, "T_el": 1500},
+                                damp=None, seqm_parameters=p, timestep=0.5, Temp=300.0)
+    md.run(mol, steps=30, reuse_P=True, remove_com=None, seed=0)
+    T, Ek, Ep, Etot = md.thermo()
+    for m, g in enumerate(golden(gname)):
+        print(gname, m, "max |dEtot| vs golden:", np.abs(Etot[:, m] - np.array(g["Etot"])).max(), flush=True)
+        assert abs(Etot[0, m] - g["Etot"][0]) < 1e-8
+        np.testing.assert_allclose(Etot[:, m], g["Etot"], atol=1e-2, rtol=1e-2)
+        assert np.abs(Etot[:, m] - Etot[0, m]).max() <= 5e-2
+    assert Ek.min() >= 0.0
