@@ -409,8 +409,14 @@ def main():
             "note": "speedup_vs_fp64_eigh compares with the headline FP64 run (eps %g, DIIS test %g); "
                     "fp64_eigh_same_tolerances reruns the FP64 path at the SP2 run's eps and DIIS test"
                     % (args.eps, eng.diis_tol)}
+    c3_cpu = None
     if args.sp2_roofline_config:
         m3, sp3, xyz3 = make_workload(args.sp2_roofline_config, args.seed)
+        if pool is not None:
+            # CPU baseline of this workload: the oracle's FP64 single point of its first molecule on
+            # one worker (one BLAS thread), running while the GPU phases below execute
+            K3 = int((sp3[:1] > 0).sum())
+            c3_cpu = pool.apply_async(_oracle_chunk, ((0, m3, sp3[:1, :K3], xyz3[:1, :K3], args.eps, conv),))
         e3 = SCFEngine(sp3, None, m3, dev)
         c3 = torch.as_tensor(xyz3, dtype=torch.float64, device=dev)
 
@@ -466,6 +472,14 @@ def main():
             "fock": dict(fock_roofline(e4, r4.P, args.fock_reps), traffic=measured_traffic(args.scale_config)),
             "sp2": sp2_roofline(e4, step4)}
         del e4
+    if c3_cpu is not None:
+        _, hf_o, conv_o, sec = c3_cpu.get()
+        line["cpu_baseline_c3"] = {
+            "value": 1.0 / sec, "unit": "molecules/s", "cores": 1, "kind": "port",
+            "sample": f"molecule 0 of {args.sp2_roofline_config} ({int(e3.b.norb[0])} orbitals, {m3}), oracle FP64 "
+                      f"numpy single point at the FP64 run's eps / mixer, 1 worker x 1 BLAS thread, {sec:.1f} s",
+            "converged": conv_o, "max_abs_dHf_vs_gpu_eV": float(abs(hf_o[0] - float(hf6[0]))),
+            "gpu_fp64_molecules_per_s": e3.b.nmol / t6, "gpu_sp2_molecules_per_s": e3.b.nmol / t3}
     if args.serve_window > 0:
         sv = serve(args, dev)
         line["serving"] = sv
