@@ -147,7 +147,7 @@ __device__ inline double start_value(int j, int k) {
   return (double)h * (2.0 / 4294967296.0) - 1.0;
 }
 
-// EVEC = false: the density P = 2 C_occ C_occ^T of the nv lowest vectors.  EVEC = true: every
+// EVEC = false: the density P = 2 C_occ C_occ^T of the nocc lowest vectors.  EVEC = true: every
 // eigenpair (nv = n vectors), written as Qout (packed layout, column j = eigenvector j) and evals
 // (orb_off[m] + j, the Rayleigh quotients of the final vectors: exact to rounding whatever the
 // multisection bracket width).
@@ -185,8 +185,9 @@ __global__ __launch_bounds__(GR* GC) void k_eigh_density(pyseqm_layout_t L, cons
 
   // ---- 1. Householder tridiagonalisation (dsytd2, lower) in registers, two barriers per step.
   // Thread (tr, tc) of the GR x GC grid holds A[tr + GR a][tc + GC b], a < RR, b < RC.
-  // Step i: the owners of column i (tc == i mod GC) write x = A[i+1:, i] to LDS with partial sums
-  // of |x[1:]|^2; every thread forms tau, beta and v = (1, x[1:] / (alpha - beta)) at its columns;
+  // Step i: x = A[i+1:, i] is in LDS with partial sums of |x[1:]|^2 (written by the owners of
+  // column i, tc == i mod GC, during the previous step's update); every thread forms tau, beta
+  // and v = (1, x[1:] / (alpha - beta)) at its columns;
   // p = tau A v (row sums over the lanes of each row group, rows <= i masked by v = 0), with
   // partial sums of p.v; then A -= v w^T + w v^T, w = p - (tau / 2)(p.v) v, on every slot (v = w
   // = 0 at indices <= i leave the finished rows and columns alone).
@@ -199,27 +200,26 @@ __global__ __launch_bounds__(GR* GC) void k_eigh_density(pyseqm_layout_t L, cons
       const int r = tr + GR * a, c = tc + GC * b;
       Ar[a][b] = (r < n && c < n) ? Fm[(size_t)r * n + c] : 0.0;
     }
+  // column 0 (the owners of column 0, tc == 0, slot b = 0); later columns are written by the
+  // rank-2 update of the step before, which computes them
+  if (tc == 0) {
+    double part = 0.0;
+#pragma unroll
+    for (int a = 0; a < RR; ++a) {
+      const int r = tr + GR * a;
+      const double val = Ar[a][0];
+      if (r == 0) {
+        d[0] = val;
+      } else if (r < n) {
+        xs0[r] = val;
+        if (r > 1) part += val * val;
+      }
+    }
+    red1[tr] = part;
+  }
   for (int i = 0; i + 2 < n; ++i) {
     double* x = (i & 1) ? xs1 : xs0;
-    const int bi = i / GC, ti = i - bi * GC;
-    if (tc == ti) {
-      double part = 0.0;
-#pragma unroll
-      for (int a = 0; a < RR; ++a) {
-        const int r = tr + GR * a;
-        double val = 0.0;
-#pragma unroll
-        for (int b = 0; b < RC; ++b)
-          if (b == bi) val = Ar[a][b];
-        if (r == i) {
-          d[i] = val;
-        } else if (r > i && r < n) {
-          x[r] = val;
-          if (r > i + 1) part += val * val;
-        }
-      }
-      red1[tr] = part;
-    }
+    double* xnext = (i & 1) ? xs0 : xs1;
     __syncthreads();
     double xn2 = 0.0;
 #pragma unroll
@@ -244,13 +244,14 @@ __global__ __launch_bounds__(GR* GC) void k_eigh_density(pyseqm_layout_t L, cons
     for (int a = 0; a < RR; ++a) s[a] = 0.0;
 #pragma unroll
     for (int b = 0; b < RC; ++b) {
-      if (GC * b + GC - 1 <= i) continue;  // every column of the slot is finished: v = 0 (uniform)
+      // slots whose columns are all finished (v = 0) or all beyond the block are skipped (uniform)
+      if (GC * b + GC - 1 <= i || GC * b >= n) continue;
       const int c = tc + GC * b;
       const double vb = vcol(c);
       if (tr == 0 && c > i && c < n) HH[(size_t)i * nmax + c] = vb;
 #pragma unroll
       for (int a = 0; a < RR; ++a)
-        if (GR * a + GR - 1 > i) s[a] += Ar[a][b] * vb;  // rows of finished slots are not needed
+        if (GR * a + GR - 1 > i && GR * a < n) s[a] += Ar[a][b] * vb;  // live rows only
     }
     double q = 0.0;
 #pragma unroll
@@ -280,16 +281,33 @@ __global__ __launch_bounds__(GR* GC) void k_eigh_density(pyseqm_layout_t L, cons
         wr[a] = pp[r] + a2 * vr[a];
       }
     }
+    // the owners of column i + 1 (tc == (i + 1) mod GC, slot bn) hand it to the next step as they
+    // update it: the reflector input x, its partial squared norm and the diagonal element
+    const int bn = (i + 1) / GC, tn = (i + 1) - bn * GC;
+    double partn = 0.0;
 #pragma unroll
     for (int b = 0; b < RC; ++b) {
-      if (GC * b + GC - 1 <= i) continue;  // v = w = 0 on every column of the slot
+      if (GC * b + GC - 1 <= i || GC * b >= n) continue;  // v = w = 0 on every column of the slot
       const int c = tc + GC * b;
       const double vb = vcol(c);
       const double wb = (c > i && c < n) ? pp[c] + a2 * vb : 0.0;
 #pragma unroll
       for (int a = 0; a < RR; ++a)
-        if (GR * a + GR - 1 > i) Ar[a][b] = fma(-vr[a], wb, fma(-wr[a], vb, Ar[a][b]));
+        if (GR * a + GR - 1 > i && GR * a < n) {
+          const double val = fma(-vr[a], wb, fma(-wr[a], vb, Ar[a][b]));
+          Ar[a][b] = val;
+          if (b == bn && tc == tn) {
+            const int r = tr + GR * a;
+            if (r == i + 1) {
+              d[i + 1] = val;
+            } else if (r > i + 1 && r < n) {
+              xnext[r] = val;
+              if (r > i + 2) partn += val * val;
+            }
+          }
+        }
     }
+    if (tc == tn) red1[tr] = partn;
   }
   // the trailing 2 x 2 block (or the 1 x 1 of a one-orbital block)
 #pragma unroll
@@ -505,12 +523,12 @@ __global__ __launch_bounds__(GR* GC) void k_eigh_density(pyseqm_layout_t L, cons
           if (GR * c >= nv) continue;  // no column of this slot exists (uniform)
           double sd = 0.0;
 #pragma unroll
-          for (int b = 0; b < RC; ++b)
-            if (16 * b + 15 > i) sd += Ar[c][b] * vcur[b];  // slots wholly at rows <= i have v = 0
+          for (int b = 0; b < RC; ++b)  // slots wholly at rows <= i (v = 0) or beyond the block skipped
+            if (16 * b + 15 > i && 16 * b < n) sd += Ar[c][b] * vcur[b];
           sd = tcur * row16_sum(sd);
 #pragma unroll
           for (int b = 0; b < RC; ++b)
-            if (16 * b + 15 > i) Ar[c][b] = fma(-vcur[b], sd, Ar[c][b]);
+            if (16 * b + 15 > i && 16 * b < n) Ar[c][b] = fma(-vcur[b], sd, Ar[c][b]);
         }
       }
 #pragma unroll
