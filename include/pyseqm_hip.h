@@ -64,6 +64,10 @@ typedef struct {
   const long long* tile_woff; /* [ntiles_total] offset (doubles) of the tile's integrals in its kind's
                                 buffer: NW rows (XX 100, XH 10, HH 1) of round_up(#valid pairs, 16)
                                 doubles, valid lanes packed in lane order */
+  int max_mol_atoms;         /* atoms of the largest molecule (0 = not given: the Fock build then
+                                takes its tile-parallel kernels for every molecule) */
+  int max_mol_slots;         /* max over molecules of 10 * heavy atoms + hydrogens */
+  const int* mol_order;      /* [nmol] molecules by decreasing Fock work, or NULL (batch order) */
 } pyseqm_layout_t;
 
 /* Builds the pair tiles of a packed batch on the device: the pair enumeration of the reference
@@ -131,8 +135,11 @@ int pyseqm_pair_forces_xl(const pyseqm_layout_t* L, int method, const double* co
 /* Fock build F = H + G(P) with the electronic energy 0.5*sum P*(H+F) per molecule fused in.
  * Replaces fock() (seqm/seqm_functions/fock.py:132-347) and elec_energy (energy.py:26-49).
  *   active [nmol] (may be NULL = all): molecules with active[m] == 0 are skipped.
- *   part [ntiles_total*16*10], etile [ntiles_total], eatom [natoms]: scratch.
+ *   part [ntiles_total*16*10], etile [ntiles_total], eatom [natoms]: scratch (untouched when
+ *     every molecule has at most PYSEQM_FOCK_MOL_ATOMS atoms: one workgroup per molecule then
+ *     builds F and Eelec with the partial sums in LDS).
  *   eelec [nmol] output. */
+#define PYSEQM_FOCK_MOL_ATOMS 24
 int pyseqm_fock(const pyseqm_layout_t* L, const double* atom_par, const double* w_xx,
                 const double* w_xh, const double* w_hh, const double* P, const double* H,
                 double* F, double* part, double* etile, double* eatom, double* eelec,

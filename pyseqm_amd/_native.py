@@ -24,6 +24,7 @@ class Layout(ctypes.Structure):
         ("mol_mat_off", c_void_p), ("mol_tile_off", c_void_p), ("atom_mol", c_void_p),
         ("atom_ao", c_void_p), ("atom_Z", c_void_p), ("tiles", c_void_p), ("csr_off", c_void_p),
         ("csr_idx", c_void_p), ("tile_woff", c_void_p),
+        ("max_mol_atoms", c_int), ("max_mol_slots", c_int), ("mol_order", c_void_p),
     ]
 
 

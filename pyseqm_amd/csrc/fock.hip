@@ -328,6 +328,510 @@ __global__ __launch_bounds__(256) void k_fock_diag(pyseqm_layout_t L, const doub
   }
 }
 
+// ---------------------------------------------------------------------------------------------
+// Per-molecule Fock build (molecules of at most PYSEQM_FOCK_MOL_ATOMS atoms): one workgroup of NW
+// waves per molecule does the pair tiles, the Coulomb sums and the diagonal blocks in one launch,
+// with the sums in LDS instead of tile partials in global memory.
+//
+// Work items of a molecule, dealt to the waves in a fixed round robin (item k -> wave k mod NW):
+// every heavy-heavy tile twice (exchange block and F; Coulomb sums), then groups of two heavy-H
+// and of four H-H tiles whose loads are all issued before any is used.  The two passes over a
+// heavy-heavy tile stream the same integrals on neighbouring waves, so the second pass reads them
+// from L2.  Each wave adds its Coulomb sums into its own per-atom LDS slots (10 per heavy atom, 1
+// per hydrogen; heavy atoms first), in its fixed item order, so the sums do not depend on timing;
+// after a barrier one thread per atom adds the NW slots in wave order, the one-center terms and H,
+// and writes the diagonal block.
+//
+// Where it pays: a molecule's items run one after another on its NW waves, so the molecule with
+// the most items sets a floor under the launch, while the tile-parallel kernels spread every tile
+// over the whole chip but pay five launches and the partial round trip through global memory.
+// On batches of 1024 molecules the per-molecule kernel is faster up to ~24 atoms per molecule
+// (16 benzenes 10 us vs 23 us; (H2O)_8 13 vs 26 us) and slower from ~30 atoms on (alkanes of 2-30
+// carbons 171-181 us vs 157 us; (H2O)_40 108-115 vs 72 us), hence the PYSEQM_FOCK_MOL_ATOMS rule.
+
+// 64-bit DPP move (two 32-bit halves).
+template <int CTRL>
+__device__ inline double dpp_d(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+// v + the value of the lane 16 (32) lanes away, with v_permlane16_swap (v_permlane32_swap): the
+// swap leaves the even-row (lower-half) value in one register and the odd-row (upper-half) value
+// in the other in both partner lanes, so both add the same two numbers in the same order.
+__device__ inline double swap16_sum(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const auto lo = __builtin_amdgcn_permlane16_swap((unsigned)b, (unsigned)b, false, false);
+  const auto hi = __builtin_amdgcn_permlane16_swap((unsigned)(b >> 32), (unsigned)(b >> 32), false, false);
+  const double x = __longlong_as_double((long long)(((unsigned long long)hi[0] << 32) | lo[0]));
+  const double y = __longlong_as_double((long long)(((unsigned long long)hi[1] << 32) | lo[1]));
+  return x + y;
+}
+__device__ inline double swap32_sum(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)b, (unsigned)b, false, false);
+  const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)(b >> 32), (unsigned)(b >> 32), false, false);
+  const double x = __longlong_as_double((long long)(((unsigned long long)hi[0] << 32) | lo[0]));
+  const double y = __longlong_as_double((long long)(((unsigned long long)hi[1] << 32) | lo[1]));
+  return x + y;
+}
+// Sum over the 8 lanes of a tile row (lanes 8r..8r+7): xor 1, xor 2 (quad permutations), then the
+// other quad (half-row mirror).  Sum over the 8 lanes of a tile column (c, c+8, ..., c+56): xor 8
+// (rotate a 16-lane row by 8), xor 16, xor 32.  Every lane of the row (column) ends with the same
+// bits.  All 64 lanes must be active.
+__device__ inline double row8(double v) {
+  v += dpp_d<0xB1>(v);
+  v += dpp_d<0x4E>(v);
+  v += dpp_d<0x141>(v);
+  return v;
+}
+__device__ inline double col8(double v) {
+  v += dpp_d<0x128>(v);
+  v = swap16_sum(v);
+  return swap32_sum(v);
+}
+__device__ inline double wave64(double v) { return col8(row8(v)); }
+
+// Element k (0..9) of a per-lane 10-vector, k varying over lanes, without dynamic register indexing.
+__device__ inline double pick10(const double (&v)[10], int k) {
+  double x = v[0];
+#pragma unroll
+  for (int t = 1; t < 10; ++t) x = k == t ? v[t] : x;
+  return x;
+}
+
+// Local LDS slot of atom la (index inside the molecule; nh heavy atoms first) and its first orbital
+// (4 per heavy atom, then 1 per hydrogen).
+__device__ inline int mslot(int la, int nh) { return la < nh ? 10 * la : 9 * nh + la; }
+__device__ inline int mao(int la, int nh) { return la < nh ? 4 * la : 3 * nh + la; }
+
+// A pair tile in local atom indices, with the offset of its integrals.
+struct MTile {
+  int row0, col0, nrow, ncol, diag;
+  long long woff;
+};
+// A tile record read from LDS, made wave-uniform (scalar registers) so that the integral offsets
+// and strides are scalar and the loads take a scalar base plus the lane's offset.
+__device__ inline MTile uniform_tile(const MTile* tl, int u) {
+  const MTile x = tl[u];
+  MTile t;
+  t.row0 = __builtin_amdgcn_readfirstlane(x.row0);
+  t.col0 = __builtin_amdgcn_readfirstlane(x.col0);
+  t.nrow = __builtin_amdgcn_readfirstlane(x.nrow);
+  t.ncol = __builtin_amdgcn_readfirstlane(x.ncol);
+  t.diag = __builtin_amdgcn_readfirstlane(x.diag);
+  const unsigned long long wo = (unsigned long long)x.woff;
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)wo);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(wo >> 32));
+  t.woff = (long long)(((unsigned long long)hi << 32) | lo);
+  return t;
+}
+__device__ inline bool mlane_pair(const MTile& t, int lane, int& i, int& j) {
+  const int r = lane >> 3, c = lane & 7;
+  i = t.row0 + r;
+  j = t.col0 + c;
+  return r < t.nrow && c < t.ncol && (!t.diag || c > r);
+}
+__device__ inline int mlane_rank(const MTile& t, int lane) {
+  const int r = lane >> 3, c = lane & 7;
+  return t.diag ? r * t.nrow - r * (r + 1) / 2 + (c - r - 1) : r * t.ncol + c;
+}
+__device__ inline int mwstride(const MTile& t) {
+  const int nv = t.diag ? t.nrow * (t.nrow - 1) / 2 : t.nrow * t.ncol;
+  return (nv + 15) & ~15;
+}
+
+// The 10 integrals (k la si), la si = 0..9, of integral row k of a heavy-heavy pair: wb is the
+// tile's (wave-uniform) integral base, S its row stride, rk the lane's rank in the tile; the row
+// address is scalar and the lane adds its offset.
+__device__ inline void load_row10(const double* __restrict__ wb, int S, unsigned rk, int k, double (&v)[10]) {
+#pragma unroll
+  for (int l = 0; l < 10; ++l) v[l] = __builtin_nontemporal_load(wb + (size_t)(k * 10 + l) * S + rk);
+}
+
+// Heavy-heavy tile, exchange pass: F_ij = H_ij - 1/2 K_ij in both triangles, energy share.
+__device__ inline void mol_xx_exchange(const MTile& t, const double* __restrict__ w, const double* Pm,
+                                       const double* Hm, double* Fm, int n, int lane, double& e) {
+  int i, j;
+  if (!mlane_pair(t, lane, i, j)) return;
+  const int S = mwstride(t);
+  const double* wb = w + t.woff;
+  const unsigned rk = (unsigned)mlane_rank(t, lane);
+  const int ai = 4 * i, aj = 4 * j;
+  // K starts at -2 H, so that F = -K/2 at the end (H is read with P, up front, and not kept)
+  double pab[4][4], K[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const d2u u0 = ld2(Pm + (size_t)(ai + a) * n + aj), u1 = ld2(Pm + (size_t)(ai + a) * n + aj + 2);
+    const d2u h0 = ld2(Hm + (size_t)(ai + a) * n + aj), h1 = ld2(Hm + (size_t)(ai + a) * n + aj + 2);
+    pab[a][0] = u0[0];
+    pab[a][1] = u0[1];
+    pab[a][2] = u1[0];
+    pab[a][3] = u1[1];
+    const double h[4] = {h0[0], h0[1], h1[0], h1[1]};
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      e += pab[a][b] * h[b];
+      K[a][b] = -2.0 * h[b];
+    }
+  }
+  // integral rows k = (mu nu), 10 integrals (la si) each, loaded one row ahead of their use
+  double cur[10], nxt[10];
+  load_row10(wb, S, rk, 0, cur);
+#pragma unroll
+  for (int k = 0; k < 10; ++k) {
+    const int mu = kTriI[k], nu = kTriJ[k];
+    if (k < 9) load_row10(wb, S, rk, k + 1, nxt);
+#pragma unroll
+    for (int l = 0; l < 10; ++l) {
+      const int la = kTriI[l], si = kTriJ[l];
+      const double v = cur[l];
+      K[mu][la] += v * pab[nu][si];
+      if (mu != nu) K[nu][la] += v * pab[mu][si];
+      if (la != si) K[mu][si] += v * pab[nu][la];
+      if (mu != nu && la != si) K[nu][si] += v * pab[mu][la];
+    }
+#pragma unroll
+    for (int l = 0; l < 10; ++l) cur[l] = nxt[l];
+  }
+  double f[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      f[a][b] = -0.5 * K[a][b];
+      e += pab[a][b] * f[a][b];
+    }
+    st2(Fm + (size_t)(ai + a) * n + aj, f[a][0], f[a][1]);
+    st2(Fm + (size_t)(ai + a) * n + aj + 2, f[a][2], f[a][3]);
+  }
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    st2(Fm + (size_t)(aj + b) * n + ai, f[0][b], f[1][b]);
+    st2(Fm + (size_t)(aj + b) * n + ai + 2, f[2][b], f[3][b]);
+  }
+}
+
+// Heavy-heavy tile, Coulomb pass: atom i += sum_{la si} P_jj (mu nu|la si), atom j += sum_{mu nu}
+// P_ii (mu nu|la si), off-diagonal density elements doubled (fock.py:259-290).
+__device__ inline void mol_xx_coulomb(const MTile& t, const double* __restrict__ w, const double* pdg,
+                                      double* acc, int lane) {
+  int i, j;
+  const bool valid = mlane_pair(t, lane, i, j);
+  double rowv[10], colv[10];
+#pragma unroll
+  for (int k = 0; k < 10; ++k) rowv[k] = colv[k] = 0.0;
+  if (valid) {
+    const int S = mwstride(t);
+    const double* wb = w + t.woff;
+    const unsigned rk = (unsigned)mlane_rank(t, lane);
+    const double* pi = pdg + 10 * i;  // P_ii read from LDS one element per integral row
+    const double* pj = pdg + 10 * j;
+    double pb[10];
+#pragma unroll
+    for (int k = 0; k < 10; ++k) pb[k] = (kTriI[k] == kTriJ[k] ? 1.0 : 2.0) * pj[k];
+    double cur[10], nxt[10];
+    load_row10(wb, S, rk, 0, cur);
+#pragma unroll
+    for (int k = 0; k < 10; ++k) {
+      if (k < 9) load_row10(wb, S, rk, k + 1, nxt);
+      const double pa = (kTriI[k] == kTriJ[k] ? 1.0 : 2.0) * pi[k];
+#pragma unroll
+      for (int l = 0; l < 10; ++l) {
+        colv[l] += cur[l] * pa;
+        rowv[k] += cur[l] * pb[l];
+      }
+#pragma unroll
+      for (int l = 0; l < 10; ++l) cur[l] = nxt[l];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 10; ++k) {
+    rowv[k] = row8(rowv[k]);
+    colv[k] = col8(colv[k]);
+  }
+  const int r = lane >> 3, c = lane & 7;
+  if (r < t.nrow) {  // lane (r, c) adds elements c and c + 8 of row atom r
+    double* d = acc + 10 * (t.row0 + r);
+    d[c] += pick10(rowv, c);
+    if (c < 2) d[8 + c] += pick10(rowv, 8 + c);
+  }
+  if (c < t.ncol) {  // lane (r, c) adds elements r and r + 8 of column atom c
+    double* d = acc + 10 * (t.col0 + c);
+    d[r] += pick10(colv, r);
+    if (r < 2) d[8 + r] += pick10(colv, 8 + r);
+  }
+}
+
+// G heavy-H tiles (rows heavy, columns H) u0 .. u0+G-1 of the local list, those below u1: every
+// tile's loads are issued before any of them is used.  Per tile: exchange column, F in both
+// triangles, Coulomb sums, energy.
+template <int G>
+__device__ inline void mol_xh(const MTile* tl, int u0, int u1, const double* __restrict__ w, const double* Pm,
+                              const double* Hm, double* Fm, int n, const double* pdg, double* acc, int nh, int lane,
+                              double& e) {
+  MTile t[G];
+  bool valid[G];
+  int ai[G], aj[G];
+  double wv[G][10], pab[G][4], hab[G][4];
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    valid[g] = false;
+    t[g].nrow = t[g].ncol = 0;
+    if (u0 + g < u1) {
+      t[g] = uniform_tile(tl, u0 + g);
+      int i, j;
+      valid[g] = mlane_pair(t[g], lane, i, j);
+      ai[g] = 4 * i;
+      aj[g] = 3 * nh + j;
+      if (valid[g]) {
+        const int S = mwstride(t[g]);
+        const double* wb = w + t[g].woff;
+        const unsigned rk = (unsigned)mlane_rank(t[g], lane);
+#pragma unroll
+        for (int k = 0; k < 10; ++k) wv[g][k] = __builtin_nontemporal_load(wb + (size_t)k * S + rk);
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+          pab[g][a] = Pm[(size_t)(ai[g] + a) * n + aj[g]];
+          hab[g][a] = Hm[(size_t)(ai[g] + a) * n + aj[g]];
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    if (u0 + g >= u1) break;  // uniform over the wave
+    double rowv[10], colv = 0.0;
+#pragma unroll
+    for (int k = 0; k < 10; ++k) rowv[k] = 0.0;
+    if (valid[g]) {
+      const int i = ai[g] / 4, j = aj[g] - 3 * nh;
+      const double* pi = pdg + 10 * i;
+      const double pb = pdg[9 * nh + j];
+      double K[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int k = 0; k < 10; ++k) {
+        const int mu = kTriI[k], nu = kTriJ[k];
+        const double v = wv[g][k];
+        colv += v * ((mu == nu ? 1.0 : 2.0) * pi[k]);
+        rowv[k] = v * pb;
+        K[mu] += v * pab[g][nu];
+        if (mu != nu) K[nu] += v * pab[g][mu];
+      }
+      double f[4];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        f[a] = hab[g][a] - 0.5 * K[a];
+        Fm[(size_t)(ai[g] + a) * n + aj[g]] = f[a];
+        e += pab[g][a] * (hab[g][a] + f[a]);
+      }
+      st2(Fm + (size_t)aj[g] * n + ai[g], f[0], f[1]);
+      st2(Fm + (size_t)aj[g] * n + ai[g] + 2, f[2], f[3]);
+    }
+#pragma unroll
+    for (int k = 0; k < 10; ++k) rowv[k] = row8(rowv[k]);
+    colv = col8(colv);
+    const int r = lane >> 3, c = lane & 7;
+    if (r < t[g].nrow) {
+      double* d = acc + 10 * (t[g].row0 + r);
+      d[c] += pick10(rowv, c);
+      if (c < 2) d[8 + c] += pick10(rowv, 8 + c);
+    }
+    if (r == 0 && c < t[g].ncol) acc[9 * nh + t[g].col0 + c] += colv;
+  }
+}
+
+// G H-H tiles u0 .. u0+G-1 (those below u1), loads first.
+template <int G>
+__device__ inline void mol_hh(const MTile* tl, int u0, int u1, const double* __restrict__ w, const double* Pm,
+                              const double* Hm, double* Fm, int n, const double* pdg, double* acc, int nh, int lane,
+                              double& e) {
+  MTile t[G];
+  bool valid[G];
+  int ai[G], aj[G];
+  double v[G], pij[G], h[G];
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    valid[g] = false;
+    t[g].nrow = t[g].ncol = 0;
+    if (u0 + g < u1) {
+      t[g] = uniform_tile(tl, u0 + g);
+      int i, j;
+      valid[g] = mlane_pair(t[g], lane, i, j);
+      ai[g] = 3 * nh + i;
+      aj[g] = 3 * nh + j;
+      if (valid[g]) {
+        v[g] = __builtin_nontemporal_load((w + t[g].woff) + (unsigned)mlane_rank(t[g], lane));
+        pij[g] = Pm[(size_t)ai[g] * n + aj[g]];
+        h[g] = Hm[(size_t)ai[g] * n + aj[g]];
+      }
+    }
+  }
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    if (u0 + g >= u1) break;  // uniform over the wave
+    double rowv = 0.0, colv = 0.0;
+    if (valid[g]) {
+      rowv = v[g] * pdg[aj[g] + 6 * nh];  // slot 9 nh + j of H atom j, orbital 3 nh + j
+      colv = v[g] * pdg[ai[g] + 6 * nh];
+      const double f = h[g] - 0.5 * v[g] * pij[g];
+      Fm[(size_t)ai[g] * n + aj[g]] = f;
+      Fm[(size_t)aj[g] * n + ai[g]] = f;
+      e += pij[g] * (h[g] + f);
+    }
+    rowv = row8(rowv);
+    colv = col8(colv);
+    const int r = lane >> 3, c = lane & 7;
+    if (c == 0 && r < t[g].nrow) acc[9 * nh + t[g].row0 + r] += rowv;
+    if (r == 0 && c < t[g].ncol) acc[9 * nh + t[g].col0 + c] += colv;
+  }
+}
+
+#ifndef PYSEQM_FOCK_MOL_WAVES
+#define PYSEQM_FOCK_MOL_WAVES 4  // waves per molecule workgroup
+#endif
+#ifndef PYSEQM_FOCK_MOL_OCC
+#define PYSEQM_FOCK_MOL_OCC 3  // waves per SIMD the register allocation must leave room for
+#endif
+// Tiles of a molecule of at most PYSEQM_FOCK_MOL_ATOMS atoms: at most B (B + 1) / 2 for the
+// B = A / 8 + 1 blocks of 8 atoms that a heavy and a hydrogen run of A atoms in all can span.
+constexpr int kMolTiles = (PYSEQM_FOCK_MOL_ATOMS / 8 + 1) * (PYSEQM_FOCK_MOL_ATOMS / 8 + 2) / 2;
+
+template <int NW>
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(PYSEQM_FOCK_MOL_OCC))) void k_fock_mol(pyseqm_layout_t L, const double* __restrict__ par,
+                                                      const double* __restrict__ w_xx,
+                                                      const double* __restrict__ w_xh,
+                                                      const double* __restrict__ w_hh, const double* __restrict__ P,
+                                                      const double* __restrict__ H, double* __restrict__ F,
+                                                      double* __restrict__ eelec,
+                                                      const unsigned char* __restrict__ active, int stride) {
+  extern __shared__ double lds[];  // [NW][stride] Coulomb slots of each wave, then [stride] diagonal blocks of P
+  __shared__ MTile tl[kMolTiles];
+  __shared__ double ew[NW];
+  const int m = L.mol_order ? L.mol_order[blockIdx.x] : (int)blockIdx.x;
+  if (active && !active[m]) return;  // uniform over the block
+  const int tid = threadIdx.x, q = tid >> 6, lane = tid & 63;
+  const int a0 = L.mol_atom_off[m], na = L.mol_atom_off[m + 1] - a0;
+  const int n = L.mol_norb[m];
+  const int nh = (n - na) / 3;  // n = 4 nh + (na - nh)
+  const long long mo = L.mol_mat_off[m];
+  const double* Pm = P + mo;
+  const double* Hm = H + mo;
+  double* Fm = F + mo;
+  double* pdg = lds + NW * stride;
+  const int T = L.nmol + 1;
+  const int x0 = L.mol_tile_off[m], nxx = L.mol_tile_off[m + 1] - x0;
+  const int h0 = L.mol_tile_off[T + m], nxh = L.mol_tile_off[T + m + 1] - h0;
+  const int g0 = L.mol_tile_off[2 * T + m], nhh = L.mol_tile_off[2 * T + m + 1] - g0;
+  const int ntl = nxx + nxh + nhh;  // <= kMolTiles for at most PYSEQM_FOCK_MOL_ATOMS atoms
+  for (int u = tid; u < ntl; u += 64 * NW) {
+    const int tile = u < nxx ? x0 + u : u < nxx + nxh ? h0 + (u - nxx) : g0 + (u - nxx - nxh);
+    const Tile g = load_tile(L.tiles, tile);
+    MTile t;
+    t.row0 = g.row0 - a0;
+    t.col0 = g.col0 - a0;
+    t.nrow = g.nrow;
+    t.ncol = g.ncol;
+    t.diag = g.diag;
+    t.woff = L.tile_woff[tile];
+    tl[u] = t;
+  }
+  for (int x = tid; x < NW * stride; x += 64 * NW) lds[x] = 0.0;
+  if (tid < na) {
+    const int ao = mao(tid, nh);
+    if (tid < nh) {
+      double v[10];
+      ld_tri10(Pm + (size_t)ao * n + ao, n, v);
+#pragma unroll
+      for (int k = 0; k < 10; ++k) pdg[10 * tid + k] = v[k];
+    } else {
+      pdg[9 * nh + tid] = Pm[(size_t)ao * n + ao];
+    }
+  }
+  __syncthreads();
+
+  // items: two per heavy-heavy tile, then groups of GXH heavy-H and GHH H-H tiles
+  constexpr int GXH = 2, GHH = 4;
+  const int nixh = (nxh + GXH - 1) / GXH, nihh = (nhh + GHH - 1) / GHH;
+  const int items = 2 * nxx + nixh + nihh;
+  double* acc = lds + q * stride;
+  double e = 0.0;
+  for (int it = q; it < items; it += NW) {  // uniform over the wave
+    if (it < 2 * nxx) {
+      const MTile t = uniform_tile(tl, it >> 1);
+      if (it & 1)
+        mol_xx_coulomb(t, w_xx, pdg, acc, lane);
+      else
+        mol_xx_exchange(t, w_xx, Pm, Hm, Fm, n, lane, e);
+    } else if (it < 2 * nxx + nixh) {
+      const int u0 = nxx + GXH * (it - 2 * nxx);
+      mol_xh<GXH>(tl, u0, nxx + nxh, w_xh, Pm, Hm, Fm, n, pdg, acc, nh, lane, e);
+    } else {
+      const int u0 = nxx + nxh + GHH * (it - 2 * nxx - nixh);
+      mol_hh<GHH>(tl, u0, nxx + nxh + nhh, w_hh, Pm, Hm, Fm, n, pdg, acc, nh, lane, e);
+    }
+  }
+  __syncthreads();
+
+  // Diagonal blocks: H_ii + one-center G(P_ii) (fock.py:196-224) + the Coulomb slots of every wave.
+  if (tid < na) {
+    const int ao = mao(tid, nh);
+    const double* p = par + (size_t)(a0 + tid) * PYSEQM_NPAR;
+    if (tid < nh) {
+      double acc10[10], hd[10];
+      const double* pd = pdg + 10 * tid;
+      ld_tri10(Hm + (size_t)ao * n + ao, n, hd);
+#pragma unroll
+      for (int k = 0; k < 10; ++k) {
+        double sum = lds[10 * tid + k];
+#pragma unroll
+        for (int w2 = 1; w2 < NW; ++w2) sum += lds[w2 * stride + 10 * tid + k];
+        acc10[k] = sum + hd[k];
+      }
+      const double gss = p[PAR_GSS], gsp = p[PAR_GSP], gpp = p[PAR_GPP], gp2 = p[PAR_GP2], hsp = p[PAR_HSP];
+      const double pss = pd[0], ppt = pd[2] + pd[5] + pd[9];
+      const double f1 = gsp - 0.5 * hsp, f2 = 1.5 * hsp - 0.5 * gsp;
+      const double fd = 1.25 * gp2 - 0.25 * gpp, fo = 0.75 * gpp - 1.25 * gp2;
+      acc10[0] += 0.5 * pss * gss + ppt * f1;
+      acc10[2] += pss * f1 + 0.5 * pd[2] * gpp + (ppt - pd[2]) * fd;
+      acc10[5] += pss * f1 + 0.5 * pd[5] * gpp + (ppt - pd[5]) * fd;
+      acc10[9] += pss * f1 + 0.5 * pd[9] * gpp + (ppt - pd[9]) * fd;
+      acc10[1] += pd[1] * f2;
+      acc10[3] += pd[3] * f2;
+      acc10[6] += pd[6] * f2;
+      acc10[4] += pd[4] * fo;
+      acc10[7] += pd[7] * fo;
+      acc10[8] += pd[8] * fo;
+#pragma unroll
+      for (int k = 0; k < 10; ++k) e += (kTriI[k] == kTriJ[k] ? 0.5 : 1.0) * pd[k] * (hd[k] + acc10[k]);
+#pragma unroll
+      for (int mu = 0; mu < 4; ++mu) {
+        st2(Fm + (size_t)(ao + mu) * n + ao, acc10[tri_index(mu, 0)], acc10[tri_index(mu, 1)]);
+        st2(Fm + (size_t)(ao + mu) * n + ao + 2, acc10[tri_index(mu, 2)], acc10[tri_index(mu, 3)]);
+      }
+    } else {
+      const int s = 9 * nh + tid;
+      double sum = lds[s];
+#pragma unroll
+      for (int w2 = 1; w2 < NW; ++w2) sum += lds[w2 * stride + s];
+      const double pd = pdg[s], hd = Hm[(size_t)ao * n + ao];
+      const double f = sum + hd + 0.5 * pd * p[PAR_GSS];
+      Fm[(size_t)ao * n + ao] = f;
+      e += 0.5 * pd * (hd + f);
+    }
+  }
+  const double es = wave64(e);
+  if (lane == 0) ew[q] = es;
+  __syncthreads();
+  if (tid == 0) {
+    double t = ew[0];
+#pragma unroll
+    for (int w2 = 1; w2 < NW; ++w2) t += ew[w2];
+    eelec[m] = t;
+  }
+}
+
 }  // namespace
 
 namespace {
@@ -335,6 +839,14 @@ namespace {
 int launch_fock(const pyseqm_layout_t* L, const double* atom_par, const double* w_xx, const double* w_xh,
                 const double* w_hh, const double* P, const double* H, double* F, double* part, double* etile,
                 double* eatom, double* eelec, const unsigned char* active, hipStream_t s) {
+  if (L->max_mol_atoms > 0 && L->max_mol_atoms <= PYSEQM_FOCK_MOL_ATOMS) {
+    constexpr int NW = PYSEQM_FOCK_MOL_WAVES;
+    const int stride = L->max_mol_slots;
+    if (stride < 1 || stride > 10 * PYSEQM_FOCK_MOL_ATOMS) return PYSEQM_EINVAL;
+    k_fock_mol<NW><<<L->nmol, 64 * NW, (size_t)(NW + 1) * stride * sizeof(double), s>>>(
+        *L, atom_par, w_xx, w_xh, w_hh, P, H, F, eelec, active, stride);
+    return hipGetLastError() == hipSuccess ? PYSEQM_OK : PYSEQM_ELAUNCH;
+  }
   int base = 0;
   if (L->ntile[0]) k_fock_xx2<<<L->ntile[0], 128, 0, s>>>(*L, w_xx, P, H, F, part, etile, active);
   base += L->ntile[0];

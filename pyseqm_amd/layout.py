@@ -366,6 +366,14 @@ class PackedBatch:
         for name in ("mol_atom_off", "mol_norb", "mol_nocc", "mol_mat_off", "mol_tile_off", "atom_mol",
                      "atom_ao", "atom_Z", "tiles", "csr_off", "csr_idx", "tile_woff"):
             setattr(L, name, self.d[name].data_ptr())
+        # per-molecule Fock build: its LDS slots and the launch order, heaviest molecule first
+        # (bytes of the build: 1184 per heavy-heavy pair, 176 heavy-H, 32 H-H, 240 per atom)
+        nh, nH = self.nheavy.astype(np.int64), self.nhydro.astype(np.int64)
+        work = 1184 * nh * (nh - 1) // 2 + 176 * nh * nH + 32 * nH * (nH - 1) // 2 + 240 * (nh + nH)
+        self.d["mol_order"] = i32(np.argsort(-work, kind="stable") if self.nmol else np.zeros(1, np.int64))
+        L.max_mol_atoms = int(self.nat.max()) if self.nmol else 0
+        L.max_mol_slots = int((10 * nh + nH).max()) if self.nmol else 0
+        L.mol_order = self.d["mol_order"].data_ptr()
         self.L = L
 
     @classmethod

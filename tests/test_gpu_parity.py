@@ -136,6 +136,34 @@ def test_c1_benzene_batch_pm3_pulay():
     np.testing.assert_allclose(e.cpu().numpy(), ro["e"], atol=1e-6, rtol=0)
 
 
+@pytest.mark.parametrize("name", ["methanal+benzene", "water8", "benzene4"])
+def test_fock_per_molecule_kernel_matches_tile_kernels(name):
+    """Batches whose molecules have at most PYSEQM_FOCK_MOL_ATOMS atoms take the per-molecule
+    Fock kernel; the same build through the tile-parallel kernels (taken when the layout gives no
+    molecule size) agrees to 1e-12, and the per-molecule build is bitwise repeatable."""
+    sp, c = _inputs(name)
+    eng = SCFEngine(sp, c, "PM3", DEV)
+    L = eng.b.L
+    assert 0 < L.max_mol_atoms <= 24
+    g = torch.Generator(device=DEV).manual_seed(4)
+    P = eng.initial_density()
+    P = P + 0.05 * torch.randn(P.shape, generator=g, device=DEV, dtype=torch.float64)
+    F1, F2, F3 = (torch.full_like(P, float("nan")) for _ in range(3))
+    e1 = eng.fock(P, F1).clone()
+    e3 = eng.fock(P, F3).clone()
+    saved = L.max_mol_atoms
+    L.max_mol_atoms = 0
+    try:
+        e2 = eng.fock(P, F2).clone()
+    finally:
+        L.max_mol_atoms = saved
+    assert torch.equal(F1, F3) and torch.equal(e1, e3)
+    assert not torch.isnan(F1).any()
+    scale = float(F2.abs().max())
+    assert float((F1 - F2).abs().max()) <= 1e-12 * scale
+    assert float((e1 - e2).abs().max()) <= 1e-12 * float(e2.abs().max())
+
+
 def test_fock_is_deterministic():
     sp, c = W.alkanes(32, seed=2)
     eng = SCFEngine(sp, c, "AM1", DEV)

@@ -1,7 +1,8 @@
 """pyseqm_fock_noalloc (library-owned scratch, no allocation, no scratch argument) against
 pyseqm_fock (caller scratch) through the C ABI: the same kernels in the same order, so F and
-Eelec are bitwise identical.  Two calls in a row, the second on a smaller batch (the scratch
-keeps the first call's contents), plus an active-molecule mask."""
+Eelec are bitwise identical.  Two calls in a row on the tile-parallel kernels that use the
+scratch, the second on a smaller batch (the scratch keeps the first call's contents); then a
+batch of small molecules (the per-molecule kernel, no scratch), plus an active-molecule mask."""
 import numpy as np
 import pytest
 import torch
@@ -46,6 +47,10 @@ def test_fock_noalloc_equals_fock_on_two_calls_the_second_smaller():
     sp, x = W.water_clusters(1024, 1, seed=0)  # 3072 atoms, 6144 orbitals: 37 of the 48 MiB
     big = SCFEngine(sp, x, "PM3", DEV)
     _compare(big, _random_density(big, 1))
+    sp, x = W.water_clusters(64, 2, seed=1)  # 2 x 192 atoms: still above the per-molecule kernel's 128
+    mid = SCFEngine(sp, x, "PM3", DEV)
+    assert mid.b.L.max_mol_atoms > 128 and mid.b.natoms < big.b.natoms
+    _compare(mid, _random_density(mid, 3))
     sp, x = W.alkanes(16, seed=3)
     small = SCFEngine(sp, x, "AM1", DEV)
     P = _random_density(small, 2)
