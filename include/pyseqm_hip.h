@@ -119,6 +119,21 @@ int pyseqm_pair_integrals(const pyseqm_layout_t* L, int method, const double* co
  * part: tile partial scratch as in pyseqm_pair_integrals. */
 int pyseqm_pair_forces(const pyseqm_layout_t* L, int method, const double* coords, const double* atom_par,
                        const double* P, double h, double* part, double* forces, void* stream);
+
+/* Energy gradient with respect to the atom parameters at fixed density P: out[natoms][2][24] holds
+ * dEelec/d(slot) and dEnuc/d(slot) for each atom's 24 input slots (PAR_USS .. the Gaussians).  At a
+ * converged SCF density this is the exact gradient of Etot, the one the reference's autograd
+ * returns for learned parameters (basics.py:781-789, Molecule.py:82-88) with scf_backward = 0
+ * (scf_loop.py:1996-2031).  Pair terms: 4th-order central differences of each pair's energy in
+ * z_s, z_p, beta_s, beta_p, g_ss, g_pp, g_p2, h_sp (through the multipoles, overlaps and
+ * resonance integrals), alpha and the Gaussian terms, step hrel * max(|value|, 0.1), overlap
+ * branches pinned; one-center terms (U_ss, U_pp, g_ss, g_sp, g_pp, g_p2, h_sp) in closed form.
+ *   rec: pyseqm_param_grad_workspace(L) doubles; part: tile partial scratch as in
+ *   pyseqm_pair_integrals. */
+long long pyseqm_param_grad_workspace(const pyseqm_layout_t* L);
+int pyseqm_param_grad(const pyseqm_layout_t* L, int method, const double* coords, const double* atom_par,
+                      const double* P, double hrel, double* rec, double* part, double* out, void* stream);
+
 /* Core-core repulsion E_nuc(i, j) of an explicit pair list, pair_nuclear_energy
  * (seqm/seqm_functions/energy.py:91-134): idxi/idxj [npairs] atom rows of atom_par (PAR_Z, PAR_TORE,
  * PAR_ALPHA and the Gaussian K/L/M slots are read), rij [npairs] in bohr, gam [npairs] = (ss|ss). */

@@ -35,6 +35,7 @@ _SIGS = {
     "pyseqm_pair_integrals": [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                               c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
     "pyseqm_pair_forces": [c_int, c_void_p, c_void_p, c_void_p, c_double, c_void_p, c_void_p, c_void_p],
+    "pyseqm_param_grad": [c_int, c_void_p, c_void_p, c_void_p, c_double, c_void_p, c_void_p, c_void_p, c_void_p],
     "pyseqm_fock": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                     c_void_p, c_void_p, c_void_p, c_void_p],
     "pyseqm_fock_noalloc": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
@@ -119,6 +120,8 @@ def lib():
             f.restype = c_int
         L.pyseqm_fock_scratch_doubles.argtypes = [_P]
         L.pyseqm_fock_scratch_doubles.restype = c_longlong
+        L.pyseqm_param_grad_workspace.argtypes = [_P]
+        L.pyseqm_param_grad_workspace.restype = c_longlong
         L.pyseqm_eigh_density_workspace.argtypes = [c_int, c_int]
         L.pyseqm_eigh_density_workspace.restype = c_longlong
         L.pyseqm_eigh_density_lds_ok.argtypes = [c_int, c_int]
@@ -132,7 +135,8 @@ def lib():
 
 
 def exported_symbols():
-    return list(_SIGS) + list(_RAW_SIGS) + ["pyseqm_fock_scratch_doubles", "pyseqm_eigh_density_workspace",
+    return list(_SIGS) + list(_RAW_SIGS) + ["pyseqm_fock_scratch_doubles", "pyseqm_param_grad_workspace",
+                                            "pyseqm_eigh_density_workspace",
                                             "pyseqm_eigh_density_lds_ok", "pyseqm_fermi_workspace",
                                             "pyseqm_ksa_max_rank"]
 

@@ -146,11 +146,66 @@ class Energy(torch.nn.Module):
         eng = molecule.engine()
         res = _run_scf(molecule, self.hamiltonian, P0)
         en = res["energies"]
+        trained = _trained_parameters(molecule.learned_parameters)
         if not all_terms:
+            if trained:
+                raise NotImplementedError("parameter gradients of the per-pair core repulsion EnucAB "
+                                          "(all_terms=False) are not supported; use all_terms=True")
             return en["Eelec"], eng_pair_enuc(eng), res["Ppad"], res["notconverged"]
         Hf = en["Hf"] if self.Hf_flag else en["Etot"] - en["Eiso"]
-        return (Hf, en["Etot"], en["Eelec"], en["Enuc"], en["Eiso"], eng_pair_enuc(eng), res["e_gap"], res["e"],
+        Etot, Eelec, Enuc, Eiso = en["Etot"], en["Eelec"], en["Enuc"], en["Eiso"]
+        if trained:
+            names = [n for n, _ in trained]
+            Hf, Etot, Eelec, Enuc, Eiso = _ParamEnergies.apply(eng, eng._last_P, names, Hf, Etot, Eelec, Enuc, Eiso,
+                                                               *[t for _, t in trained])
+        return (Hf, Etot, Eelec, Enuc, Eiso, eng_pair_enuc(eng), res["e_gap"], res["e"],
                 res["Ppad"], res["charge"], res["notconverged"])
+
+
+def _trained_parameters(learned):
+    """(name, tensor) of the learned per-atom parameters that ask for a gradient (grad mode on)."""
+    if not torch.is_grad_enabled():
+        return []
+    out = []
+    for name, v in (learned or {}).items():
+        if torch.is_tensor(v) and v.requires_grad:
+            if name not in COLUMNS:
+                raise NotImplementedError(f"gradients with respect to {name!r} are not supported "
+                                          f"(per-atom parameters: {', '.join(COLUMNS)})")
+            out.append((name, v))
+    return out
+
+
+class _ParamEnergies(torch.autograd.Function):
+    """Hf, Etot, Eelec, Enuc, Eiso of a converged SCF as functions of the learned per-atom
+    parameters.  The backward is the fixed-density derivative (SCFEngine.param_grad), which for the
+    energy at a converged SCF density is the full derivative -- the gradient the reference's
+    autograd returns with scf_backward = 0 (scf_loop.py:1996-2031, basics.py:781-789); Eiso is
+    linear in U and the one-center integrals (energy.py:8-23)."""
+
+    @staticmethod
+    def forward(ctx, eng, P, names, Hf, Etot, Eelec, Enuc, Eiso, *params):
+        # the engine may be re-parametrised or moved before backward runs: keep what it needs
+        ctx.eng, ctx.names = eng, names
+        ctx.state = (P.detach().clone(), eng.atom_par.clone(), eng.coords.clone())
+        ctx.meta = [(p.shape, p.dtype, p.device) for p in params]
+        return Hf.clone(), Etot.clone(), Eelec.clone(), Enuc.clone(), Eiso.clone()
+
+    @staticmethod
+    def backward(ctx, gHf, gEtot, gEelec, gEnuc, gEiso):
+        eng = ctx.eng
+        P, atom_par, coords = ctx.state
+        G = eng.param_grad(P, atom_par=atom_par, coords=coords)  # (natoms, 2, 24)
+        dI = eng.eiso_grad()
+        mol = torch.as_tensor(eng.b.atom_mol, device=G.device).long()
+        dT = G[:, 0] + G[:, 1]
+        tot = torch.zeros_like(dT)
+        for g, d in ((gEelec, G[:, 0]), (gEnuc, G[:, 1]), (gEtot, dT), (gEiso, dI), (gHf, dT - dI)):
+            if g is not None:
+                tot = tot + g.to(d)[mol][:, None] * d
+        grads = [tot[:, COLUMNS.index(n)].reshape(shape).to(dtype=dtype, device=dev)
+                 for n, (shape, dtype, dev) in zip(ctx.names, ctx.meta)]
+        return (None, None, None, None, None, None, None, None, *grads)
 
 
 def eng_pair_enuc(eng):
@@ -187,6 +242,9 @@ class Force(torch.nn.Module):
         eng = molecule.engine()
         if do_force:
             force = eng.b.atom_to_padded(eng.forces(eng._last_P))
+            # the reference returns every output detached when it computes forces, and Eelec / Enuc
+            # with their parameter gradients otherwise (basics.py:1296-1365)
+            Eelec, Enuc = Eelec.detach(), Enuc.detach()
         else:
             force = torch.tensor([])
-        return force, D, Hf, Etot, Eelec, Enuc, Eiso, e, e_gap, charge, notconverged
+        return force, D, Hf.detach(), Etot.detach(), Eelec, Enuc, Eiso.detach(), e, e_gap, charge, notconverged

@@ -41,11 +41,8 @@ __device__ double secant(int which, double target, double D, double x1) {
   return x2;
 }
 
-__global__ void k_atom_setup(pyseqm_layout_t L, double* par) {
-  const int a = blockIdx.x * blockDim.x + threadIdx.x;
-  if (a >= L.natoms) return;
-  double* p = par + (size_t)a * PYSEQM_NPAR;
-  const int Z = L.atom_Z[a];
+// The derived slots (dd, qq, rho0/1/2, tore, qn, Z) of an atom-parameter record from its inputs.
+__device__ void derive_atom(double* p, int Z) {
   const double qn = kQn[Z];
   double dd = 0.0, qq = 0.0, r1 = 0.0, r2 = 0.0;
   if (Z > 2) {
@@ -69,6 +66,12 @@ __global__ void k_atom_setup(pyseqm_layout_t L, double* par) {
   p[PAR_TORE] = kTore[Z];
   p[PAR_QN] = qn;
   p[PAR_Z] = (double)Z;
+}
+
+__global__ void k_atom_setup(pyseqm_layout_t L, double* par) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= L.natoms) return;
+  derive_atom(par + (size_t)a * PYSEQM_NPAR, L.atom_Z[a]);
 }
 
 // ------------------------------------------------------------ local-frame integrals
@@ -605,7 +608,8 @@ struct PairDensity {
 
 template <int KIND, bool XL>
 __device__ double pair_energy(int method, const PairGeom& g, const double* pi, const double* pj,
-                              const PairDensity& P, const PairDensity& D, int wmode, int smode, int bsel) {
+                              const PairDensity& P, const PairDensity& D, int wmode, int smode, int bsel,
+                              double* rep_out = nullptr) {
   constexpr int NI = KIND == HH ? 1 : 4;
   constexpr int NJ = KIND == XX ? 4 : 1;
   const double ti = pi[PAR_TORE], tj = pj[PAR_TORE];
@@ -661,7 +665,9 @@ __device__ double pair_energy(int method, const PairGeom& g, const double* pi, c
   for (int a = 0; a < NI; ++a)
 #pragma unroll
     for (int b = 0; b < NJ; ++b) e += 2.0 * D.pab[a][b] * Hb[a][b];
-  return e + core_repulsion(method, (int)pi[PAR_Z], (int)pj[PAR_Z], g.r, gam, pi, pj);
+  const double rep = core_repulsion(method, (int)pi[PAR_Z], (int)pj[PAR_Z], g.r, gam, pi, pj);
+  if (rep_out) *rep_out = rep;
+  return e + rep;
 }
 
 template <int NI, int NJ>
@@ -742,6 +748,159 @@ __global__ __launch_bounds__(256) void k_atom_forces(pyseqm_layout_t L, const do
   double f[3];
   gather_slots<G>(part, L.csr_idx, L.csr_off[a], L.csr_off[a + 1], lane, f);
   if (lane < 3) F[3 * a + lane] = lane == 0 ? f[0] : (lane == 1 ? f[1] : f[2]);
+}
+
+// ------------------------------------------------- energy gradient w.r.t. atom parameters
+// At a converged closed-shell SCF density the energy is stationary in P, so dE/dtheta is the
+// derivative at fixed P: the gradient the reference's autograd returns through H, the integrals and
+// the core repulsion when the SCF density is detached (scf_backward = 0, scf_loop.py:1996-2031;
+// learned parameters re-packed per call, basics.py:781-789).  Each pair's energy (its Eelec share and
+// its core repulsion) depends on the input slots below of its two atoms, through the multipoles
+// (dd, qq, rho0/1/2), the overlaps, the resonance integrals and the core-repulsion terms; a
+// 4th-order central difference of the pair energy in each slot, with the overlap branches pinned at
+// the unperturbed values, gives that pair's share.  U_ss, U_pp and g_sp reach the one-center terms
+// only, which k_param_onecenter differentiates in closed form.
+__device__ constexpr int kPairParams[21] = {PAR_ZS,        PAR_ZP,        PAR_BS,        PAR_BP,        PAR_GSS,
+                                            PAR_GPP,       PAR_GP2,       PAR_HSP,       PAR_ALPHA,     PAR_GAUSS,
+                                            PAR_GAUSS + 1, PAR_GAUSS + 2, PAR_GAUSS + 3, PAR_GAUSS + 4, PAR_GAUSS + 5,
+                                            PAR_GAUSS + 6, PAR_GAUSS + 7, PAR_GAUSS + 8, PAR_GAUSS + 9,
+                                            PAR_GAUSS + 10, PAR_GAUSS + 11};
+constexpr int kParamGroup = 5;  // slots per pass: 2 values (Eelec, core repulsion) each in a 10-wide partial
+__host__ __device__ inline int pair_param_count(int method) {
+  return method == PYSEQM_AM1 ? 21 : (method == PYSEQM_PM3 ? 15 : 9);
+}
+// Step of slot k of a record: hrel * max(|value|, 0.1).
+__device__ inline double param_step(const double* p, int k, double hrel) { return hrel * fmax(fabs(p[k]), 0.1); }
+__device__ constexpr int kStencil[4] = {-2, -1, 1, 2};
+__device__ constexpr double kStencilW[4] = {1.0 / 12.0, -8.0 / 12.0, 8.0 / 12.0, -1.0 / 12.0};
+
+// rec[((a * kParamGroup + g) * 4 + s) * NPAR ...]: atom a's record with slot kPairParams[p0 + g]
+// moved by kStencil[s] steps and the derived slots recomputed.  One thread per (atom, g, s).
+__global__ void k_param_records(pyseqm_layout_t L, const double* __restrict__ par, int p0, int np, double hrel,
+                                double* __restrict__ rec) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = t & 3, g = (t >> 2) % kParamGroup, a = (t >> 2) / kParamGroup;
+  if (a >= L.natoms || g >= np) return;
+  const double* src = par + (size_t)a * PYSEQM_NPAR;
+  double* q = rec + ((size_t)(a * kParamGroup + g) * 4 + s) * PYSEQM_NPAR;
+  for (int k = 0; k < PYSEQM_NPAR; ++k) q[k] = src[k];
+  const int slot = kPairParams[p0 + g];
+  q[slot] = src[slot] + kStencil[s] * param_step(src, slot, hrel);
+  derive_atom(q, L.atom_Z[a]);
+}
+
+// Per pair and slot of the pass: d(pair Eelec share)/d(slot) and d(core repulsion)/d(slot) for
+// atom i (row partial values 2g, 2g + 1) and for atom j (column partial).
+template <int KIND>
+__global__ __launch_bounds__(64) void k_pair_param_grad(pyseqm_layout_t L, int method, int tile_base,
+                                                        const double* __restrict__ coords,
+                                                        const double* __restrict__ par, const double* __restrict__ P,
+                                                        const double* __restrict__ rec, int p0, int np, double hrel,
+                                                        double* __restrict__ part) {
+  const int tile = tile_base + xcd_run(blockIdx.x, gridDim.x);
+  const int lane = threadIdx.x;
+  const Tile t = load_tile(L.tiles, tile);
+  int i, j;
+  const bool valid = lane_pair(t, lane, i, j);
+  double rowv[10], colv[10];
+#pragma unroll
+  for (int k = 0; k < 10; ++k) rowv[k] = colv[k] = 0.0;
+  if (valid) {
+    const double* pi = par + (size_t)i * PYSEQM_NPAR;
+    const double* pj = par + (size_t)j * PYSEQM_NPAR;
+    const long long mo = L.mol_mat_off[t.mol];
+    const int n = L.mol_norb[t.mol];
+    constexpr int NI = KIND == HH ? 1 : 4;
+    constexpr int NJ = KIND == XX ? 4 : 1;
+    PairDensity Pd;
+    load_pair_density<NI, NJ>(P + mo, n, L.atom_ao[i], L.atom_ao[j], Pd);
+    const PairGeom g0 = pair_geom(coords, i, j);
+    const int wmode = quat_flips(-g0.x[0]) ? 2 : 1;
+    const int smode = quat_flips(g0.x[0]) ? 2 : 1;
+    const int bsel = overlap_branches(g0.r, pi[PAR_ZS], pi[PAR_ZP], pj[PAR_ZS], pj[PAR_ZP]);
+#pragma unroll 1
+    for (int g = 0; g < np; ++g) {
+      const int slot = kPairParams[p0 + g];
+#pragma unroll 1
+      for (int side = 0; side < 2; ++side) {
+        const int a = side == 0 ? i : j;
+        const double inv = 1.0 / param_step(side == 0 ? pi : pj, slot, hrel);
+        double de = 0.0, dn = 0.0;
+#pragma unroll 1
+        for (int s = 0; s < 4; ++s) {
+          const double* q = rec + ((size_t)(a * kParamGroup + g) * 4 + s) * PYSEQM_NPAR;
+          double rep = 0.0;
+          const double e = pair_energy<KIND, false>(method, g0, side == 0 ? q : pi, side == 0 ? pj : q, Pd, Pd,
+                                                    wmode, smode, bsel, &rep);
+          de += kStencilW[s] * (e - rep);
+          dn += kStencilW[s] * rep;
+        }
+        if (side == 0) {
+          rowv[2 * g] = de * inv;
+          rowv[2 * g + 1] = dn * inv;
+        } else {
+          colv[2 * g] = de * inv;
+          colv[2 * g + 1] = dn * inv;
+        }
+      }
+    }
+  }
+  write_partials(part, tile, lane, rowv, colv);
+}
+
+// out[a][0][k] += sum of atom a's pair shares of dEelec/d(slot k), out[a][1][k] += the core
+// repulsion's, for the pass's slots (fixed gather order).
+template <int G>
+__global__ __launch_bounds__(256) void k_param_gather(pyseqm_layout_t L, const double* __restrict__ part, int p0,
+                                                      int np, double* __restrict__ out) {
+  const int a = blockIdx.x * (256 / G) + threadIdx.x / G;
+  const int lane = threadIdx.x % G;
+  if (a >= L.natoms) return;  // uniform over the lane group
+  double acc[10];
+  gather_slots<G>(part, L.csr_idx, L.csr_off[a], L.csr_off[a + 1], lane, acc);
+  if (lane < np) {
+    double de = acc[0], dn = acc[1];
+#pragma unroll
+    for (int g = 1; g < kParamGroup; ++g)
+      if (lane == g) {
+        de = acc[2 * g];
+        dn = acc[2 * g + 1];
+      }
+    const int slot = kPairParams[p0 + lane];
+    out[(size_t)a * 48 + slot] += de;
+    out[(size_t)a * 48 + 24 + slot] += dn;
+  }
+}
+
+// The closed-form one-center part: E_A = sum_mu P_mumu U_mu + 1/2 sum P G1(P) over the atom's
+// block (the one-center terms of the Fock build, fock.py:196-224), differentiated in U_ss, U_pp,
+// g_ss, g_sp, g_pp, g_p2 and h_sp.  Writes every slot of out[a] (the pair passes add to it).
+__global__ void k_param_onecenter(pyseqm_layout_t L, const double* __restrict__ P, double* __restrict__ out) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= L.natoms) return;
+  const int m = L.atom_mol[a];
+  const int n = L.mol_norb[m];
+  const double* Pm = P + L.mol_mat_off[m];
+  const int ao = L.atom_ao[a];
+  double* o = out + (size_t)a * 48;
+  for (int k = 0; k < 48; ++k) o[k] = 0.0;
+  const double pss = Pm[(size_t)ao * n + ao];
+  o[PAR_USS] = pss;
+  o[PAR_GSS] = 0.25 * pss * pss;
+  if (L.atom_Z[a] > 1) {
+    double pd[10];
+#pragma unroll
+    for (int k = 0; k < 10; ++k) pd[k] = Pm[(size_t)(ao + kTriI[k]) * n + ao + kTriJ[k]];
+    const double ppt = pd[2] + pd[5] + pd[9];
+    const double psp2 = pd[1] * pd[1] + pd[3] * pd[3] + pd[6] * pd[6];
+    const double ppd2 = pd[2] * pd[2] + pd[5] * pd[5] + pd[9] * pd[9];
+    const double ppo2 = pd[4] * pd[4] + pd[7] * pd[7] + pd[8] * pd[8];
+    o[PAR_UPP] = ppt;
+    o[PAR_GSP] = pss * ppt - 0.5 * psp2;
+    o[PAR_HSP] = -0.5 * pss * ppt + 1.5 * psp2;
+    o[PAR_GPP] = 0.375 * ppd2 - 0.125 * ppt * ppt + 0.75 * ppo2;
+    o[PAR_GP2] = 0.625 * (ppt * ppt - ppd2) - 1.25 * ppo2;
+  }
 }
 
 // Diagonal core-Hamiltonian blocks: U on the diagonal plus the electron-core attractions
@@ -883,6 +1042,43 @@ extern "C" int pyseqm_pair_forces_xl(const pyseqm_layout_t* L, int method, const
                                      double* part, double* forces, void* stream) {
   if (!D || !P) return PYSEQM_EINVAL;
   return launch_pair_forces<true>(L, method, coords, atom_par, P, D, h, part, forces, stream);
+}
+
+extern "C" long long pyseqm_param_grad_workspace(const pyseqm_layout_t* L) {
+  if (!L || L->natoms < 0) return -1;
+  return (long long)L->natoms * kParamGroup * 4 * PYSEQM_NPAR;
+}
+
+extern "C" int pyseqm_param_grad(const pyseqm_layout_t* L, int method, const double* coords, const double* atom_par,
+                                 const double* P, double hrel, double* rec, double* part, double* out,
+                                 void* stream) {
+  if (!L || method < PYSEQM_MNDO || method > PYSEQM_PM3 || !(hrel > 0.0)) return PYSEQM_EINVAL;
+  if (L->natoms == 0) return PYSEQM_OK;
+  hipStream_t s = (hipStream_t)stream;
+  k_param_onecenter<<<(L->natoms + 127) / 128, 128, 0, s>>>(*L, P, out);
+  const int npp = pair_param_count(method);
+  for (int p0 = 0; p0 < npp; p0 += kParamGroup) {
+    const int np = npp - p0 < kParamGroup ? npp - p0 : kParamGroup;
+    const long long nthr = (long long)L->natoms * kParamGroup * 4;
+    k_param_records<<<(unsigned)((nthr + 255) / 256), 256, 0, s>>>(*L, atom_par, p0, np, hrel, rec);
+    int base = 0;
+    if (L->ntile[0])
+      k_pair_param_grad<XX><<<L->ntile[0], 64, 0, s>>>(*L, method, base, coords, atom_par, P, rec, p0, np, hrel,
+                                                       part);
+    base += L->ntile[0];
+    if (L->ntile[1])
+      k_pair_param_grad<XH><<<L->ntile[1], 64, 0, s>>>(*L, method, base, coords, atom_par, P, rec, p0, np, hrel,
+                                                       part);
+    base += L->ntile[1];
+    if (L->ntile[2])
+      k_pair_param_grad<HH><<<L->ntile[2], 64, 0, s>>>(*L, method, base, coords, atom_par, P, rec, p0, np, hrel,
+                                                       part);
+    if (lanes_per_atom(L) == 64)
+      k_param_gather<64><<<(L->natoms + 3) / 4, 256, 0, s>>>(*L, part, p0, np, out);
+    else
+      k_param_gather<16><<<(L->natoms + 15) / 16, 256, 0, s>>>(*L, part, p0, np, out);
+  }
+  return hipGetLastError() == hipSuccess ? PYSEQM_OK : PYSEQM_ELAUNCH;
 }
 
 extern "C" int pyseqm_atom_setup(const pyseqm_layout_t* L, double* atom_par, void* stream) {

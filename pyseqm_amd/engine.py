@@ -442,6 +442,36 @@ class SCFEngine:
                  nat.ptr(P), float(h), nat.ptr(self.part), nat.ptr(out), self.stream)
         return out[:b.natoms]
 
+    def param_grad(self, P, hrel=1.0e-4, atom_par=None, coords=None):
+        """Derivatives of the energy at fixed density P with respect to every atom's 24 input
+        parameters (COLUMNS order): (natoms, 2, 24), [:, 0] for Eelec and [:, 1] for the core
+        repulsion Enuc.  At a converged SCF density this is the exact gradient of Etot (the energy is
+        stationary in P), the one the reference's autograd returns for learned parameters with
+        scf_backward = 0 (basics.py:781-789, scf_loop.py:1996-2031).  Pair terms by 4th-order
+        central differences of each pair's energy (step hrel * max(|value|, 0.1)), one-center terms
+        in closed form.  atom_par / coords default to the engine's current ones."""
+        b = self.b
+        out = torch.zeros(max(b.natoms, 1), 2, 24, dtype=torch.float64, device=self.device)
+        rec = torch.empty(max(int(nat.lib().pyseqm_param_grad_workspace(nat.ctypes.byref(b.L))), 1),
+                          dtype=torch.float64, device=self.device)
+        ap = self.atom_par if atom_par is None else atom_par.contiguous()
+        xyz = self.coords if coords is None else coords.contiguous()
+        nat.call("pyseqm_param_grad", b.L, nat.METHODS[self.method], nat.ptr(xyz), nat.ptr(ap),
+                 nat.ptr(P), float(hrel), nat.ptr(rec), nat.ptr(self.part), nat.ptr(out), self.stream)
+        return out[:b.natoms]
+
+    def eiso_grad(self):
+        """d(isolated-atom energy)/d(input parameter) per atom, (natoms, 24): the coefficients of
+        U_ss, U_pp, g_ss, g_sp, g_pp, g_p2, h_sp in Eiso (energy.py:8-23)."""
+        b = self.b
+        Z = torch.as_tensor(b.Z, device=self.device).long()
+        tab = lambda t: torch.as_tensor(t, dtype=torch.float64, device=self.device)[Z]
+        g = torch.zeros(b.natoms, 24, dtype=torch.float64, device=self.device)
+        for name, coef in (("U_ss", USSC), ("U_pp", UPPC), ("g_ss", GSSC), ("g_sp", GSPC), ("g_pp", GPPC),
+                           ("g_p2", GP2C), ("h_sp", HSPC)):
+            g[:, COLUMNS.index(name)] = tab(coef)
+        return g
+
     def eig(self, F):
         """Orbital energies in the reference layout e (nmol, 4*molsize), zero padded, and the packed
         eigenvectors (nmol, nmax, nmax), sym_eig_trunc(eig_only=True) (diag.py:110-241)."""

@@ -1,0 +1,149 @@
+"""Energy gradients with respect to the per-atom parameters (SCFEngine.param_grad ->
+pyseqm_param_grad) and their autograd path through Energy(...)(molecule, learned_parameters=...).
+
+Pins:
+* Fixed-density derivatives against 4th-order central differences of the oracle's energy at the
+  same density (Eelec and the core repulsion Enuc separately, every input parameter the method
+  has, MNDO / AM1 / PM3; C, N, O, H with O-H and N-H pairs).  The oracle's parameters are per
+  element, so its derivative is the sum of the device's per-atom derivatives over that element's
+  atoms in each molecule.  Bar: 1e-6 (1 + |d|).  Where an element has equal s and p exponents
+  (MNDO C, N, O), its homonuclear pairs sit at x = 0 of every B integral, where the reference's
+  B_k are the x = 0 constants for |x| <= 1e-6 (diat_overlap_PM6_SP.py:522-670): its derivative
+  through x vanishes there (the device keeps the unperturbed branch), so the oracle's difference
+  step for those exponents is 1e-7, which stays inside that flat region (bar 2e-5 (1 + |d|) for
+  the larger rounding of the short step).
+* The fixed-density derivative is the derivative of the converged energy (the SCF energy is
+  stationary in P): Etot re-converged at single-atom parameter values theta +- h gives the same
+  slope to 1e-5 relative.
+* Autograd: Energy(learned_parameters=...) with tensors that require grad returns Hf, Etot, Eelec,
+  Enuc, Eiso whose backward equals param_grad / the Eiso coefficients, and Force with
+  do_force=True returns them detached, as the reference does (basics.py:1296-1365)."""
+import numpy as np
+import pytest
+import torch
+
+import pyseqm_amd
+from oracle import seqm_ref as R
+from pyseqm_amd.basics import Energy, Force
+from pyseqm_amd.engine import SCFEngine
+from pyseqm_amd.seqm_functions.parameters import COLUMNS
+from tests.conftest import read_xyz
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda"
+
+NH3 = (np.array([[7, 1, 1, 1]]), np.array([[[0.0, 0.0, 0.0], [0.9377, 0.0, -0.3816], [-0.4689, 0.8121, -0.3816],
+                                              [-0.4689, -0.8121, -0.3816]]]))
+WATER2 = (np.array([[8, 8, 1, 1, 1, 1]]),
+          np.array([[[0.0, 0.0, 0.0], [2.9, 0.1, 0.2], [0.95, 0.05, 0.1], [-0.3, 0.91, 0.0],
+                     [3.2, 1.0, 0.1], [3.3, -0.5, 0.75]]]))
+
+
+def _batch():
+    mols = [read_xyz(["methanal.1.xyz"]), read_xyz(["benzene.xyz"]), WATER2, NH3]
+    K = max(m[0].shape[1] for m in mols)
+    sp = np.zeros((len(mols), K), dtype=np.int64)
+    xyz = np.zeros((len(mols), K, 3))
+    for i, (s, x) in enumerate(mols):
+        sp[i, :s.shape[1]] = s[0]
+        xyz[i, :s.shape[1]] = x[0]
+    return sp, xyz
+
+
+def _fixed_density_energy(ob, Pp):
+    H, T, _, gam, _ = R.hcore(ob)
+    F = R.fock(ob, Pp, H, T)
+    Enuc = np.zeros(ob.nmol)
+    np.add.at(Enuc, ob.pair_mol, R.pair_nuclear_energy(ob, gam))
+    return R.elec_energy(Pp, F, H), Enuc
+
+
+@pytest.mark.parametrize("method", ["MNDO", "AM1", "PM3"])
+def test_param_grad_matches_oracle_at_fixed_density(method):
+    sp, c = _batch()
+    eng = SCFEngine(sp, c, method, DEV)
+    r = eng.scf(eps=1e-8, converger=(2,))
+    assert not r.notconverged.any()
+    G = eng.param_grad(r.P).cpu().numpy()
+    Pp = eng.b.to_padded(r.P).cpu().numpy()
+    ob = R.parse(sp, c, method)
+    Zat, mol = np.asarray(eng.b.Z), np.asarray(eng.b.atom_mol)
+    names = R.PARAM_NAMES[:12 + 3 * R.N_GAUSS[method]]
+    worst = 0.0
+    for Z in sorted(set(Zat.tolist())):
+        sel = Zat == Z
+        flat = Z > 1 and ob.prm["zeta_s"][Z] == ob.prm["zeta_p"][Z]
+        for name in names:
+            base = ob.prm[name][Z]
+            short = flat and name in ("zeta_s", "zeta_p")
+            h = 1e-7 if short else 1e-4 * max(abs(base), 0.1)
+            bar = 2e-5 if short else 1e-6
+            ev = []
+            for s in (-2, -1, 1, 2):
+                ob.prm[name][Z] = base + s * h
+                ev.append(_fixed_density_energy(ob, Pp))
+            ob.prm[name][Z] = base
+            w = (1.0, -8.0, 8.0, -1.0)
+            d_el = sum(wk * e[0] for wk, e in zip(w, ev)) / (12 * h)
+            d_nu = sum(wk * e[1] for wk, e in zip(w, ev)) / (12 * h)
+            col = COLUMNS.index(name)
+            g_el = np.bincount(mol[sel], weights=G[sel, 0, col], minlength=ob.nmol)
+            g_nu = np.bincount(mol[sel], weights=G[sel, 1, col], minlength=ob.nmol)
+            for a, b in ((g_el, d_el), (g_nu, d_nu)):
+                err = np.abs(a - b) / (1.0 + np.abs(b))
+                worst = max(worst, float(err.max()))
+                assert err.max() < bar, (method, Z, name, a, b)
+    print(f"{method}: worst relative difference {worst:.2e}", flush=True)
+
+
+@pytest.mark.parametrize("name", ["g_ss", "zeta_p", "beta_s", "alpha", "h_sp"])
+def test_fixed_density_gradient_is_the_converged_energy_slope(name):
+    sp, c = _batch()
+    eng = SCFEngine(sp, c, "AM1", DEV)
+    r = eng.scf(eps=1e-10, converger=(2,))
+    G = eng.param_grad(r.P)
+    col = COLUMNS.index(name)
+    a = int(np.nonzero(np.asarray(eng.b.Z) == 8)[0][0])  # the methanal oxygen
+    m = int(eng.b.atom_mol[a])
+    base = eng.atom_par[:, col].clone()
+    h = 1e-3 * max(abs(float(base[a])), 0.1)
+    et = []
+    for s in (-1, 1):
+        v = base.clone()
+        v[a] += s * h
+        eng.set_parameters({name: v})
+        rs = eng.scf(eps=1e-10, converger=(2,))
+        assert not rs.notconverged.any()
+        et.append(float(eng.energies(rs.P)["Etot"][m]))
+    eng.set_parameters({name: base})
+    slope = (et[1] - et[0]) / (2 * h)
+    d = float(G[a, 0, col] + G[a, 1, col])
+    assert abs(slope - d) <= 1e-5 * max(abs(d), 1.0), (name, slope, d)
+
+
+def test_energy_autograd_with_learned_parameters():
+    files = ["methanal.1.xyz", "benzene.xyz"]
+    sp, c = read_xyz(files)
+    p = {"method": "AM1", "scf_eps": 1.0e-9, "scf_converger": [2]}
+    const = pyseqm_amd.Constants().to(DEV)
+    mol = pyseqm_amd.Molecule(const, p, torch.as_tensor(c, device=DEV), torch.as_tensor(sp, device=DEV))
+    eng = mol.engine()
+    lp = {n: eng.atom_par[:, COLUMNS.index(n)].clone().requires_grad_(True) for n in ("g_ss", "beta_p", "U_ss")}
+    Hf, Etot, Eelec, Enuc, Eiso = Energy(p)(mol, learned_parameters=lp, all_terms=True)[:5]
+    assert all(t.requires_grad for t in (Hf, Etot, Eelec, Enuc, Eiso))
+    w = torch.tensor([1.0, -2.0], dtype=torch.float64, device=DEV)
+    ((Etot * w).sum() + 0.5 * Hf.sum() + 3.0 * Enuc.sum() - Eiso.sum()).backward()
+    G = eng.param_grad(eng._last_P)
+    dI = eng.eiso_grad()
+    wa = w[torch.as_tensor(eng.b.atom_mol, device=DEV).long()]
+    for n, t in lp.items():
+        k = COLUMNS.index(n)
+        dT = G[:, 0, k] + G[:, 1, k]
+        want = wa * dT + 0.5 * (dT - dI[:, k]) + 3.0 * G[:, 1, k] - dI[:, k]
+        torch.testing.assert_close(t.grad, want, rtol=1e-12, atol=1e-12)
+    # the drop-in Force detaches its energies when it computes forces (basics.py:1296-1365)
+    out = Force(p)(mol, learned_parameters=lp)
+    assert not any(x.requires_grad for x in out[2:7])
+    with torch.no_grad():
+        Hf2 = Energy(p)(mol, learned_parameters=lp, all_terms=True)[0]
+    assert not Hf2.requires_grad
