@@ -133,6 +133,14 @@ int pyseqm_pair_forces(const pyseqm_layout_t* L, int method, const double* coord
 long long pyseqm_param_grad_workspace(const pyseqm_layout_t* L);
 int pyseqm_param_grad(const pyseqm_layout_t* L, int method, const double* coords, const double* atom_par,
                       const double* P, double hrel, double* rec, double* part, double* out, void* stream);
+/* Vector-Jacobian product of the Fock matrix with the atom parameters: out[a][0][k] =
+ * d(sum Y F(theta; P))/d(slot k of atom a) at fixed P and symmetric Y (out[a][1][.] = 0), the
+ * contraction (dP_out/dtheta)^T u of the SCF implicit backward after the density response
+ * (scf_loop.py:1779-1993, agrad(Pout, gv, grad_outputs=u) over M, w, gss .. hsp and on to the
+ * parameters).  Same kernels, steps and workspace as pyseqm_param_grad. */
+int pyseqm_fock_param_vjp(const pyseqm_layout_t* L, int method, const double* coords, const double* atom_par,
+                          const double* P, const double* Y, double hrel, double* rec, double* part, double* out,
+                          void* stream);
 
 /* Core-core repulsion E_nuc(i, j) of an explicit pair list, pair_nuclear_energy
  * (seqm/seqm_functions/energy.py:91-134): idxi/idxj [npairs] atom rows of atom_par (PAR_Z, PAR_TORE,
@@ -330,6 +338,11 @@ int pyseqm_fermi_density(const pyseqm_layout_t* L, const int* idx, int nact, int
  * eigenbasis transforms Q^T X Q and Q X Q^T of Canon_DM_PRT (canon_dm_prt.py:17, 37). */
 int pyseqm_block_gemm(const pyseqm_layout_t* L, const int* idx, int nact, int nmax, int ta, int tb,
                       const double* A, const double* B, double* C, void* stream);
+/* First-order closed-shell density response in the eigenbasis, in place on Y = Q^T X Q:
+ * Y_ia = Y_ai <- 2 Y_ia / (e_i - e_a) for i < nocc <= a, zero elsewhere (the eigh backward inside
+ * the SCF implicit backward, scf_loop.py:1779-1993).  Self-adjoint. */
+int pyseqm_density_response_elem(const pyseqm_layout_t* L, const int* idx, int nact, int nmax,
+                                 const long long* orb_off, const double* evals, double* Y, void* stream);
 /* The canonical density-matrix perturbation recursion of Canon_DM_PRT (canon_dm_prt.py:18-35) in
  * place on Y = Q^T FO1 Q (eigenbasis; eigenvalues evals, chemical potential mu[m], `steps`
  * recursion steps), then the trace-keeping chemical-potential shift. */

@@ -36,6 +36,9 @@ _SIGS = {
                               c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
     "pyseqm_pair_forces": [c_int, c_void_p, c_void_p, c_void_p, c_double, c_void_p, c_void_p, c_void_p],
     "pyseqm_param_grad": [c_int, c_void_p, c_void_p, c_void_p, c_double, c_void_p, c_void_p, c_void_p, c_void_p],
+    "pyseqm_fock_param_vjp": [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_void_p, c_void_p, c_void_p,
+                              c_void_p],
+    "pyseqm_density_response_elem": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p],
     "pyseqm_fock": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                     c_void_p, c_void_p, c_void_p, c_void_p],
     "pyseqm_fock_noalloc": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,

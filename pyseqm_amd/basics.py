@@ -123,11 +123,13 @@ class Hamiltonian(torch.nn.Module):
         self.sp2 = seqm_parameters.get("sp2", [False])
         self.converger = _converger(seqm_parameters)
         self.eig = seqm_parameters.get("eig", True)
+        self.scf_backward = seqm_parameters.get("scf_backward", 0)  # basics.py:489-491
+        self.scf_backward_eps = float(seqm_parameters.get("scf_backward_eps", seqm_parameters["scf_eps"]))
 
     def forward(self, molecule, themethod, P0=None):
         from .seqm_functions.scf_loop import scf_loop
         return scf_loop(molecule, eps=self.eps, P=P0, sp2=self.sp2, scf_converger=list(self.converger),
-                        eig=self.eig)
+                        eig=self.eig, scf_backward=self.scf_backward, scf_backward_eps=self.scf_backward_eps)
 
 
 class Energy(torch.nn.Module):
@@ -217,7 +219,8 @@ def eng_pair_enuc(eng):
 def _run_scf(molecule, ham, P0):
     from .seqm_functions.scf_loop import scf_loop
     F, e, P, Hcore, w, charge, rho0xi, rho0xj, riXH, ri, notconverged, v = scf_loop(
-        molecule, eps=ham.eps, P=P0, sp2=ham.sp2, scf_converger=list(ham.converger), eig=ham.eig)
+        molecule, eps=ham.eps, P=P0, sp2=ham.sp2, scf_converger=list(ham.converger), eig=ham.eig,
+        scf_backward=ham.scf_backward, scf_backward_eps=ham.scf_backward_eps)
     eng = molecule.engine()
     en = eng.energies(eng._last_P)
     e_gap = None

@@ -174,6 +174,26 @@ __global__ __launch_bounds__(256) void k_canon_elem(pyseqm_layout_t L, const int
   *y = P1;
 }
 
+// First-order response of the closed-shell density D = 2 sum_occ c c^T to a symmetric Fock
+// perturbation, in the eigenbasis: dD'_ia = dD'_ai = 2 F'_ia / (e_i - e_a) for i occupied and a
+// virtual (the nocc lowest eigenpairs), zero in the occupied-occupied and virtual-virtual blocks.
+// The operator is self-adjoint, so it is also the transpose the SCF implicit backward applies
+// (the eigh backward of sym_eig_trunc1 inside scf_loop.py:1779-1993).
+__global__ __launch_bounds__(256) void k_response_elem(pyseqm_layout_t L, const int* __restrict__ idx,
+                                                       const long long* __restrict__ orb_off,
+                                                       const double* __restrict__ evals, double* __restrict__ Y) {
+  const int m = idx[blockIdx.y];
+  const int n = L.mol_norb[m];
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= n * n) return;
+  const int i = t / n, j = t - (t / n) * n;
+  const int nocc = L.mol_nocc[m];
+  const double* e = evals + orb_off[m];
+  double* y = Y + L.mol_mat_off[m] + t;
+  const bool oi = i < nocc, oj = j < nocc;
+  *y = oi == oj ? 0.0 : *y * 2.0 / (oi ? e[i] - e[j] : e[j] - e[i]);
+}
+
 // The chemical-potential shift that keeps the trace: P1 += diag(dp/dmu) dmu1,
 // dmu1 = -tr P1 / sum dp/dmu, dp/dmu = beta p0 (1 - p0) of the final p0; one block per molecule.
 __global__ __launch_bounds__(256) void k_canon_shift(pyseqm_layout_t L, const int* __restrict__ idx,
@@ -327,6 +347,14 @@ extern "C" int pyseqm_canon_prt(const pyseqm_layout_t* L, const int* idx, int na
   const double beta = 1.0 / (kB * T);
   k_canon_elem<<<dim3(elem_blocks(nmax), nact), 256, 0, s>>>(*L, idx, orb_off, evals, mu, beta, steps, Y);
   k_canon_shift<<<nact, 256, 0, s>>>(*L, idx, orb_off, evals, mu, beta, steps, Y);
+  return hipGetLastError() == hipSuccess ? PYSEQM_OK : PYSEQM_ELAUNCH;
+}
+
+extern "C" int pyseqm_density_response_elem(const pyseqm_layout_t* L, const int* idx, int nact, int nmax,
+                                            const long long* orb_off, const double* evals, double* Y, void* stream) {
+  if (!L || nact < 0 || nmax < 1) return PYSEQM_EINVAL;
+  if (nact == 0) return PYSEQM_OK;
+  k_response_elem<<<dim3(elem_blocks(nmax), nact), 256, 0, (hipStream_t)stream>>>(*L, idx, orb_off, evals, Y);
   return hipGetLastError() == hipSuccess ? PYSEQM_OK : PYSEQM_ELAUNCH;
 }
 

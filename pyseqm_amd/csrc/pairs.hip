@@ -606,7 +606,9 @@ struct PairDensity {
   double pab[4][4];       // block (i, j)
 };
 
-template <int KIND, bool XL>
+// MODE 0: the pair's share of Eelec(P) + E_nuc; 1 (XL-BOMD): of sum D F(P) - 1/2 sum (F(P) - H) P +
+// E_nuc; 2 (linear): of sum D F(P), no core repulsion (rep_out = 0).
+template <int KIND, int MODE>
 __device__ double pair_energy(int method, const PairGeom& g, const double* pi, const double* pj,
                               const PairDensity& P, const PairDensity& D, int wmode, int smode, int bsel,
                               double* rep_out = nullptr) {
@@ -614,12 +616,16 @@ __device__ double pair_energy(int method, const PairGeom& g, const double* pi, c
   constexpr int NJ = KIND == XX ? 4 : 1;
   const double ti = pi[PAR_TORE], tj = pj[PAR_TORE];
   double e = 0.0, gam = 0.0;
-  // Coulomb weight of w_kl: P_a P_b (plain) or D_a P_b + P_a D_b - P_a P_b (XL)
+  // Coulomb weight of w_kl: P_a P_b (plain), D_a P_b + P_a D_b - P_a P_b (XL), D_a P_b + P_a D_b (linear)
   auto coul = [&](int k, int l) {
-    return XL ? D.pa[k] * P.pb[l] + P.pa[k] * D.pb[l] - P.pa[k] * P.pb[l] : P.pa[k] * P.pb[l];
+    return MODE == 1   ? D.pa[k] * P.pb[l] + P.pa[k] * D.pb[l] - P.pa[k] * P.pb[l]
+           : MODE == 2 ? D.pa[k] * P.pb[l] + P.pa[k] * D.pb[l]
+                       : P.pa[k] * P.pb[l];
   };
-  // exchange weight: 1/2 P_ab (plain) or D_ab - 1/2 P_ab (XL), times K(P)_ab
-  auto xw = [&](int a, int b) { return XL ? D.pab[a][b] - 0.5 * P.pab[a][b] : 0.5 * P.pab[a][b]; };
+  // exchange weight: 1/2 P_ab (plain), D_ab - 1/2 P_ab (XL), D_ab (linear), times K(P)_ab
+  auto xw = [&](int a, int b) {
+    return MODE == 1 ? D.pab[a][b] - 0.5 * P.pab[a][b] : MODE == 2 ? D.pab[a][b] : 0.5 * P.pab[a][b];
+  };
   if constexpr (KIND == XX) {
     double K[4][4];
 #pragma unroll
@@ -665,7 +671,7 @@ __device__ double pair_energy(int method, const PairGeom& g, const double* pi, c
   for (int a = 0; a < NI; ++a)
 #pragma unroll
     for (int b = 0; b < NJ; ++b) e += 2.0 * D.pab[a][b] * Hb[a][b];
-  const double rep = core_repulsion(method, (int)pi[PAR_Z], (int)pj[PAR_Z], g.r, gam, pi, pj);
+  const double rep = MODE == 2 ? 0.0 : core_repulsion(method, (int)pi[PAR_Z], (int)pj[PAR_Z], g.r, gam, pi, pj);
   if (rep_out) *rep_out = rep;
   return e + rep;
 }
@@ -790,11 +796,13 @@ __global__ void k_param_records(pyseqm_layout_t L, const double* __restrict__ pa
 }
 
 // Per pair and slot of the pass: d(pair Eelec share)/d(slot) and d(core repulsion)/d(slot) for
-// atom i (row partial values 2g, 2g + 1) and for atom j (column partial).
-template <int KIND>
+// atom i (row partial values 2g, 2g + 1) and for atom j (column partial); MODE 2: the pair's share
+// of d(sum Y F(P))/d(slot) (no core repulsion) for a symmetric Y.
+template <int KIND, int MODE>
 __global__ __launch_bounds__(64) void k_pair_param_grad(pyseqm_layout_t L, int method, int tile_base,
                                                         const double* __restrict__ coords,
                                                         const double* __restrict__ par, const double* __restrict__ P,
+                                                        const double* __restrict__ Y,
                                                         const double* __restrict__ rec, int p0, int np, double hrel,
                                                         double* __restrict__ part) {
   const int tile = tile_base + xcd_run(blockIdx.x, gridDim.x);
@@ -812,8 +820,10 @@ __global__ __launch_bounds__(64) void k_pair_param_grad(pyseqm_layout_t L, int m
     const int n = L.mol_norb[t.mol];
     constexpr int NI = KIND == HH ? 1 : 4;
     constexpr int NJ = KIND == XX ? 4 : 1;
-    PairDensity Pd;
+    PairDensity Pd, Yd;
     load_pair_density<NI, NJ>(P + mo, n, L.atom_ao[i], L.atom_ao[j], Pd);
+    if constexpr (MODE == 2) load_pair_density<NI, NJ>(Y + mo, n, L.atom_ao[i], L.atom_ao[j], Yd);
+    const PairDensity& Dr = MODE == 2 ? Yd : Pd;
     const PairGeom g0 = pair_geom(coords, i, j);
     const int wmode = quat_flips(-g0.x[0]) ? 2 : 1;
     const int smode = quat_flips(g0.x[0]) ? 2 : 1;
@@ -830,8 +840,8 @@ __global__ __launch_bounds__(64) void k_pair_param_grad(pyseqm_layout_t L, int m
         for (int s = 0; s < 4; ++s) {
           const double* q = rec + ((size_t)(a * kParamGroup + g) * 4 + s) * PYSEQM_NPAR;
           double rep = 0.0;
-          const double e = pair_energy<KIND, false>(method, g0, side == 0 ? q : pi, side == 0 ? pj : q, Pd, Pd,
-                                                    wmode, smode, bsel, &rep);
+          const double e = pair_energy<KIND, MODE>(method, g0, side == 0 ? q : pi, side == 0 ? pj : q, Pd, Dr,
+                                                   wmode, smode, bsel, &rep);
           de += kStencilW[s] * (e - rep);
           dn += kStencilW[s] * rep;
         }
@@ -874,8 +884,10 @@ __global__ __launch_bounds__(256) void k_param_gather(pyseqm_layout_t L, const d
 
 // The closed-form one-center part: E_A = sum_mu P_mumu U_mu + 1/2 sum P G1(P) over the atom's
 // block (the one-center terms of the Fock build, fock.py:196-224), differentiated in U_ss, U_pp,
-// g_ss, g_sp, g_pp, g_p2 and h_sp.  Writes every slot of out[a] (the pair passes add to it).
-__global__ void k_param_onecenter(pyseqm_layout_t L, const double* __restrict__ P, double* __restrict__ out) {
+// g_ss, g_sp, g_pp, g_p2 and h_sp; with Y given, the same for sum Y (U + G1(P)) over the block.
+// Writes every slot of out[a] (the pair passes add to it).
+__global__ void k_param_onecenter(pyseqm_layout_t L, const double* __restrict__ P, const double* __restrict__ Y,
+                                  double* __restrict__ out) {
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
   if (a >= L.natoms) return;
   const int m = L.atom_mol[a];
@@ -885,21 +897,46 @@ __global__ void k_param_onecenter(pyseqm_layout_t L, const double* __restrict__ 
   double* o = out + (size_t)a * 48;
   for (int k = 0; k < 48; ++k) o[k] = 0.0;
   const double pss = Pm[(size_t)ao * n + ao];
-  o[PAR_USS] = pss;
-  o[PAR_GSS] = 0.25 * pss * pss;
-  if (L.atom_Z[a] > 1) {
-    double pd[10];
+  if (!Y) {
+    o[PAR_USS] = pss;
+    o[PAR_GSS] = 0.25 * pss * pss;
+    if (L.atom_Z[a] > 1) {
+      double pd[10];
 #pragma unroll
-    for (int k = 0; k < 10; ++k) pd[k] = Pm[(size_t)(ao + kTriI[k]) * n + ao + kTriJ[k]];
-    const double ppt = pd[2] + pd[5] + pd[9];
-    const double psp2 = pd[1] * pd[1] + pd[3] * pd[3] + pd[6] * pd[6];
-    const double ppd2 = pd[2] * pd[2] + pd[5] * pd[5] + pd[9] * pd[9];
-    const double ppo2 = pd[4] * pd[4] + pd[7] * pd[7] + pd[8] * pd[8];
-    o[PAR_UPP] = ppt;
-    o[PAR_GSP] = pss * ppt - 0.5 * psp2;
-    o[PAR_HSP] = -0.5 * pss * ppt + 1.5 * psp2;
-    o[PAR_GPP] = 0.375 * ppd2 - 0.125 * ppt * ppt + 0.75 * ppo2;
-    o[PAR_GP2] = 0.625 * (ppt * ppt - ppd2) - 1.25 * ppo2;
+      for (int k = 0; k < 10; ++k) pd[k] = Pm[(size_t)(ao + kTriI[k]) * n + ao + kTriJ[k]];
+      const double ppt = pd[2] + pd[5] + pd[9];
+      const double psp2 = pd[1] * pd[1] + pd[3] * pd[3] + pd[6] * pd[6];
+      const double ppd2 = pd[2] * pd[2] + pd[5] * pd[5] + pd[9] * pd[9];
+      const double ppo2 = pd[4] * pd[4] + pd[7] * pd[7] + pd[8] * pd[8];
+      o[PAR_UPP] = ppt;
+      o[PAR_GSP] = pss * ppt - 0.5 * psp2;
+      o[PAR_HSP] = -0.5 * pss * ppt + 1.5 * psp2;
+      o[PAR_GPP] = 0.375 * ppd2 - 0.125 * ppt * ppt + 0.75 * ppo2;
+      o[PAR_GP2] = 0.625 * (ppt * ppt - ppd2) - 1.25 * ppo2;
+    }
+    return;
+  }
+  // sum Y (U + G1(P)): diagonal elements once, off-diagonal pairs twice (both triangles)
+  const double* Ym = Y + L.mol_mat_off[m];
+  const double yss = Ym[(size_t)ao * n + ao];
+  o[PAR_USS] = yss;
+  o[PAR_GSS] = 0.5 * yss * pss;
+  if (L.atom_Z[a] > 1) {
+    double pd[10], yd[10];
+#pragma unroll
+    for (int k = 0; k < 10; ++k) {
+      pd[k] = Pm[(size_t)(ao + kTriI[k]) * n + ao + kTriJ[k]];
+      yd[k] = Ym[(size_t)(ao + kTriI[k]) * n + ao + kTriJ[k]];
+    }
+    const double ppt = pd[2] + pd[5] + pd[9], ypt = yd[2] + yd[5] + yd[9];
+    const double ysp = yd[1] * pd[1] + yd[3] * pd[3] + yd[6] * pd[6];
+    const double ypd = yd[2] * pd[2] + yd[5] * pd[5] + yd[9] * pd[9];
+    const double ypo = yd[4] * pd[4] + yd[7] * pd[7] + yd[8] * pd[8];
+    o[PAR_UPP] = ypt;
+    o[PAR_GSP] = yss * ppt + ypt * pss - ysp;
+    o[PAR_HSP] = -0.5 * (yss * ppt + ypt * pss) + 3.0 * ysp;
+    o[PAR_GPP] = 0.75 * ypd - 0.25 * ypt * ppt + 1.5 * ypo;
+    o[PAR_GP2] = 1.25 * (ypt * ppt - ypd) - 2.5 * ypo;
   }
 }
 
@@ -1049,13 +1086,15 @@ extern "C" long long pyseqm_param_grad_workspace(const pyseqm_layout_t* L) {
   return (long long)L->natoms * kParamGroup * 4 * PYSEQM_NPAR;
 }
 
-extern "C" int pyseqm_param_grad(const pyseqm_layout_t* L, int method, const double* coords, const double* atom_par,
-                                 const double* P, double hrel, double* rec, double* part, double* out,
-                                 void* stream) {
+namespace {
+template <int MODE>
+int launch_param_grad(const pyseqm_layout_t* L, int method, const double* coords, const double* atom_par,
+                      const double* P, const double* Y, double hrel, double* rec, double* part, double* out,
+                      void* stream) {
   if (!L || method < PYSEQM_MNDO || method > PYSEQM_PM3 || !(hrel > 0.0)) return PYSEQM_EINVAL;
   if (L->natoms == 0) return PYSEQM_OK;
   hipStream_t s = (hipStream_t)stream;
-  k_param_onecenter<<<(L->natoms + 127) / 128, 128, 0, s>>>(*L, P, out);
+  k_param_onecenter<<<(L->natoms + 127) / 128, 128, 0, s>>>(*L, P, Y, out);
   const int npp = pair_param_count(method);
   for (int p0 = 0; p0 < npp; p0 += kParamGroup) {
     const int np = npp - p0 < kParamGroup ? npp - p0 : kParamGroup;
@@ -1063,22 +1102,36 @@ extern "C" int pyseqm_param_grad(const pyseqm_layout_t* L, int method, const dou
     k_param_records<<<(unsigned)((nthr + 255) / 256), 256, 0, s>>>(*L, atom_par, p0, np, hrel, rec);
     int base = 0;
     if (L->ntile[0])
-      k_pair_param_grad<XX><<<L->ntile[0], 64, 0, s>>>(*L, method, base, coords, atom_par, P, rec, p0, np, hrel,
-                                                       part);
+      k_pair_param_grad<XX, MODE><<<L->ntile[0], 64, 0, s>>>(*L, method, base, coords, atom_par, P, Y, rec, p0, np,
+                                                             hrel, part);
     base += L->ntile[0];
     if (L->ntile[1])
-      k_pair_param_grad<XH><<<L->ntile[1], 64, 0, s>>>(*L, method, base, coords, atom_par, P, rec, p0, np, hrel,
-                                                       part);
+      k_pair_param_grad<XH, MODE><<<L->ntile[1], 64, 0, s>>>(*L, method, base, coords, atom_par, P, Y, rec, p0, np,
+                                                             hrel, part);
     base += L->ntile[1];
     if (L->ntile[2])
-      k_pair_param_grad<HH><<<L->ntile[2], 64, 0, s>>>(*L, method, base, coords, atom_par, P, rec, p0, np, hrel,
-                                                       part);
+      k_pair_param_grad<HH, MODE><<<L->ntile[2], 64, 0, s>>>(*L, method, base, coords, atom_par, P, Y, rec, p0, np,
+                                                             hrel, part);
     if (lanes_per_atom(L) == 64)
       k_param_gather<64><<<(L->natoms + 3) / 4, 256, 0, s>>>(*L, part, p0, np, out);
     else
       k_param_gather<16><<<(L->natoms + 15) / 16, 256, 0, s>>>(*L, part, p0, np, out);
   }
   return hipGetLastError() == hipSuccess ? PYSEQM_OK : PYSEQM_ELAUNCH;
+}
+}  // namespace
+
+extern "C" int pyseqm_param_grad(const pyseqm_layout_t* L, int method, const double* coords, const double* atom_par,
+                                 const double* P, double hrel, double* rec, double* part, double* out,
+                                 void* stream) {
+  return launch_param_grad<0>(L, method, coords, atom_par, P, nullptr, hrel, rec, part, out, stream);
+}
+
+extern "C" int pyseqm_fock_param_vjp(const pyseqm_layout_t* L, int method, const double* coords,
+                                     const double* atom_par, const double* P, const double* Y, double hrel,
+                                     double* rec, double* part, double* out, void* stream) {
+  if (!Y) return PYSEQM_EINVAL;
+  return launch_param_grad<2>(L, method, coords, atom_par, P, Y, hrel, rec, part, out, stream);
 }
 
 extern "C" int pyseqm_atom_setup(const pyseqm_layout_t* L, double* atom_par, void* stream) {

@@ -49,8 +49,8 @@ def _check_pair_cutoff(molecule):
 
 def scf_loop(molecule, eps=1.0e-4, P=None, sp2=[False], scf_converger=[1], eig=False, scf_backward=0,
              scf_backward_eps=1.0e-2):
-    if scf_backward not in (0,):
-        raise NotImplementedError("only scf_backward = 0 (Hellmann-Feynman forces) is on this path")
+    if scf_backward not in (0, 1, 2):
+        raise ValueError(f"scf_backward must be 0, 1 or 2 (got {scf_backward!r})")
     _check_pair_cutoff(molecule)
     eng = molecule.engine()
     eng.set_coordinates(molecule.coordinates.detach())
@@ -71,6 +71,16 @@ def scf_loop(molecule, eps=1.0e-4, P=None, sp2=[False], scf_converger=[1], eig=F
     en = eng.energies(r.P)
     F = b.to_padded(en["F"])
     Ppad = b.to_padded(r.P)
+    if scf_backward >= 1:
+        # the density carries the gradient of the SCF fixed point with respect to the learned
+        # parameters (scf_loop.py:1779-1993); 2 (the reference's unrolled iterations) gives the same
+        # gradient at convergence and takes the same implicit route here
+        from ..basics import _trained_parameters
+        trained = _trained_parameters(molecule.learned_parameters)
+        if trained:
+            from ..implicit import SCFImplicit
+            Ppad = SCFImplicit.apply(eng, r.P, en["F"], float(scf_backward_eps), r.notconverged,
+                                     [n for n, _ in trained], Ppad, *[t for _, t in trained])
     Hcore = torch.triu(b.to_padded(eng.H))  # the reference fills only the upper triangle
     w = eng.w_reference()
     rho0 = eng.atom_par[:, 26]
