@@ -987,13 +987,23 @@ __global__ __launch_bounds__(NT) void k_mol_sum(pyseqm_layout_t L, const double*
   const int m = blockIdx.x;
   if (active && !active[m]) return;
   const int tid = threadIdx.x;
-  double s = 0.0;
+  // eight independent partial sums per thread keep eight loads in flight (a single running sum
+  // waits out one memory round trip per element); combined in a fixed order
+  double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  auto add_range = [&](const double* v, int b, int e) {
+    int t = b + tid;
+    for (; t + 7 * NT < e; t += 8 * NT) {
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc[u] += v[t + u * NT];
+    }
+    for (; t < e; t += NT) acc[0] += v[t];
+  };
   for (int kind = 0; kind < 3; ++kind) {
     const int* off = L.mol_tile_off + kind * (L.nmol + 1);
-    for (int t = off[m] + tid; t < off[m + 1]; t += NT) s += tile_v[t];
+    add_range(tile_v, off[m], off[m + 1]);
   }
-  if (atom_v)
-    for (int a = L.mol_atom_off[m] + tid; a < L.mol_atom_off[m + 1]; a += NT) s += atom_v[a];
+  if (atom_v) add_range(atom_v, L.mol_atom_off[m], L.mol_atom_off[m + 1]);
+  double s = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
   s = wave_sum(s);
   if ((tid & 63) == 0) wsum[tid >> 6] = s;
   __syncthreads();
