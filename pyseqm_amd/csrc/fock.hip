@@ -258,6 +258,31 @@ __global__ __launch_bounds__(128) void k_fock_xx2(pyseqm_layout_t L, const doubl
   if (lane == 0) etile[tile] = es;
 }
 
+// One-center G(P_ii) of a heavy atom (fock.py:196-224) added to acc, which holds H_ii plus the
+// atom's Coulomb sums (packed lower triangle, kTriI/kTriJ order); returns the atom's energy share
+// 1/2 sum P_ii (H_ii + F_ii) over both triangles of the block.
+__device__ inline double onecenter_heavy(const double* p, const double (&pd)[10], const double (&hd)[10],
+                                         double (&acc)[10]) {
+  const double gss = p[PAR_GSS], gsp = p[PAR_GSP], gpp = p[PAR_GPP], gp2 = p[PAR_GP2], hsp = p[PAR_HSP];
+  const double pss = pd[0], ppt = pd[2] + pd[5] + pd[9];
+  const double f1 = gsp - 0.5 * hsp, f2 = 1.5 * hsp - 0.5 * gsp;
+  const double fd = 1.25 * gp2 - 0.25 * gpp, fo = 0.75 * gpp - 1.25 * gp2;
+  acc[0] += 0.5 * pss * gss + ppt * f1;
+  acc[2] += pss * f1 + 0.5 * pd[2] * gpp + (ppt - pd[2]) * fd;
+  acc[5] += pss * f1 + 0.5 * pd[5] * gpp + (ppt - pd[5]) * fd;
+  acc[9] += pss * f1 + 0.5 * pd[9] * gpp + (ppt - pd[9]) * fd;
+  acc[1] += pd[1] * f2;
+  acc[3] += pd[3] * f2;
+  acc[6] += pd[6] * f2;
+  acc[4] += pd[4] * fo;
+  acc[7] += pd[7] * fo;
+  acc[8] += pd[8] * fo;
+  double e = 0.0;
+#pragma unroll
+  for (int k = 0; k < 10; ++k) e += (kTriI[k] == kTriJ[k] ? 0.5 : 1.0) * pd[k] * (hd[k] + acc[k]);
+  return e;
+}
+
 // Diagonal blocks: H_ii + one-center G (fock.py:196-224) + Coulomb partials; energy share.
 // One wave per atom: the lanes split the atom's partial slots, then lanes 0..15 write the block.
 template <int G>
@@ -288,26 +313,8 @@ __global__ __launch_bounds__(256) void k_fock_diag(pyseqm_layout_t L, const doub
       hd[k] = Hm[(size_t)(ao + kTriI[k]) * n + ao + kTriJ[k]];
       acc[k] += hd[k];
     }
-    const double gss = p[PAR_GSS], gsp = p[PAR_GSP], gpp = p[PAR_GPP], gp2 = p[PAR_GP2], hsp = p[PAR_HSP];
-    const double pss = pd[0], ppt = pd[2] + pd[5] + pd[9];
-    const double f1 = gsp - 0.5 * hsp, f2 = 1.5 * hsp - 0.5 * gsp;
-    const double fd = 1.25 * gp2 - 0.25 * gpp, fo = 0.75 * gpp - 1.25 * gp2;
-    acc[0] += 0.5 * pss * gss + ppt * f1;
-    acc[2] += pss * f1 + 0.5 * pd[2] * gpp + (ppt - pd[2]) * fd;
-    acc[5] += pss * f1 + 0.5 * pd[5] * gpp + (ppt - pd[5]) * fd;
-    acc[9] += pss * f1 + 0.5 * pd[9] * gpp + (ppt - pd[9]) * fd;
-    acc[1] += pd[1] * f2;
-    acc[3] += pd[3] * f2;
-    acc[6] += pd[6] * f2;
-    acc[4] += pd[4] * fo;
-    acc[7] += pd[7] * fo;
-    acc[8] += pd[8] * fo;
-    if (lane == 0) {
-      double e = 0.0;
-#pragma unroll
-      for (int k = 0; k < 10; ++k) e += (kTriI[k] == kTriJ[k] ? 0.5 : 1.0) * pd[k] * (hd[k] + acc[k]);
-      eatom[a] = e;
-    }
+    const double e = onecenter_heavy(p, pd, hd, acc);
+    if (lane == 0) eatom[a] = e;
     if (lane < 16) {
       const int mu = lane >> 2, nu = lane & 3;
       double v = acc[0];
@@ -789,22 +796,10 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(PYSEQM_
         for (int w2 = 1; w2 < NW; ++w2) sum += lds[w2 * stride + 10 * tid + k];
         acc10[k] = sum + hd[k];
       }
-      const double gss = p[PAR_GSS], gsp = p[PAR_GSP], gpp = p[PAR_GPP], gp2 = p[PAR_GP2], hsp = p[PAR_HSP];
-      const double pss = pd[0], ppt = pd[2] + pd[5] + pd[9];
-      const double f1 = gsp - 0.5 * hsp, f2 = 1.5 * hsp - 0.5 * gsp;
-      const double fd = 1.25 * gp2 - 0.25 * gpp, fo = 0.75 * gpp - 1.25 * gp2;
-      acc10[0] += 0.5 * pss * gss + ppt * f1;
-      acc10[2] += pss * f1 + 0.5 * pd[2] * gpp + (ppt - pd[2]) * fd;
-      acc10[5] += pss * f1 + 0.5 * pd[5] * gpp + (ppt - pd[5]) * fd;
-      acc10[9] += pss * f1 + 0.5 * pd[9] * gpp + (ppt - pd[9]) * fd;
-      acc10[1] += pd[1] * f2;
-      acc10[3] += pd[3] * f2;
-      acc10[6] += pd[6] * f2;
-      acc10[4] += pd[4] * fo;
-      acc10[7] += pd[7] * fo;
-      acc10[8] += pd[8] * fo;
+      double pdr[10];
 #pragma unroll
-      for (int k = 0; k < 10; ++k) e += (kTriI[k] == kTriJ[k] ? 0.5 : 1.0) * pd[k] * (hd[k] + acc10[k]);
+      for (int k = 0; k < 10; ++k) pdr[k] = pd[k];
+      e += onecenter_heavy(p, pdr, hd, acc10);
 #pragma unroll
       for (int mu = 0; mu < 4; ++mu) {
         st2(Fm + (size_t)(ao + mu) * n + ao, acc10[tri_index(mu, 0)], acc10[tri_index(mu, 1)]);
