@@ -50,6 +50,14 @@ class ResponseWorkspace:
         self.eelec = torch.zeros(b.nmol, **f64)
         n2 = norb.astype(np.int64) ** 2
         self.elem_mol = torch.as_tensor(np.repeat(np.arange(b.nmol), n2), device=dev)
+        # per packed element (i, j) of molecule m: orb_off[m] + j (its column's eigenvalue slot); per
+        # eigenvalue slot: (molecule, index) in the reference's padded orbital-energy layout
+        off = np.concatenate([[0], np.cumsum(norb)[:-1]]).astype(np.int64)
+        self.elem_col = torch.as_tensor(np.concatenate([np.tile(np.arange(n), n) + o for n, o in zip(norb, off)])
+                                        if b.nmol else np.zeros(0, np.int64), device=dev)
+        self.orb_mol = torch.as_tensor(np.repeat(np.arange(b.nmol), norb), device=dev)
+        self.orb_k = torch.as_tensor(np.concatenate([np.arange(n) for n in norb]) if b.nmol
+                                     else np.zeros(0, np.int64), device=dev)
 
 
 def eigenpairs(eng, ws, F):
@@ -124,13 +132,32 @@ def adjoint(eng, ws, gbar, atom_par, w, eps, max_iter=300, depth=6):
     return u, err > eps
 
 
-def implicit_param_grad(eng, P, F, gbar, atom_par, coords, w, eps, hrel=1.0e-4):
-    """dL/dtheta (natoms, 24) for the upstream gradient gbar = dL/dP* (packed, symmetrised), and
-    the per-molecule flag of a backward that did not reach eps."""
+def eigenvalue_source(eng, ws, ebar):
+    """Y_e = sum_k ebar_k c_k c_k^T (packed) for an upstream gradient ebar on the orbital energies
+    (one value per eigenvalue slot orb_off[m] + k): sum_k ebar_k de_k = sum Y_e dF."""
+    T = ws.Q * ebar[ws.elem_col]
+    Ye = torch.empty_like(T)
+    _gemm(eng, ws, 0, 1, T, ws.Q, Ye)
+    return Ye
+
+
+def implicit_param_grad(eng, P, F, gbar, atom_par, coords, w, eps, hrel=1.0e-4, epad_grad=None):
+    """dL/dtheta (natoms, 24) for the upstream gradient gbar = dL/dP* (packed, symmetrised) and,
+    optionally, epad_grad = dL/de on the orbital energies in the reference's padded layout
+    (nmol, 4 molsize); returns also the per-molecule flag of a backward that did not reach eps.
+    The orbital energies e_k = c_k^T F(theta, P*) c_k add Y_e = sum_k ebar_k c_k c_k^T to the
+    parameter contraction and G(Y_e) to the adjoint's source (F depends on P* through G)."""
     ws = ResponseWorkspace(eng)
     eigenpairs(eng, ws, F)
+    Ye = None
+    if epad_grad is not None:
+        ebar = epad_grad.to(torch.float64)[ws.orb_mol, ws.orb_k].contiguous()
+        Ye = eigenvalue_source(eng, ws, ebar)
+        gbar = gbar + two_electron(eng, ws, Ye, torch.empty_like(Ye), atom_par, w)
     u, bad = adjoint(eng, ws, gbar, atom_par, w, eps)
     Y = response(eng, ws, u, torch.empty_like(u))
+    if Ye is not None:
+        Y = Y + Ye
     b = eng.b
     out = torch.zeros(max(b.natoms, 1), 2, 24, dtype=torch.float64, device=eng.device)
     rec = torch.empty(max(int(nat.lib().pyseqm_param_grad_workspace(nat.ctypes.byref(b.L))), 1),
@@ -141,30 +168,33 @@ def implicit_param_grad(eng, P, F, gbar, atom_par, coords, w, eps, hrel=1.0e-4):
 
 
 class SCFImplicit(torch.autograd.Function):
-    """The converged density (padded, as scf_loop returns it) as a function of the learned per-atom
-    parameters, with the implicit backward above.  The engine's parameters, integrals and
+    """The converged density (padded, as scf_loop returns it) and, when given, the orbital energies
+    (padded, eig=True) as functions of the learned per-atom parameters, with the implicit backward
+    above.  The engine's parameters, integrals and
     coordinates at forward time are kept, so a later re-parametrisation does not leak into the
     backward.  Molecules whose forward SCF or backward iteration did not converge get zero
     gradients, as in the reference (scf_loop.py:1950-1975)."""
 
     @staticmethod
-    def forward(ctx, eng, P, F, eps, notconverged, names, Ppad, *params):
+    def forward(ctx, eng, P, F, eps, notconverged, names, Ppad, epad, *params):
         ctx.eng, ctx.names, ctx.eps = eng, names, eps
         ctx.state = (P.detach().clone(), F.detach().clone(), eng.atom_par.clone(), eng.coords.clone(),
                      (eng.w_xx.clone(), eng.w_xh.clone(), eng.w_hh.clone()),
                      torch.as_tensor(notconverged, device=P.device).bool().clone())
         ctx.meta = [(p.shape, p.dtype, p.device) for p in params]
-        return Ppad.clone()
+        ctx.has_e = epad is not None
+        return (Ppad.clone(), epad.clone()) if ctx.has_e else Ppad.clone()
 
     @staticmethod
-    def backward(ctx, gP):
+    def backward(ctx, gP, ge=None):
         import warnings
         eng = ctx.eng
         b = eng.b
         P, F, atom_par, coords, w, nc = ctx.state
         gs = 0.5 * (gP + gP.transpose(1, 2))
         gbar = b.from_padded(gs.to(torch.float64)).contiguous()
-        grad, bad = implicit_param_grad(eng, P, F, gbar, atom_par, coords, w, ctx.eps)
+        grad, bad = implicit_param_grad(eng, P, F, gbar, atom_par, coords, w, ctx.eps,
+                                        epad_grad=ge if ctx.has_e else None)
         drop = nc | bad
         if bool(bad.any()):
             warnings.warn("SCF backward for %d/%d molecules did not converge" % (int(bad.sum()), b.nmol))
@@ -173,4 +203,4 @@ class SCFImplicit(torch.autograd.Function):
         from .seqm_functions.parameters import COLUMNS
         grads = [grad[:, COLUMNS.index(n)].reshape(shape).to(dtype=dtype, device=dev)
                  for n, (shape, dtype, dev) in zip(ctx.names, ctx.meta)]
-        return (None, None, None, None, None, None, None, *grads)
+        return (None, None, None, None, None, None, None, None, *grads)

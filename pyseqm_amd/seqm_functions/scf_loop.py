@@ -71,23 +71,24 @@ def scf_loop(molecule, eps=1.0e-4, P=None, sp2=[False], scf_converger=[1], eig=F
     en = eng.energies(r.P)
     F = b.to_padded(en["F"])
     Ppad = b.to_padded(r.P)
-    if scf_backward >= 1:
-        # the density carries the gradient of the SCF fixed point with respect to the learned
-        # parameters (scf_loop.py:1779-1993); 2 (the reference's unrolled iterations) gives the same
-        # gradient at convergence and takes the same implicit route here
-        from ..basics import _trained_parameters
-        trained = _trained_parameters(molecule.learned_parameters)
-        if trained:
-            from ..implicit import SCFImplicit
-            Ppad = SCFImplicit.apply(eng, r.P, en["F"], float(scf_backward_eps), r.notconverged,
-                                     [n for n, _ in trained], Ppad, *[t for _, t in trained])
     Hcore = torch.triu(b.to_padded(eng.H))  # the reference fills only the upper triangle
     w = eng.w_reference()
     rho0 = eng.atom_par[:, 26]
     rho0xi = rho0[molecule.idxi]
     rho0xj = rho0[molecule.idxj]
+    e = V = charge = None
     if eig:
         e, V = eng.eig(en["F"])
         charge = orbital_charges(molecule, V)
-        return F, e, Ppad, Hcore, w, charge, rho0xi, rho0xj, None, None, nc, V
-    return F, None, Ppad, Hcore, w, None, rho0xi, rho0xj, None, None, nc, None
+    if scf_backward >= 1:
+        # the density (and the orbital energies) carry the gradient of the SCF fixed point with
+        # respect to the learned parameters (scf_loop.py:1779-1993); 2 (the reference's unrolled
+        # iterations) gives the same gradient at convergence and takes the same implicit route here
+        from ..basics import _trained_parameters
+        trained = _trained_parameters(molecule.learned_parameters)
+        if trained:
+            from ..implicit import SCFImplicit
+            out = SCFImplicit.apply(eng, r.P, en["F"], float(scf_backward_eps), r.notconverged,
+                                    [n for n, _ in trained], Ppad, e, *[t for _, t in trained])
+            Ppad, e = out if eig else (out, None)
+    return F, e, Ppad, Hcore, w, charge, rho0xi, rho0xj, None, None, nc, V

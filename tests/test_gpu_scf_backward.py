@@ -72,3 +72,48 @@ def test_scf_backward_zero_leaves_the_density_without_graph():
     assert not _density(mol, 1e-8, 0).requires_grad
     with pytest.raises(ValueError):
         scf_loop(mol, eps=1e-8, scf_converger=[2], scf_backward=3)
+
+
+def test_orbital_energy_gradient_matches_reconverged_slope():
+    """scf_loop(eig=True) with scf_backward = 1: the orbital energies carry the implicit gradient
+    too (e_k = c_k^T F(theta, P*) c_k).  The loss, sum of the occupied energies plus 0.05 sum e_k^2
+    plus a density term, is a symmetric function of each degenerate level's energies, so it stays
+    smooth where benzene's and NH3's levels are degenerate."""
+    sp, c = _batch()
+    p = {"method": "AM1", "scf_eps": 1e-10, "scf_converger": [2]}
+    const = pyseqm_amd.Constants().to(DEV)
+    mol = pyseqm_amd.Molecule(const, p, torch.as_tensor(c, device=DEV), torch.as_tensor(sp, device=DEV))
+    eng = mol.engine()
+    name = "U_pp"
+    col = COLUMNS.index(name)
+    base = eng.atom_par[:, col].clone()
+    n4 = 4 * mol.molsize
+    g = torch.Generator(device=DEV).manual_seed(5)
+    W = torch.randn(mol.nmol, n4, n4, generator=g, device=DEV, dtype=torch.float64)
+    W = (W + W.transpose(1, 2)) / (4.0 * n4)
+    occ = torch.arange(n4, device=DEV)[None, :] < mol.nocc.reshape(-1, 1)
+
+    def loss(out):
+        e, P = out[1], out[2]
+        return (e * occ).sum() + 0.05 * (e * e).sum() + (W * P).sum()
+
+    theta = base.clone().requires_grad_(True)
+    mol.use_parameters({name: theta})
+    out = scf_loop(mol, eps=1e-10, scf_converger=[2], eig=True, scf_backward=1, scf_backward_eps=1e-11)
+    assert out[1].requires_grad and out[2].requires_grad
+    loss(out).backward()
+    grad = theta.grad.detach().clone()
+    scale = float(grad.abs().max())
+    Z = np.asarray(eng.b.Z)
+    for a in [int(np.nonzero(Z == z)[0][0]) for z in sorted(set(Z.tolist())) if z > 1]:
+        h = 1e-3 * max(abs(float(base[a])), 0.1)
+        vals = []
+        for s in (-2, -1, 1, 2):
+            v = base.clone()
+            v[a] += s * h
+            mol.use_parameters({name: v})
+            with torch.no_grad():
+                vals.append(float(loss(scf_loop(mol, eps=1e-10, scf_converger=[2], eig=True))))
+        d = (vals[0] - 8.0 * vals[1] + 8.0 * vals[2] - vals[3]) / (12.0 * h)
+        print(f"U_pp atom {a} (Z={Z[a]}): autograd {float(grad[a]):.9e} slope {d:.9e}", flush=True)
+        assert abs(float(grad[a]) - d) <= 2e-5 * scale, (a, float(grad[a]), d, scale)
