@@ -32,6 +32,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64
 
 
 def fock_algorithmic_bytes(b):
@@ -233,6 +234,40 @@ def fock_roofline(eng, P, reps):
             "algorithmic_bytes": nbytes, "launch_ms": ms, "pairs": {"XX": nxx, "XH": nxh, "HH": nhh}}
 
 
+def eigh_roofline(eng, P, reps, step_ms, iters):
+    """The FP64 density step, the dominant kernel of the FP64 SCF: one launch of the workgroup
+    eigensolver (k_eigh_density) over every block, from the converged Fock matrices, timed with HIP
+    events on its launch stream.  Algorithmic flops per block 4/3 n^3 + 2 n^2 nocc (Householder
+    reduction; back-transformation of the occupied vectors and P = 2 C C^T)."""
+    import torch
+    b = eng.b
+    F = torch.empty_like(P)
+    eng.fock(P, F)
+    idx = torch.arange(b.nmol, dtype=torch.int32, device=eng.device)
+    Pout = torch.empty_like(P)
+    eng._idx_host = np.arange(b.nmol)
+    eng.density_eigh(F, idx, b.nmol, Pout)
+    s = torch.cuda.current_stream()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record(s)
+    for _ in range(reps):
+        eng.density_eigh(F, idx, b.nmol, Pout)
+    ev1.record(s)
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1) / reps
+    n, o = b.norb.astype(np.float64), b.nocc.astype(np.float64)
+    flops = float((4.0 / 3.0 * n ** 3 + 2.0 * n * n * o).sum())
+    ach = flops / (ms * 1e-3) / 1e12
+    return {"kernel": "k_eigh_density (FP64 workgroup eigensolver + density, one launch per density step)",
+            "bound": "latency", "achieved": ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": ach / FP64_PEAK_TFLOPS, "algorithmic_flops": flops, "launch_ms": ms,
+            "share_of_step": ms * iters / step_ms,
+            "note": "the dominant kernel of the FP64 step; one 512-thread workgroup per molecule runs a chain of "
+                    "n dependent Householder steps (two barriers each), so the FP64 rate is far below peak by "
+                    "construction; its bound is the sum of the per-block latencies over 256 CUs "
+                    "(profiles/c2_fp64_eigh_kernel_trace.md)"}
+
+
 def kernel_only(args, method, sp, xyz, dev):
     """One kernel group on the workload, --steps times, for counter collection."""
     import torch
@@ -372,6 +407,8 @@ def main():
                    "parallelism": "single GPU, batch over molecules"},
         "roofline": fock_roofline(eng, r.P, args.fock_reps),
     }
+    if args.density == "eigh":
+        line["roofline_eigh"] = eigh_roofline(eng, r.P, max(3, args.fock_reps // 4), ms, max(iters))
     traffic = measured_traffic(args.config)
     if traffic is not None:
         line["roofline"]["traffic"] = traffic
