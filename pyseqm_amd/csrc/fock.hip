@@ -732,6 +732,12 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(PYSEQM_
   const int h0 = L.mol_tile_off[T + m], nxh = L.mol_tile_off[T + m + 1] - h0;
   const int g0 = L.mol_tile_off[2 * T + m], nhh = L.mol_tile_off[2 * T + m + 1] - g0;
   const int ntl = nxx + nxh + nhh;  // <= kMolTiles for at most PYSEQM_FOCK_MOL_ATOMS atoms
+  if (na > PYSEQM_FOCK_MOL_ATOMS || ntl > kMolTiles || 9 * nh + na > stride) {
+    // a layout whose max_mol_atoms / max_mol_slots understate this molecule: nothing is written
+    // beyond the LDS arrays, and Eelec comes back NaN
+    if (tid == 0) eelec[m] = __builtin_nan("");
+    return;  // uniform over the block
+  }
   for (int u = tid; u < ntl; u += 64 * NW) {
     const int tile = u < nxx ? x0 + u : u < nxx + nxh ? h0 + (u - nxx) : g0 + (u - nxx - nxh);
     const Tile g = load_tile(L.tiles, tile);

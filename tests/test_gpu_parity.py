@@ -164,6 +164,28 @@ def test_fock_per_molecule_kernel_matches_tile_kernels(name):
     assert float((e1 - e2).abs().max()) <= 1e-12 * float(e2.abs().max())
 
 
+def test_fock_per_molecule_kernel_refuses_understated_layouts():
+    """A layout whose max_mol_atoms understates its molecules sends them to the per-molecule
+    kernel, which checks each molecule against its LDS arrays: larger ones are skipped with a NaN
+    Eelec (nothing written out of bounds), molecules that fit are built as usual."""
+    sp, c = W.alkanes(24, seed=7, nmin=2, nmax=12)
+    eng = SCFEngine(sp, c, "AM1", DEV)
+    L = eng.b.L
+    P = eng.initial_density()
+    F0 = torch.empty_like(P)
+    e0 = eng.fock(P, F0).clone()
+    saved = L.max_mol_atoms
+    L.max_mol_atoms = 20
+    try:
+        e1 = eng.fock(P, torch.empty_like(P)).clone()
+    finally:
+        L.max_mol_atoms = saved
+    big = torch.as_tensor(eng.b.nat > 24, device=DEV)
+    assert big.any() and (~big).any()
+    assert torch.isnan(e1[big]).all()
+    torch.testing.assert_close(e1[~big], e0[~big], rtol=1e-12, atol=1e-9)
+
+
 def test_fock_is_deterministic():
     sp, c = W.alkanes(32, seed=2)
     eng = SCFEngine(sp, c, "AM1", DEV)
