@@ -181,3 +181,22 @@ def test_request_records_round_trip():
         wire.decode(buf[:-5])
     with pytest.raises(ValueError, match="request record"):
         wire.decode(b"XXXX" + buf[4:])
+
+
+def test_trained_parameters_selects_per_atom_tensors_that_ask_for_gradients():
+    """Energy / scf_loop wrap their outputs for autograd exactly when a learned per-atom
+    parameter requires grad (and grad mode is on); other per-atom tensors and plain values pass
+    through, and names the device gradient does not cover are refused rather than dropped."""
+    import torch
+
+    from pyseqm_amd.basics import _trained_parameters
+    a = torch.ones(3, dtype=torch.float64, requires_grad=True)
+    b = torch.ones(3, dtype=torch.float64)
+    assert _trained_parameters({}) == []
+    assert _trained_parameters({"g_ss": b, "U_ss": 1.0}) == []
+    got = _trained_parameters({"g_ss": a, "beta_s": b})
+    assert [n for n, _ in got] == ["g_ss"] and got[0][1] is a
+    with torch.no_grad():
+        assert _trained_parameters({"g_ss": a}) == []
+    with pytest.raises(NotImplementedError, match="Kbeta"):
+        _trained_parameters({"Kbeta": torch.ones(2, 2, requires_grad=True)})
