@@ -81,19 +81,27 @@ __global__ __launch_bounds__(64) void k_fock_tiles(pyseqm_layout_t L, int tile_b
     if constexpr (KIND == XH) {
       const int S = tile_wstride(t);
       const double* wt = w + L.tile_woff[tile] + lane_rank(t, lane);
-      double pa[10], pab[4];
+      // every load of the tile is issued before any is used (the scheduler otherwise sinks the
+      // diagonal-block loads below the F stores and waits out each one in turn)
+      double pa[10], pab[4], hab[4], wv[10];
+#pragma unroll
+      for (int k = 0; k < 10; ++k) wv[k] = wt[(size_t)k * S];
       ld_tri10(Pm + (size_t)ai * n + ai, n, pa);
+      const double pb = Pm[(size_t)aj * n + aj];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        pab[a] = Pm[(size_t)(ai + a) * n + aj];
+        hab[a] = Hm[(size_t)(ai + a) * n + aj];
+      }
+      __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int k = 0; k < 10; ++k)
         if (kTriI[k] != kTriJ[k]) pa[k] *= 2.0;
-      const double pb = Pm[(size_t)aj * n + aj];
-#pragma unroll
-      for (int a = 0; a < 4; ++a) pab[a] = Pm[(size_t)(ai + a) * n + aj];
       double K[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
       for (int k = 0; k < 10; ++k) {
         const int mu = kTriI[k], nu = kTriJ[k];
-        const double v = wt[(size_t)k * S];
+        const double v = wv[k];
         colv[0] += v * pa[k];
         rowv[k] = v * pb;
         K[mu] += v * pab[nu];
@@ -102,7 +110,7 @@ __global__ __launch_bounds__(64) void k_fock_tiles(pyseqm_layout_t L, int tile_b
       double f[4];
 #pragma unroll
       for (int a = 0; a < 4; ++a) {
-        const double h = Hm[(size_t)(ai + a) * n + aj];
+        const double h = hab[a];
         f[a] = h - 0.5 * K[a];
         Fm[(size_t)(ai + a) * n + aj] = f[a];
         e += pab[a] * (h + f[a]);
