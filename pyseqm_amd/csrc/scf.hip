@@ -79,15 +79,33 @@ __global__ void k_density_from_evecs(pyseqm_layout_t L, const double* __restrict
 }
 
 // Pold = P; P = a*P + (1-a)*Pnew on the active molecules.
+// Element-wise kernels take kElPerThread elements per thread, loads first (a load-then-store loop
+// would wait out one memory round trip per element), so a molecule needs n^2 / (kEl kElPerThread)
+// blocks and small molecules do not dispatch a large molecule's block count of empty blocks.
+constexpr int kElPerThread = 4;
+
 __global__ void k_mix(pyseqm_layout_t L, const int* __restrict__ idx, double a, double* __restrict__ P,
                       const double* __restrict__ Pnew, double* __restrict__ Pold) {
   const int m = idx[blockIdx.y];
   const size_t n2 = (size_t)L.mol_norb[m] * L.mol_norb[m];
   const size_t off = L.mol_mat_off[m];
-  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n2; e += (size_t)gridDim.x * blockDim.x) {
-    const double p = P[off + e];
-    if (Pold) Pold[off + e] = p;
-    P[off + e] = a * p + (1.0 - a) * Pnew[off + e];
+  const size_t step = (size_t)gridDim.x * blockDim.x;
+  for (size_t e0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e0 < n2; e0 += kElPerThread * step) {
+    double p[kElPerThread], q[kElPerThread];
+#pragma unroll
+    for (int u = 0; u < kElPerThread; ++u) {
+      const size_t e = e0 + u * step;
+      p[u] = e < n2 ? P[off + e] : 0.0;
+      q[u] = e < n2 ? Pnew[off + e] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kElPerThread; ++u) {
+      const size_t e = e0 + u * step;
+      if (e < n2) {
+        if (Pold) Pold[off + e] = p[u];
+        P[off + e] = a * p[u] + (1.0 - a) * q[u];
+      }
+    }
   }
 }
 
@@ -203,10 +221,23 @@ __global__ void k_amix_elems(pyseqm_layout_t L, const int* __restrict__ idx, int
   const size_t n2 = (size_t)L.mol_norb[m] * L.mol_norb[m];
   const size_t off = L.mol_mat_off[m];
   const double f = third ? fac[m] : 0.0;
-  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n2; e += (size_t)gridDim.x * blockDim.x) {
-    const double p = P[off + e], pn = Pnew[off + e];
-    Pold[off + e] = p;
-    P[off + e] = third ? (1.0 + f) * pn - f * p : pn;
+  const size_t step = (size_t)gridDim.x * blockDim.x;
+  for (size_t e0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e0 < n2; e0 += kElPerThread * step) {
+    double p[kElPerThread], q[kElPerThread];
+#pragma unroll
+    for (int u = 0; u < kElPerThread; ++u) {
+      const size_t e = e0 + u * step;
+      p[u] = e < n2 ? P[off + e] : 0.0;
+      q[u] = e < n2 ? Pnew[off + e] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kElPerThread; ++u) {
+      const size_t e = e0 + u * step;
+      if (e < n2) {
+        Pold[off + e] = p[u];
+        P[off + e] = third ? (1.0 + f) * q[u] - f * p[u] : q[u];
+      }
+    }
   }
 }
 
@@ -591,21 +622,22 @@ inline int ceil_div(long long a, long long b) { return (int)((a + b - 1) / b); }
 
 inline int elem_blocks(int nmax) {
   const long long n2 = (long long)nmax * nmax;
-  int b = ceil_div(n2, kEl);
+  int b = ceil_div(n2, (long long)kEl * kElPerThread);
   return b < 1 ? 1 : (b > 1024 ? 1024 : b);
 }
 
 inline int status() { return hipGetLastError() == hipSuccess ? PYSEQM_OK : PYSEQM_ELAUNCH; }
 
-// blocks per molecule of the one-wave-per-row kernels (4 rows per block) and of the 32 x 32
-// tile-pair kernels, capped (the kernels loop)
+// blocks per molecule of the one-wave-per-row kernels (4 rows per wave, 16 per block) and of the
+// 32 x 32 tile-pair kernels (2 pairs per block), capped (the kernels loop): sized by the largest
+// block, so fewer and fuller blocks keep small molecules from dispatching many empty ones
 inline int row_blocks(int nmax) {
-  const int b = ceil_div(nmax, kEl / 64);
+  const int b = ceil_div(nmax, 4 * (kEl / 64));
   return b < 1 ? 1 : (b > 1024 ? 1024 : b);
 }
 inline int tile_pair_blocks(int nmax) {
   const long long t = ceil_div(nmax, 32);
-  const long long b = t * (t + 1) / 2;
+  const long long b = ceil_div(t * (t + 1) / 2, 2);
   return (int)(b < 1 ? 1 : (b > 1024 ? 1024 : b));
 }
 
