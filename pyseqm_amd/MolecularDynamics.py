@@ -9,6 +9,8 @@ writers of the reference are not part of this path.  ``XL_BOMD.save_checkpoint``
 ``XL_BOMD.run_from_checkpoint`` (MolecularDynamics.py:1099-1166) write and resume the device-resident
 trajectory state (safetensors, nothing executed on load), continuing a run bit for bit.
 """
+import warnings
+
 import numpy as np
 import torch
 
@@ -270,6 +272,13 @@ class XL_BOMD(Molecular_Dynamics_Basic):
             molecule.Electronic_entropy = tr.EEnt.clone()
             if tr.ksa:
                 molecule.dP2dt2 = b.to_padded(tr.dP2dt2)
+                # eigenvectors the Fermi density's eigensolver could not normalise, counted on the
+                # device over the steps since the last write-back
+                lost = int(tr.ks.fail.item())
+                if lost:
+                    tr.ks.fail.zero_()
+                    warnings.warn(f"KSA-XL-BOMD: the eigensolver lost {lost} eigenvector(s) to a cluster's span "
+                                  "since the last write-back; the thermal densities involved may be inaccurate")
 
 
 class KSA_XL_BOMD(XL_BOMD):

@@ -6,6 +6,7 @@ convergence flags), like the reference's ``int(notconverged.sum())`` host sync
 """
 import collections
 import math
+import warnings
 
 import numpy as np
 import torch
@@ -308,6 +309,14 @@ class SCFEngine:
         else:
             raise ValueError("scf_converger must be [0, alpha], [1] or [2]")
         P, it, nc_host = res
+        if density == "eigh":
+            # the workgroup eigensolver counts eigenvectors it could not normalise (lost to their
+            # cluster's span); one read per SCF
+            lost = int(self.eigh_fail.item())
+            if lost:
+                self.eigh_fail.zero_()
+                warnings.warn(f"FP64 eigensolver: {lost} eigenvector(s) lost to a cluster's span during this SCF; "
+                              "the densities of the molecules involved may be inaccurate")
         out = SCFResult()
         out.P = P
         out.iterations = it

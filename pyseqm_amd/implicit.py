@@ -62,8 +62,12 @@ class ResponseWorkspace:
 
 def eigenpairs(eng, ws, F):
     b = eng.b
+    ws.fail.zero_()
     nat.call("pyseqm_eigh_vectors", b.L, nat.ptr(ws.idx), b.nmol, ws.nmax, nat.ptr(F), nat.ptr(ws.orb_off),
              nat.ptr(ws.evals), nat.ptr(ws.Q), nat.ptr(ws.eig_ws), nat.ptr(ws.fail), eng.stream)
+    if int(ws.fail.item()):
+        raise RuntimeError(f"eigensolver lost {int(ws.fail.item())} eigenvector(s) to a cluster's span: "
+                           "the density response of the SCF backward is not defined for these blocks")
 
 
 def _gemm(eng, ws, ta, tb, A, B, C):
@@ -121,7 +125,7 @@ def adjoint(eng, ws, gbar, atom_par, w, eps, max_iter=300, depth=6):
             k = len(dF)
             A = torch.stack([torch.stack([dots(dF[i], dF[j]) for j in range(k)], 1) for i in range(k)], 1)
             rhs = torch.stack([dots(dF[i], f) for i in range(k)], 1)
-            reg = 1e-12 * A.diagonal(dim1=1, dim2=2).sum(1).clamp_min(1e-300)
+            reg = 1e-12 * A.diagonal(dim1=1, dim2=2).sum(1) + 1e-280  # stays a normal number
             A = A + reg[:, None, None] * torch.eye(k, dtype=A.dtype, device=A.device)
             gam = torch.linalg.solve(A, rhs[..., None])[..., 0]
             u = gu.clone()
