@@ -159,8 +159,9 @@ int pyseqm_pair_forces_xl(const pyseqm_layout_t* L, int method, const double* co
  * Replaces fock() (seqm/seqm_functions/fock.py:132-347) and elec_energy (energy.py:26-49).
  *   active [nmol] (may be NULL = all): molecules with active[m] == 0 are skipped.
  *   part [ntiles_total*16*10], etile [ntiles_total], eatom [natoms]: scratch (untouched when
- *     every molecule has at most PYSEQM_FOCK_MOL_ATOMS atoms: one workgroup per molecule then
- *     builds F and Eelec with the partial sums in LDS).
+ *     the layout's max_mol_atoms is 1 .. PYSEQM_FOCK_MOL_ATOMS: one workgroup per molecule then
+ *     builds F and Eelec with the partial sums in LDS, sized by max_mol_slots; a molecule larger
+ *     than those fields state is skipped with eelec[m] = NaN, nothing written out of bounds).
  *   eelec [nmol] output. */
 #define PYSEQM_FOCK_MOL_ATOMS 24
 int pyseqm_fock(const pyseqm_layout_t* L, const double* atom_par, const double* w_xx,
