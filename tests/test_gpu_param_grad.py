@@ -141,9 +141,19 @@ def test_energy_autograd_with_learned_parameters():
         dT = G[:, 0, k] + G[:, 1, k]
         want = wa * dT + 0.5 * (dT - dI[:, k]) + 3.0 * G[:, 1, k] - dI[:, k]
         torch.testing.assert_close(t.grad, want, rtol=1e-12, atol=1e-12)
-    # the drop-in Force detaches its energies when it computes forces (basics.py:1296-1365)
+    # the drop-in Force detaches its energies when it computes forces, and returns Eelec / Enuc
+    # with their parameter gradients when it does not (basics.py:1296-1365)
     out = Force(p)(mol, learned_parameters=lp)
     assert not any(x.requires_grad for x in out[2:7])
+    out = Force(p)(mol, learned_parameters=lp, do_force=False)
+    assert out[4].requires_grad and out[5].requires_grad
+    assert not any(x.requires_grad for x in (out[2], out[3], out[6]))
+    for t in lp.values():
+        t.grad = None
+    out[5].sum().backward()
+    G = eng.param_grad(eng._last_P)
+    for n, t in lp.items():
+        torch.testing.assert_close(t.grad, G[:, 1, COLUMNS.index(n)], rtol=1e-12, atol=1e-12)
     with torch.no_grad():
         Hf2 = Energy(p)(mol, learned_parameters=lp, all_terms=True)[0]
     assert not Hf2.requires_grad
