@@ -481,6 +481,39 @@ class SCFEngine:
             g[:, COLUMNS.index(name)] = tab(coef)
         return g
 
+    def eigenpairs(self, F, orb_off, evals, Q, eig_ws, fail):
+        """Every eigenpair of every block of F: Q in the packed layout (column j = eigenvector j,
+        ascending eigenvalues) and evals[orb_off[m] + j].  Blocks whose full decomposition fits the
+        workgroup eigensolver's LDS (at most 139 orbitals) take pyseqm_eigh_vectors; larger ones the
+        batched library eigensolver on Gershgorin-padded stacks per size group, scattered into the
+        same layout.  eig_ws holds 4 nmol n^2 doubles for the n of the workgroup blocks; fail counts
+        vectors the workgroup solver lost (read by the caller)."""
+        b = self.b
+        norb = np.asarray(b.norb)
+        wg = (norb <= EIGH_WG_MAX_N) & (norb * norb + 6 * norb + 128 <= EIGH_WG_LDS_DOUBLES)
+        ids = np.nonzero(wg)[0]
+        if len(ids):
+            idx = torch.as_tensor(ids.astype(np.int32), device=self.device)
+            nat.call("pyseqm_eigh_vectors", b.L, nat.ptr(idx), len(ids), int(norb[ids].max()), nat.ptr(F),
+                     nat.ptr(orb_off), nat.ptr(evals), nat.ptr(Q), nat.ptr(eig_ws), nat.ptr(fail), self.stream)
+        rest = np.nonzero(~wg)[0]
+        if len(rest) == 0:
+            return
+        off_h = np.concatenate([[0], np.cumsum(norb)[:-1]]).astype(np.int64)
+        for sel in size_groups(norb[rest]):
+            g = rest[sel]
+            nmax = int(norb[g].max())
+            idx = torch.as_tensor(g.astype(np.int32), device=self.device)
+            Fp = torch.empty(len(g), nmax, nmax, dtype=torch.float64, device=self.device)
+            nat.call("pyseqm_pad_for_eigh", b.L, nat.ptr(F), nat.ptr(idx), len(g), nmax, nat.ptr(Fp),
+                     nat.ptr(self.red), nat.red_blocks(nmax), self.stream)
+            e0, V = torch.linalg.eigh(Fp)  # padded eigenvalues lie above each block's spectrum
+            for k, m in enumerate(g):
+                n = int(norb[m])
+                o = int(b.mol_mat_off[m])
+                Q[o:o + n * n].view(n, n).copy_(V[k, :n, :n])
+                evals[int(off_h[m]):int(off_h[m]) + n].copy_(e0[k, :n])
+
     def eig(self, F):
         """Orbital energies in the reference layout e (nmol, 4*molsize), zero padded, and the packed
         eigenvectors (nmol, nmax, nmax), sym_eig_trunc(eig_only=True) (diag.py:110-241)."""

@@ -10,7 +10,8 @@ nocc lowest eigenpairs), an upstream gradient gbar = dL/dP* gives
 with G the two-electron part of the Fock build (self-adjoint in the Frobenius product) and R the
 first-order density response of D to a Fock perturbation,
 R(X) = C [M o (C^T X C)] C^T, M_ia = M_ai = 2 / (e_i - e_a) for occupied i and virtual a
-(self-adjoint as well).  Every piece is a device kernel: the eigenpairs (pyseqm_eigh_vectors),
+(self-adjoint as well).  Every piece is a device kernel: the eigenpairs (pyseqm_eigh_vectors; the
+batched library eigensolver for blocks above its 139-orbital all-eigenpairs limit),
 the eigenbasis products (pyseqm_block_gemm), M (pyseqm_density_response_elem), G (pyseqm_fock
 with a zero core Hamiltonian) and the parameter contraction (pyseqm_fock_param_vjp).  The
 Anderson mixing of the adjoint iterates runs as per-molecule torch reductions on the packed
@@ -29,10 +30,6 @@ class ResponseWorkspace:
         b = eng.b
         dev = eng.device
         norb, nocc = np.asarray(b.norb), np.asarray(b.nocc)
-        ok = (norb <= 192) & np.array([bool(nat.lib().pyseqm_eigh_density_lds_ok(int(n), int(n))) for n in norb])
-        if not ok.all():
-            raise NotImplementedError("the SCF implicit backward takes blocks whose full eigen-decomposition fits the "
-                                      f"workgroup eigensolver's LDS (largest here: {int(norb.max())} orbitals)")
         if ((nocc < 1) | (nocc >= norb)).any():
             raise ValueError("the density response needs 1 <= nocc < norb in every molecule")
         f64 = dict(dtype=torch.float64, device=dev)
@@ -42,7 +39,8 @@ class ResponseWorkspace:
         self.orb_off = torch.as_tensor(np.concatenate([[0], np.cumsum(norb)[:-1]]).astype(np.int64), device=dev)
         self.evals = torch.zeros(max(int(norb.sum()), 1), **f64)
         self.Q = torch.zeros(total, **f64)
-        self.eig_ws = torch.zeros(max(4 * b.nmol * int(norb.max()) ** 2, 1), **f64)
+        wg = norb[norb * norb + 6 * norb + 128 <= 20480]
+        self.eig_ws = torch.zeros(max(4 * b.nmol * int(wg.max()) ** 2 if len(wg) else 1, 1), **f64)
         self.fail = torch.zeros(1, dtype=torch.int32, device=dev)
         self.Hzero = torch.zeros(total, **f64)
         self.T1 = torch.zeros(total, **f64)
@@ -61,10 +59,8 @@ class ResponseWorkspace:
 
 
 def eigenpairs(eng, ws, F):
-    b = eng.b
     ws.fail.zero_()
-    nat.call("pyseqm_eigh_vectors", b.L, nat.ptr(ws.idx), b.nmol, ws.nmax, nat.ptr(F), nat.ptr(ws.orb_off),
-             nat.ptr(ws.evals), nat.ptr(ws.Q), nat.ptr(ws.eig_ws), nat.ptr(ws.fail), eng.stream)
+    eng.eigenpairs(F, ws.orb_off, ws.evals, ws.Q, ws.eig_ws, ws.fail)
     if int(ws.fail.item()):
         raise RuntimeError(f"eigensolver lost {int(ws.fail.item())} eigenvector(s) to a cluster's span: "
                            "the density response of the SCF backward is not defined for these blocks")

@@ -36,10 +36,9 @@ class KSAWorkspace:
         if not 1 <= int(rank) <= self.R:
             raise NotImplementedError(f"max_rank = {rank}: the device Krylov fit takes 1 .. {self.R}")
         norb, nocc = np.asarray(b.norb), np.asarray(b.nocc)
-        ok = (norb <= 192) & np.array([bool(nat.lib().pyseqm_eigh_density_lds_ok(int(n), int(n))) for n in norb])
-        if not ok.all():
-            raise NotImplementedError("the thermal density takes blocks whose full eigen-decomposition fits the "
-                                      f"workgroup eigensolver's LDS (largest here: {int(norb.max())} orbitals)")
+        if int(norb.max()) > 256:
+            raise NotImplementedError("the thermal density and its perturbation recursion take blocks of up to 256 "
+                                      f"orbitals (largest here: {int(norb.max())})")
         if ((nocc < 1) | (nocc >= norb)).any():
             raise ValueError("the Fermi density needs 1 <= nocc < norb in every molecule (HOMO and LUMO)")
         f64 = dict(dtype=torch.float64, device=dev)
@@ -53,7 +52,8 @@ class KSAWorkspace:
         self.S = torch.zeros(b.nmol, **f64)
         self.Q = torch.zeros(total, **f64)
         self.work = torch.zeros(int(nat.lib().pyseqm_fermi_workspace(b.nmol)), dtype=torch.uint8, device=dev)
-        self.eig_ws = torch.zeros(max(4 * b.nmol * int(norb.max()) ** 2, 1), **f64)
+        wg = norb[norb * norb + 6 * norb + 128 <= 20480]
+        self.eig_ws = torch.zeros(max(4 * b.nmol * int(wg.max()) ** 2 if len(wg) else 1, 1), **f64)
         self.fail = torch.zeros(1, dtype=torch.int32, device=dev)
         self.Hzero = torch.zeros(total, **f64)
         self.G = torch.zeros(total, **f64)
@@ -96,8 +96,7 @@ def fermi_density(eng, ks, F, T, D, kB=KB_EV):
     stay in the workspace (ks.Q, ks.evals, ks.occ, ks.mu, ks.S)."""
     b = eng.b
     nmax = int(b.nmax)
-    nat.call("pyseqm_eigh_vectors", b.L, nat.ptr(ks.idx), b.nmol, nmax, nat.ptr(F), nat.ptr(ks.orb_off),
-             nat.ptr(ks.evals), nat.ptr(ks.Q), nat.ptr(ks.eig_ws), nat.ptr(ks.fail), eng.stream)
+    eng.eigenpairs(F, ks.orb_off, ks.evals, ks.Q, ks.eig_ws, ks.fail)
     nat.call("pyseqm_fermi_density", b.L, nat.ptr(ks.idx), b.nmol, nmax, nat.ptr(ks.orb_off), nat.ptr(ks.evals),
              nat.ptr(ks.Q), float(T), float(kB), nat.ptr(ks.occ), nat.ptr(ks.mu), nat.ptr(D), nat.ptr(ks.S),
              nat.ptr(ks.work), eng.stream)

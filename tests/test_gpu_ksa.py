@@ -57,6 +57,41 @@ def test_eigenpairs_reconstruct_the_blocks():
     assert int(ks.fail.item()) == 0
 
 
+def test_eigenpairs_and_fermi_density_above_the_workgroup_limit():
+    """A (H2O)_40 block (240 orbitals) is past the workgroup eigensolver's 139-orbital all-eigenpairs
+    limit and takes the batched library eigensolver; methane beside it the workgroup solver.  Both
+    land in the same packed layout: the eigenpairs reconstruct each block and the Fermi density
+    matches the oracle's at 15000 K."""
+    from pyseqm_amd.workloads import _pad, water_cluster
+    sp, c = _pad([water_cluster(40, seed=3), tuple(a[0] for a in read_xyz(["methane.xyz"]))])
+    print("(H2O)_40 + methane, AM1: SCF", flush=True)
+    eng = SCFEngine(sp, c, "AM1", DEV)
+    r = eng.scf(eps=1e-8, converger=(1,))
+    F = torch.empty_like(r.P)
+    eng.fock(r.P, F)
+    b = eng.b
+    assert int(b.norb.max()) == 240
+    ks = K.KSAWorkspace(eng, 3)
+    D = torch.zeros_like(F)
+    K.fermi_density(eng, ks, F, 15000.0, D)
+    torch.cuda.synchronize()
+    assert int(ks.fail.item()) == 0
+    offs = ks.orb_off.cpu().numpy()
+    for m, (f, q) in enumerate(zip(_blocks(eng, F), _blocks(eng, ks.Q))):
+        n = f.shape[0]
+        e = ks.evals[int(offs[m]):int(offs[m]) + n].cpu().numpy()
+        scale = np.abs(f).max()
+        np.testing.assert_allclose(e, np.linalg.eigvalsh(f), atol=1e-10 * scale, rtol=0)
+        np.testing.assert_allclose(q.T @ q, np.eye(n), atol=1e-12)
+        np.testing.assert_allclose((q * e) @ q.T, f, atol=1e-10 * scale, rtol=0)
+    ob = R.parse(sp, c, "AM1")
+    Do, So, es, Qs, fs, mus = R.fermi_q(ob, b.to_padded(F).cpu().numpy(), 15000.0)
+    np.testing.assert_allclose(b.to_padded(D).cpu().numpy(), Do, atol=1e-10, rtol=0)
+    np.testing.assert_allclose(ks.S.cpu().numpy(), So, atol=1e-12, rtol=1e-10)
+    for m in range(b.nmol):
+        assert abs(float(ks.mu[m]) - mus[m]) < 1e-9
+
+
 @pytest.mark.parametrize("T", [1500.0, 15000.0])
 def test_fermi_density_matches_oracle(T):
     eng, P, F, ob, H, Tt, *_ = _engine()

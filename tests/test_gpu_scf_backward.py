@@ -117,3 +117,44 @@ def test_orbital_energy_gradient_matches_reconverged_slope():
         d = (vals[0] - 8.0 * vals[1] + 8.0 * vals[2] - vals[3]) / (12.0 * h)
         print(f"U_pp atom {a} (Z={Z[a]}): autograd {float(grad[a]):.9e} slope {d:.9e}", flush=True)
         assert abs(float(grad[a]) - d) <= 2e-5 * scale, (a, float(grad[a]), d, scale)
+
+
+def test_implicit_backward_above_the_workgroup_limit():
+    """(H2O)_32 (192 orbitals, past the workgroup eigensolver's 139-orbital all-eigenpairs limit: its
+    eigenpairs come from the batched library eigensolver) beside methanal (workgroup solver): the
+    implicit gradient of the weighted-density loss against the re-converged slope, PM3 g_ss."""
+    from pyseqm_amd.workloads import _pad, water_cluster
+    from tests.conftest import read_xyz
+    sp, c = _pad([water_cluster(32, seed=1), tuple(a[0] for a in read_xyz(["methanal.1.xyz"]))])
+    p = {"method": "PM3", "scf_eps": 1e-10, "scf_converger": [2]}
+    const = pyseqm_amd.Constants().to(DEV)
+    mol = pyseqm_amd.Molecule(const, p, torch.as_tensor(c, device=DEV), torch.as_tensor(sp, device=DEV))
+    eng = mol.engine()
+    assert int(eng.b.norb.max()) == 192
+    name = "g_ss"
+    base = eng.atom_par[:, COLUMNS.index(name)].clone()
+    g = torch.Generator(device=DEV).manual_seed(3)
+    n4 = 4 * mol.molsize
+    W = torch.randn(mol.nmol, n4, n4, generator=g, device=DEV, dtype=torch.float64)
+    W = (W + W.transpose(1, 2)) / (2.0 * n4)
+    theta = base.clone().requires_grad_(True)
+    mol.use_parameters({name: theta})
+    print("(H2O)_32 + methanal PM3: implicit backward", flush=True)
+    (W * _density(mol, 1e-10, 1)).sum().backward()
+    grad = theta.grad.detach().clone()
+    scale = float(grad.abs().max())
+    assert torch.isfinite(grad).all() and scale > 0.0
+    Z = np.asarray(eng.b.Z)
+    # the first O and H of the cluster, methanal's C
+    for a in [0, 32, int(np.nonzero(Z == 6)[0][0])]:
+        h = 1e-3 * max(abs(float(base[a])), 0.1)
+        vals = []
+        for s in (-2, -1, 1, 2):
+            v = base.clone()
+            v[a] += s * h
+            mol.use_parameters({name: v})
+            with torch.no_grad():
+                vals.append(float((W * _density(mol, 1e-10, 0)).sum()))
+        d = (vals[0] - 8.0 * vals[1] + 8.0 * vals[2] - vals[3]) / (12.0 * h)
+        print(f"atom {a} (Z={Z[a]}): autograd {float(grad[a]):.9e} slope {d:.9e}", flush=True)
+        assert abs(float(grad[a]) - d) <= 2e-5 * scale, (a, float(grad[a]), d, scale)
