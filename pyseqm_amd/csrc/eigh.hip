@@ -108,6 +108,13 @@ __device__ inline double row16_sum(double v) {
   return v;
 }
 
+// Sum of 32 values held by lanes 0..31 (and again by lanes 32..63): each 16-lane row sums by DPP,
+// then the two rows of each half add (a + b in one lane, b + a in its partner: the same bits).
+__device__ inline double sum32(double v) {
+  v = row16_sum(v);
+  return v + __shfl_xor(v, 16);
+}
+
 // Sum / max / min over aligned groups of t lanes (t a power of two, the same in every lane):
 // DPP within a 16-lane row, shuffles across rows.
 __device__ inline double group_sum_t(double v, int t) {
@@ -217,95 +224,128 @@ __global__ __launch_bounds__(GR* GC) void k_eigh_density(pyseqm_layout_t L, cons
     }
     red1[tr] = part;
   }
+  const int lane32 = tid & 31;
   for (int i = 0; i + 2 < n; ++i) {
-    double* x = (i & 1) ? xs1 : xs0;
+    const double* x = (i & 1) ? xs1 : xs0;
     double* xnext = (i & 1) ? xs0 : xs1;
     __syncthreads();
-    double xn2 = 0.0;
+    // every LDS read of the step's first half is issued here, unconditionally, so that they travel
+    // together instead of one round trip per slot.  Indices run to GC RC - 1 = GR RR - 1, past this
+    // block's n: the reads land in the arrays after x and p (the other reflector buffer, p, kRed
+    // doubles of scratch), and the LDS is sized for the launch's largest block nmax, which picks
+    // the variant (192, 96, 48 slots for nmax > 96, > 48, >= 1), so p + GC RC - 1 <= (the block's
+    // offset of p) + nmax + kRed - 1 stays inside it.  Values at indices >= n are masked out below.
+    double xc[RC], xr[RR];
 #pragma unroll
-    for (int k = 0; k < GR; ++k) xn2 += red1[k];  // fixed order in every thread
+    for (int b = 0; b < RC; ++b) xc[b] = x[tc + GC * b];
+#pragma unroll
+    for (int a = 0; a < RR; ++a) xr[a] = x[tr + GR * a];
     const double alpha = x[i + 1];
+    // |x[1:]|^2 from the GR partials: lane k of every wave takes partial k mod 32, then a fixed
+    // tree (16-lane DPP row, then the row pair): the same bits in every thread
+    const double r1 = red1[lane32];  // red1 holds kRed / 2 = 64 doubles: in bounds
+    const double xn2 = sum32(lane32 < GR ? r1 : 0.0);
     double tau = 0.0, beta = alpha, scal = 0.0;
     if (xn2 > 0.0) {
       beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
-      tau = (beta - alpha) / beta;
-      scal = 1.0 / (alpha - beta);
+      const double am = alpha - beta;
+      scal = rcp_f64(am);
+      tau = -am * rcp_f64(beta);
     }
     // tau = 0 (column already zero) runs the same steps with p = w = 0: A unchanged
     if (tid == 0) {
       e[i] = beta;
       HH[(size_t)i * nmax + i] = tau;
     }
-    // v at column c (0 at c <= i); the loops run over this thread's columns outermost so that only
-    // one column's v and w are live beside the register block
-    auto vcol = [&](int c) { return (c <= i || c >= n) ? 0.0 : (c == i + 1 ? 1.0 : x[c] * scal); };
+    // v at this thread's columns and rows (0 at indices <= i and beyond the block)
+    auto vval = [&](int c, double xv) { return (c <= i || c >= n) ? 0.0 : (c == i + 1 ? 1.0 : xv * scal); };
+    double vc[RC], vr[RR];
+#pragma unroll
+    for (int b = 0; b < RC; ++b) vc[b] = vval(tc + GC * b, xc[b]);
+#pragma unroll
+    for (int a = 0; a < RR; ++a) vr[a] = vval(tr + GR * a, xr[a]);
+    if (tr == 0) {
+#pragma unroll
+      for (int b = 0; b < RC; ++b) {
+        const int c = tc + GC * b;
+        if (c > i && c < n) HH[(size_t)i * nmax + c] = vc[b];
+      }
+    }
+    // p = tau A v.  Column slots whose columns are all finished (v = 0) or all beyond the block are
+    // skipped (a uniform test); every row of a live slot is summed: finished rows and rows beyond
+    // the block give values that are never used (p is kept at live rows only).
     double s[RR];
 #pragma unroll
     for (int a = 0; a < RR; ++a) s[a] = 0.0;
 #pragma unroll
     for (int b = 0; b < RC; ++b) {
-      // slots whose columns are all finished (v = 0) or all beyond the block are skipped (uniform)
       if (GC * b + GC - 1 <= i || GC * b >= n) continue;
-      const int c = tc + GC * b;
-      const double vb = vcol(c);
-      if (tr == 0 && c > i && c < n) HH[(size_t)i * nmax + c] = vb;
 #pragma unroll
-      for (int a = 0; a < RR; ++a)
-        if (GR * a + GR - 1 > i && GR * a < n) s[a] += Ar[a][b] * vb;  // live rows only
+      for (int a = 0; a < RR; ++a) s[a] = fma(Ar[a][b], vc[b], s[a]);
     }
     double q = 0.0;
 #pragma unroll
     for (int a = 0; a < RR; ++a) {
-      const double sa = row16_sum(s[a]);
+      const double pa = tau * row16_sum(s[a]);
       const int r = tr + GR * a;
-      if (tc == 0 && r > i && r < n) {
-        const double pr = tau * sa;
-        pp[r] = pr;
-        q += pr * (r == i + 1 ? 1.0 : x[r] * scal);
-      }
+      q = fma(pa, vr[a], q);  // vr = 0 off the live rows
+      if (tc == 0 && r > i && r < n) pp[r] = pa;
     }
     if (tc == 0) red2[tr] = q;
     __syncthreads();
-    double pv = 0.0;
+    double wr[RR];
 #pragma unroll
-    for (int k = 0; k < GR; ++k) pv += red2[k];
+    for (int a = 0; a < RR; ++a) wr[a] = pp[tr + GR * a];
+    const double r2 = red2[lane32];
+    const double pv = sum32(lane32 < GR ? r2 : 0.0);
     const double a2 = -0.5 * tau * pv;
-    double vr[RR], wr[RR];
 #pragma unroll
     for (int a = 0; a < RR; ++a) {
       const int r = tr + GR * a;
-      vr[a] = 0.0;
-      wr[a] = 0.0;
-      if (r > i && r < n) {
-        vr[a] = r == i + 1 ? 1.0 : x[r] * scal;
-        wr[a] = pp[r] + a2 * vr[a];
+      wr[a] = (r > i && r < n) ? fma(a2, vr[a], wr[a]) : 0.0;
+    }
+    // A -= v w^T + w v^T on the live column slots, every row (v = w = 0 at finished rows and rows
+    // beyond the block leave them exactly unchanged).  Column i + 1 (slot bn, a uniform index) is
+    // copied out as it is updated and handed to the next step by its owners (tc == tn): the
+    // reflector input x, its partial squared norm and the diagonal element.
+    const int bn = (i + 1) / GC, tn = (i + 1) - bn * GC;
+    double coln[RR];
+#pragma unroll
+    for (int a = 0; a < RR; ++a) coln[a] = 0.0;
+    // columns in chunks of kUpdChunk slots: the chunk's p values are read together, then applied
+    constexpr int kUpdChunk = 4;
+#pragma unroll
+    for (int b0 = 0; b0 < RC; b0 += kUpdChunk) {
+      if (GC * min(b0 + kUpdChunk, RC) - 1 <= i || GC * b0 >= n) continue;  // whole chunk idle (uniform)
+      double wc[kUpdChunk];
+#pragma unroll
+      for (int j = 0; j < kUpdChunk; ++j)
+        if (b0 + j < RC) wc[j] = pp[tc + GC * (b0 + j)];
+#pragma unroll
+      for (int j = 0; j < kUpdChunk; ++j) {
+        const int b = b0 + j;
+        if (b >= RC) break;
+        if (GC * b + GC - 1 <= i || GC * b >= n) continue;
+        const int c = tc + GC * b;
+        const double vb = vc[b];
+        const double wb = (c > i && c < n) ? fma(a2, vb, wc[j]) : 0.0;
+#pragma unroll
+        for (int a = 0; a < RR; ++a) Ar[a][b] = fma(-vr[a], wb, fma(-wr[a], vb, Ar[a][b]));
+        if (b == bn) {
+#pragma unroll
+          for (int a = 0; a < RR; ++a) coln[a] = Ar[a][b];
+        }
       }
     }
-    // the owners of column i + 1 (tc == (i + 1) mod GC, slot bn) hand it to the next step as they
-    // update it: the reflector input x, its partial squared norm and the diagonal element
-    const int bn = (i + 1) / GC, tn = (i + 1) - bn * GC;
     double partn = 0.0;
+    if (tc == tn) {
 #pragma unroll
-    for (int b = 0; b < RC; ++b) {
-      if (GC * b + GC - 1 <= i || GC * b >= n) continue;  // v = w = 0 on every column of the slot
-      const int c = tc + GC * b;
-      const double vb = vcol(c);
-      const double wb = (c > i && c < n) ? pp[c] + a2 * vb : 0.0;
-#pragma unroll
-      for (int a = 0; a < RR; ++a)
-        if (GR * a + GR - 1 > i && GR * a < n) {
-          const double val = fma(-vr[a], wb, fma(-wr[a], vb, Ar[a][b]));
-          Ar[a][b] = val;
-          if (b == bn && tc == tn) {
-            const int r = tr + GR * a;
-            if (r == i + 1) {
-              d[i + 1] = val;
-            } else if (r > i + 1 && r < n) {
-              xnext[r] = val;
-              if (r > i + 2) partn += val * val;
-            }
-          }
-        }
+      for (int a = 0; a < RR; ++a) {
+        const int r = tr + GR * a;
+        if (r == i + 1) d[i + 1] = coln[a];
+        if (r > i + 1 && r < n) xnext[r] = coln[a];
+        if (r > i + 2 && r < n) partn = fma(coln[a], coln[a], partn);
+      }
     }
     if (tc == tn) red1[tr] = partn;
   }
