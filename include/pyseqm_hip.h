@@ -199,9 +199,10 @@ int pyseqm_pad_for_eigh(const pyseqm_layout_t* L, const double* F, const int* id
 /* FP64 density step by diagonalisation with one workgroup per listed molecule, for blocks of up to
  * 192 orbitals: P = 2 C_occ C_occ^T of each packed Fock block F (sym_eig_trunc, diag.py:110-241,
  * and the density of make_Pnew, scf_loop.py:55-103), written to the packed P, with no padding.
- * Householder tridiagonalisation in LDS, multisection for the nocc lowest eigenvalues, inverse
- * iteration, Gram-Schmidt inside clusters of close eigenvalues, back-transformation.  nmax /
- * nocc_max bound the listed blocks (pyseqm_eigh_density_lds_ok(nmax, nocc_max) must hold); ws
+ * Householder tridiagonalisation with the block in registers, multisection for the nocc lowest
+ * eigenvalues, inverse iteration, Gram-Schmidt inside clusters of close eigenvalues,
+ * back-transformation.  nmax / nocc_max must be at least every listed block's norb / nocc (they
+ * size the LDS and pick the kernel variant; pyseqm_eigh_density_lds_ok(nmax, nocc_max) must hold); ws
  * holds pyseqm_eigh_density_workspace(nact, nmax) doubles; fail [1] counts vectors that
  * Gram-Schmidt reduced to zero (dependent vectors; it stays 0). */
 long long pyseqm_eigh_density_workspace(int nact, int nmax);

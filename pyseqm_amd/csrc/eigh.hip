@@ -154,11 +154,65 @@ __device__ inline double start_value(int j, int k) {
   return (double)h * (2.0 / 4294967296.0) - 1.0;
 }
 
+// C = Q Z on the register block (thread (tr, tc) holds C[tc + 16 b][tr + GR c]): the reflectors
+// last to first, C -= v (tau v^T C).  Only the first CMAX column slots hold vectors (a compile-time
+// count, so the work follows nv).  Every row slot takes part in every step: v is zero at rows <= i
+// and beyond the block, so those products and updates are exact no-ops, and the loops stay free
+// of per-slot selects; the CMAX column chains run interleaved.  The next reflector is read from
+// the workspace while the current one is applied.  C (in LDS, n x nv, over Z) is written after a
+// barrier that follows every thread's read of Z.
+template <int GR, int RR, int RC, int CMAX>
+__device__ __forceinline__ void back_transform(double (&Ar)[RR][RC], const double* __restrict__ HH, int nmax, int n,
+                                               int nv, int tr, int tc, double* C) {
+  double vcur[RC], vnext[RC];
+  double tcur = 0.0, tnext = 0.0;
+  auto load_v = [&](int i, double* v, double& t) {
+    t = HH[(size_t)i * nmax + i];
+#pragma unroll
+    for (int b = 0; b < RC; ++b) {
+      const int k = tc + 16 * b;
+      v[b] = (k > i && k < n) ? HH[(size_t)i * nmax + k] : 0.0;
+    }
+  };
+  if (n >= 3) load_v(n - 3, vcur, tcur);
+  for (int i = n - 3; i >= 0; --i) {
+    if (i >= 1) load_v(i - 1, vnext, tnext);
+    if (tcur != 0.0) {  // the same in every thread
+      double sd[CMAX];
+#pragma unroll
+      for (int c = 0; c < CMAX; ++c) sd[c] = 0.0;
+#pragma unroll
+      for (int b = 0; b < RC; ++b)
+#pragma unroll
+        for (int c = 0; c < CMAX; ++c) sd[c] = fma(Ar[c][b], vcur[b], sd[c]);
+#pragma unroll
+      for (int c = 0; c < CMAX; ++c) sd[c] = tcur * row16_sum(sd[c]);
+#pragma unroll
+      for (int b = 0; b < RC; ++b)
+#pragma unroll
+        for (int c = 0; c < CMAX; ++c) Ar[c][b] = fma(-vcur[b], sd[c], Ar[c][b]);
+    }
+#pragma unroll
+    for (int b = 0; b < RC; ++b) vcur[b] = vnext[b];
+    tcur = tnext;
+  }
+  __syncthreads();  // every thread has read Z
+#pragma unroll
+  for (int c = 0; c < CMAX; ++c)
+#pragma unroll
+    for (int b = 0; b < RC; ++b) {
+      const int k = tc + 16 * b, a = tr + GR * c;
+      if (k < n && a < nv) C[k * nv + a] = Ar[c][b];
+    }
+}
+
 // EVEC = false: the density P = 2 C_occ C_occ^T of the nocc lowest vectors.  EVEC = true: every
 // eigenpair (nv = n vectors), written as Qout (packed layout, column j = eigenvector j) and evals
 // (orb_off[m] + j, the Rayleigh quotients of the final vectors: exact to rounding whatever the
 // multisection bracket width).
-template <int GR, int GC, int RR, int RC, bool EVEC>
+// CM: column slots of the back-transformation that hold vectors (GR CM >= nv of every listed block,
+// chosen per launch from the launch's largest nv).
+template <int GR, int GC, int RR, int RC, bool EVEC, int CM>
 __global__ __launch_bounds__(GR* GC) void k_eigh_density(pyseqm_layout_t L, const int* __restrict__ idx,
                                                        const double* __restrict__ F, double* __restrict__ P,
                                                        double* __restrict__ ws, int nmax,
@@ -544,45 +598,7 @@ __global__ __launch_bounds__(GR* GC) void k_eigh_density(pyseqm_layout_t L, cons
         const int k = tc + 16 * b, a = tr + GR * c;
         Ar[c][b] = (k < n && a < nv) ? Z[k * nv + a] : 0.0;
       }
-    double vcur[RC], vnext[RC];
-    double tcur = 0.0, tnext = 0.0;
-    auto load_v = [&](int i, double* v, double& t) {
-      t = HH[(size_t)i * nmax + i];
-#pragma unroll
-      for (int b = 0; b < RC; ++b) {
-        const int k = tc + 16 * b;
-        v[b] = (k > i && k < n) ? HH[(size_t)i * nmax + k] : 0.0;
-      }
-    };
-    if (n >= 3) load_v(n - 3, vcur, tcur);
-    for (int i = n - 3; i >= 0; --i) {
-      if (i >= 1) load_v(i - 1, vnext, tnext);
-      if (tcur != 0.0) {  // the same in every thread
-#pragma unroll
-        for (int c = 0; c < RR; ++c) {
-          if (GR * c >= nv) continue;  // no column of this slot exists (uniform)
-          double sd = 0.0;
-#pragma unroll
-          for (int b = 0; b < RC; ++b)  // slots wholly at rows <= i (v = 0) or beyond the block skipped
-            if (16 * b + 15 > i && 16 * b < n) sd += Ar[c][b] * vcur[b];
-          sd = tcur * row16_sum(sd);
-#pragma unroll
-          for (int b = 0; b < RC; ++b)
-            if (16 * b + 15 > i && 16 * b < n) Ar[c][b] = fma(-vcur[b], sd, Ar[c][b]);
-        }
-      }
-#pragma unroll
-      for (int b = 0; b < RC; ++b) vcur[b] = vnext[b];
-      tcur = tnext;
-    }
-    __syncthreads();  // every thread has read Z
-#pragma unroll
-    for (int c = 0; c < RR; ++c)
-#pragma unroll
-      for (int b = 0; b < RC; ++b) {
-        const int k = tc + 16 * b, a = tr + GR * c;
-        if (k < n && a < nv) C[k * nv + a] = Ar[c][b];
-      }
+    back_transform<GR, RR, RC, CM>(Ar, HH, nmax, n, nv, tr, tc, C);
     __syncthreads();
   }
   if (EVEC) {
@@ -634,15 +650,30 @@ extern "C" long long pyseqm_eigh_density_workspace(int nact, int nmax) {
 }
 
 // thread grid GR x 16 with RR x RC register slots per thread: GR RR and 16 RC cover nmax
-template <bool EVEC>
-void launch_eigh(const pyseqm_layout_t* L, const int* idx, int nact, int nmax, const double* F, double* out,
-                 double* ws, int* fail, const long long* orb_off, double* evals, size_t lds, hipStream_t s) {
-  if (nmax <= 48)
-    k_eigh_density<4, 16, 12, 3, EVEC><<<nact, 64, lds, s>>>(*L, idx, F, out, ws, nmax, fail, orb_off, evals);
-  else if (nmax <= 96)
-    k_eigh_density<16, 16, 6, 6, EVEC><<<nact, 256, lds, s>>>(*L, idx, F, out, ws, nmax, fail, orb_off, evals);
+// (the back-transformation's column slots: half of them when every listed block's vector count
+// nvmax fits, which the density step's nocc usually does)
+template <int GR, int RR, int RC, bool EVEC>
+void launch_variant(const pyseqm_layout_t* L, const int* idx, int nact, int nmax, int nvmax, const double* F,
+                    double* out, double* ws, int* fail, const long long* orb_off, double* evals, size_t lds,
+                    hipStream_t s) {
+  if (nvmax <= GR * (RR / 2))
+    k_eigh_density<GR, 16, RR, RC, EVEC, RR / 2><<<nact, GR * 16, lds, s>>>(*L, idx, F, out, ws, nmax, fail, orb_off,
+                                                                         evals);
   else
-    k_eigh_density<32, 16, 6, 12, EVEC><<<nact, 512, lds, s>>>(*L, idx, F, out, ws, nmax, fail, orb_off, evals);
+    k_eigh_density<GR, 16, RR, RC, EVEC, RR><<<nact, GR * 16, lds, s>>>(*L, idx, F, out, ws, nmax, fail, orb_off,
+                                                                      evals);
+}
+
+template <bool EVEC>
+void launch_eigh(const pyseqm_layout_t* L, const int* idx, int nact, int nmax, int nvmax, const double* F,
+                 double* out, double* ws, int* fail, const long long* orb_off, double* evals, size_t lds,
+                 hipStream_t s) {
+  if (nmax <= 48)
+    launch_variant<4, 12, 3, EVEC>(L, idx, nact, nmax, nvmax, F, out, ws, fail, orb_off, evals, lds, s);
+  else if (nmax <= 96)
+    launch_variant<16, 6, 6, EVEC>(L, idx, nact, nmax, nvmax, F, out, ws, fail, orb_off, evals, lds, s);
+  else
+    launch_variant<32, 6, 12, EVEC>(L, idx, nact, nmax, nvmax, F, out, ws, fail, orb_off, evals, lds, s);
 }
 
 extern "C" int pyseqm_eigh_density_lds_ok(int norb, int nocc) {
@@ -658,7 +689,7 @@ extern "C" int pyseqm_eigh_density(const pyseqm_layout_t* L, const int* idx, int
   if (nact == 0) return PYSEQM_OK;
   const size_t lds = (size_t)lds_doubles(nmax, nocc_max) * sizeof(double);
   hipStream_t s = (hipStream_t)stream;
-  launch_eigh<false>(L, idx, nact, nmax, F, P, ws, fail, nullptr, nullptr, lds, s);
+  launch_eigh<false>(L, idx, nact, nmax, nocc_max, F, P, ws, fail, nullptr, nullptr, lds, s);
   return hipGetLastError() == hipSuccess ? PYSEQM_OK : PYSEQM_ELAUNCH;
 }
 
@@ -669,6 +700,6 @@ extern "C" int pyseqm_eigh_vectors(const pyseqm_layout_t* L, const int* idx, int
   if (!pyseqm_eigh_density_lds_ok(nmax, nmax)) return PYSEQM_EINVAL;
   if (nact == 0) return PYSEQM_OK;
   const size_t lds = (size_t)lds_doubles(nmax, nmax) * sizeof(double);
-  launch_eigh<true>(L, idx, nact, nmax, F, Q, ws, fail, orb_off, evals, lds, (hipStream_t)stream);
+  launch_eigh<true>(L, idx, nact, nmax, nmax, F, Q, ws, fail, orb_off, evals, lds, (hipStream_t)stream);
   return hipGetLastError() == hipSuccess ? PYSEQM_OK : PYSEQM_ELAUNCH;
 }
