@@ -73,27 +73,44 @@ __device__ inline double rcp_f64(double q) {
   return r;
 }
 
-// number of eigenvalues of the tridiagonal (d, e) below x: the negative pivots of the LDL^T
-// factorisation of T - x I (e2[k] = e[k]^2)
+// number of eigenvalues of the tridiagonal (d, e) below x: the negative pivots q_k of the LDL^T
+// factorisation of T - x I (e2[k] = e[k]^2), counted as sign changes of the Sturm sequence
+// p_0 = 1, p_k = (d_k - x) p_{k-1} - e2_{k-1} p_{k-2} (q_k = p_k / p_{k-1}), whose dependent chain
+// is one FMA per step instead of a reciprocal.  |q_k| < pivmin is floored to q_k = -pivmin, as in
+// the ratio form: |p_k| < pivmin |p_{k-1}| -> p_k = -pivmin p_{k-1}.  Every 8 steps both terms are
+// scaled by the same power of two (exact; signs and ratios unchanged), which keeps them in range.
+__device__ inline void sturm_step(double dk, double e2k, double x, double pivmin, double& p0, double& p1, int& cnt) {
+  double p2 = fma(dk - x, p1, -(e2k * p0));
+  if (fabs(p2) < pivmin * fabs(p1)) p2 = -pivmin * p1;
+  cnt += (__double_as_longlong(p2) ^ __double_as_longlong(p1)) < 0;  // sign bits differ
+  p0 = p1;
+  p1 = p2;
+}
 __device__ inline int sturm_count(const double* d, const double* e2, int n, double x, double pivmin) {
-  int cnt = 0;
-  double q = d[0] - x;
-  if (fabs(q) < pivmin) q = -pivmin;
-  cnt += q < 0.0;
-  for (int k = 1; k < n; ++k) {
-    q = (d[k] - x) - e2[k - 1] * rcp_f64(q);
-    if (fabs(q) < pivmin) q = -pivmin;
-    cnt += q < 0.0;
+  double p0 = 1.0, p1 = d[0] - x;
+  if (fabs(p1) < pivmin) p1 = -pivmin;
+  int cnt = p1 < 0.0;
+  int k = 1;
+  for (; k + 8 <= n; k += 8) {
+#pragma unroll
+    for (int t = 0; t < 8; ++t) sturm_step(d[k + t], e2[k + t - 1], x, pivmin, p0, p1, cnt);
+    int ex;
+    frexp(fmax(fabs(p0), fabs(p1)), &ex);
+    p0 = ldexp(p0, -ex);
+    p1 = ldexp(p1, -ex);
   }
+  for (; k < n; ++k) sturm_step(d[k], e2[k - 1], x, pivmin, p0, p1, cnt);
   return cnt;
 }
 
 // One DPP lane permutation of a double (two 32-bit DPP moves; VALU, no LDS round trip).
+// Every control used here (quad permutations, row mirrors) names a source lane inside the row for
+// every lane, so no lane keeps an old value and none needs initialising.
 template <int CTRL>
 __device__ inline double dpp_f64(double v) {
   const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+  const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, false);
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
@@ -340,6 +357,7 @@ __global__ __launch_bounds__(GR* GC) void k_eigh_density(pyseqm_layout_t L, cons
     double q = 0.0;
 #pragma unroll
     for (int a = 0; a < RR; ++a) {
+      if (GR * a + GR - 1 <= i) continue;  // every row of the slot finished (uniform)
       const double pa = tau * row16_sum(s[a]);
       const int r = tr + GR * a;
       q = fma(pa, vr[a], q);  // vr = 0 off the live rows
@@ -360,8 +378,8 @@ __global__ __launch_bounds__(GR* GC) void k_eigh_density(pyseqm_layout_t L, cons
     }
     // A -= v w^T + w v^T on the live column slots, every row (v = w = 0 at finished rows and rows
     // beyond the block leave them exactly unchanged).  Column i + 1 (slot bn, a uniform index) is
-    // copied out as it is updated and handed to the next step by its owners (tc == tn): the
-    // reflector input x, its partial squared norm and the diagonal element.
+    // then copied out and handed to the next step by its owners (tc == tn): the reflector input x,
+    // its partial squared norm and the diagonal element.
     const int bn = (i + 1) / GC, tn = (i + 1) - bn * GC;
     double coln[RR];
 #pragma unroll
@@ -385,12 +403,14 @@ __global__ __launch_bounds__(GR* GC) void k_eigh_density(pyseqm_layout_t L, cons
         const double wb = (c > i && c < n) ? fma(a2, vb, wc[j]) : 0.0;
 #pragma unroll
         for (int a = 0; a < RR; ++a) Ar[a][b] = fma(-vr[a], wb, fma(-wr[a], vb, Ar[a][b]));
-        if (b == bn) {
-#pragma unroll
-          for (int a = 0; a < RR; ++a) coln[a] = Ar[a][b];
-        }
       }
     }
+#pragma unroll
+    for (int b = 0; b < RC; ++b)
+      if (b == bn) {  // uniform
+#pragma unroll
+        for (int a = 0; a < RR; ++a) coln[a] = Ar[a][b];
+      }
     double partn = 0.0;
     if (tc == tn) {
 #pragma unroll

@@ -200,3 +200,60 @@ def test_trained_parameters_selects_per_atom_tensors_that_ask_for_gradients():
         assert _trained_parameters({"g_ss": a}) == []
     with pytest.raises(NotImplementedError, match="Kbeta"):
         _trained_parameters({"Kbeta": torch.ones(2, 2, requires_grad=True)})
+
+
+def _sturm_ratio(d, e2, x, pivmin):
+    q = d[0] - x
+    q = -pivmin if abs(q) < pivmin else q
+    cnt = int(q < 0.0)
+    for k in range(1, len(d)):
+        q = (d[k] - x) - e2[k - 1] / q
+        q = -pivmin if abs(q) < pivmin else q
+        cnt += int(q < 0.0)
+    return cnt
+
+
+def _sturm_sequence(d, e2, x, pivmin):
+    """The three-term form the workgroup eigensolver's multisection uses (eigh.hip sturm_count)."""
+    import math
+    p0, p1 = 1.0, d[0] - x
+    p1 = -pivmin if abs(p1) < pivmin else p1
+    cnt = int(p1 < 0.0)
+    for k in range(1, len(d)):
+        p2 = (d[k] - x) * p1 - e2[k - 1] * p0
+        if abs(p2) < pivmin * abs(p1):
+            p2 = -pivmin * p1
+        cnt += int(math.copysign(1.0, p2) != math.copysign(1.0, p1))
+        p0, p1 = p1, p2
+        if k % 8 == 0:
+            ex = math.frexp(max(abs(p0), abs(p1)))[1]
+            p0, p1 = math.ldexp(p0, -ex), math.ldexp(p1, -ex)
+    return cnt
+
+
+def test_sturm_sequence_counts_equal_the_ratio_form_and_the_spectrum():
+    """Sign changes of p_k = (d_k - x) p_{k-1} - e_{k-1}^2 p_{k-2} (rescaled by powers of two) count
+    the negative LDL^T pivots of T - x I, i.e. the eigenvalues below x: decoupled blocks, equal
+    diagonals, shifts on diagonal elements and tiny couplings included."""
+    rng = np.random.default_rng(5)
+    for trial in range(60):
+        n = int(rng.integers(1, 80))
+        d = rng.uniform(-50, 50, n)
+        e = rng.uniform(-10, 10, n)
+        mode = trial % 4
+        if mode == 1:
+            e[rng.random(n) < 0.25] = 0.0
+        elif mode == 2:
+            d[:] = 3.0
+        elif mode == 3:
+            e[rng.random(n) < 0.3] *= 1e-9
+        e2 = e[:-1] ** 2
+        pivmin = 1e-300 * max(1.0, float(e2.max()) if n > 1 else 1.0)
+        T = np.diag(d) + np.diag(e[:-1], 1) + np.diag(e[:-1], -1)
+        lam = np.linalg.eigvalsh(T)
+        for x in list(rng.uniform(-80, 80, 6)) + [float(d[0]), 3.0]:
+            a = _sturm_ratio(d, e2, x, pivmin)
+            b = _sturm_sequence(d, e2, x, pivmin)
+            assert a == b, (trial, x, a, b)
+            if np.min(np.abs(lam - x)) > 1e-9 * max(1.0, np.abs(lam).max()):
+                assert b == int((lam < x).sum()), (trial, x, b)
