@@ -489,35 +489,36 @@ __global__ __launch_bounds__(GR* GC) void k_eigh_density(pyseqm_layout_t L, cons
     const double sig = lam[j];
     for (int rep = 0; rep < 3; ++rep) {
       // LU of T - sig I with partial pivoting, the right-hand side eliminated alongside; the
-      // running row and right-hand side stay in registers, U goes to the workspace
+      // running row and right-hand side stay in registers, U goes to the workspace.  Each step
+      // keeps one of two rows as the pivot row (its diagonal and sub-diagonal entries compared)
+      // and eliminates the other, both cases as one select-based update (lanes differ in their
+      // choices); the next step's d, e and right-hand side are read from LDS one step ahead.
       double diag = d[0] - sig, sup = n > 1 ? e[0] : 0.0;
       double yk = Z[j];
+      double sub_n = n > 1 ? e[0] : 0.0, dn_n = n > 1 ? d[1] : 0.0, ns_n = n > 2 ? e[1] : 0.0;
+      double y_n = n > 1 ? Z[nv + j] : 0.0;
       for (int k = 0; k + 1 < n; ++k) {
-        const double sub = e[k], nd = d[k + 1] - sig, ns = k + 2 < n ? e[k + 1] : 0.0;
-        const double yk1 = Z[(k + 1) * nv + j];
-        const size_t o = (size_t)k * nmax + j;
-        if (fabs(diag) >= fabs(sub)) {
-          const double piv = fabs(diag) < tiny ? copysign(tiny, diag) : diag;
-          const double rp = rcp_f64(piv);
-          const double mlt = sub * rp;
-          U0[o] = rp;
-          U1[o] = sup;
-          U2[o] = 0.0;
-          Z[k * nv + j] = yk;
-          yk = yk1 - mlt * yk;
-          diag = nd - mlt * sup;
-          sup = ns;
-        } else {
-          const double rp = rcp_f64(sub);
-          const double mlt = diag * rp;
-          U0[o] = rp;
-          U1[o] = nd;
-          U2[o] = ns;
-          Z[k * nv + j] = yk1;
-          yk = yk - mlt * yk1;
-          diag = sup - mlt * nd;
-          sup = -mlt * ns;
+        const double sub = sub_n, nd = dn_n - sig, ns = ns_n, yk1 = y_n;
+        if (k + 2 < n) {
+          sub_n = ns_n;
+          dn_n = d[k + 2];
+          ns_n = k + 3 < n ? e[k + 2] : 0.0;
+          y_n = Z[(k + 2) * nv + j];
         }
+        const size_t o = (size_t)k * nmax + j;
+        const bool keep = fabs(diag) >= fabs(sub);  // no row exchange
+        const double piv = keep ? (fabs(diag) < tiny ? copysign(tiny, diag) : diag) : sub;
+        const double rp = rcp_f64(piv);
+        const double mlt = (keep ? sub : diag) * rp;
+        const double p1 = keep ? sup : nd, p2 = keep ? 0.0 : ns, py = keep ? yk : yk1;
+        const double o1 = keep ? nd : sup, o2 = keep ? ns : 0.0, oy = keep ? yk1 : yk;
+        U0[o] = rp;
+        U1[o] = p1;
+        U2[o] = p2;
+        Z[k * nv + j] = py;
+        yk = fma(-mlt, py, oy);
+        diag = fma(-mlt, p1, o1);
+        sup = fma(-mlt, p2, o2);
       }
       const double rlast = 1.0 / (fabs(diag) < tiny ? copysign(tiny, diag) : diag);
       double x1 = yk * rlast, x2 = 0.0, xmax = fabs(x1);
@@ -627,38 +628,52 @@ __global__ __launch_bounds__(GR* GC) void k_eigh_density(pyseqm_layout_t L, cons
     for (int j = tid; j < n; j += NT) evals[orb_off[m] + j] = lam[j];
     return;
   }
-  // ---- P = 2 C C^T, 2 x 2 output blocks per thread over the upper triangle, both triangles written
+  // ---- P = 2 C C^T, 4 x 4 output blocks per thread over the upper triangle of blocks (tasks
+  //      enumerated over the triangle only), both triangles written; every element sums over the
+  //      vectors in order, and the two copies of a diagonal block's element are the same bits
   double* Pm = P + mo;
-  const int nb2 = (n + 1) / 2;
-  for (int t = tid; t < nb2 * nb2; t += NT) {
-    const int bi = t / nb2, bj = t % nb2;
-    if (bj < bi) continue;
-    const int r0 = 2 * bi, c0 = 2 * bj;
-    const int r1 = min(r0 + 1, n - 1), c1 = min(c0 + 1, n - 1);
-    double s00 = 0.0, s01 = 0.0, s10 = 0.0, s11 = 0.0;
+  const int nb4 = (n + 3) / 4;
+  const int ntask = nb4 * (nb4 + 1) / 2;
+  for (int t = tid; t < ntask; t += NT) {
+    int bi = 0, rem = t;
+    while (rem >= nb4 - bi) {
+      rem -= nb4 - bi;
+      ++bi;
+    }
+    const int bj = bi + rem;
+    int ro[4], co[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      ro[q] = min(4 * bi + q, n - 1) * nv;
+      co[q] = min(4 * bj + q, n - 1) * nv;
+    }
+    double acc[4][4];
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) acc[p][q] = 0.0;
     for (int a = 0; a < nv; ++a) {
-      const double x0 = C[(size_t)r0 * nv + a], x1 = C[(size_t)r1 * nv + a];
-      const double y0 = C[(size_t)c0 * nv + a], y1 = C[(size_t)c1 * nv + a];
-      s00 += x0 * y0;
-      s01 += x0 * y1;
-      s10 += x1 * y0;
-      s11 += x1 * y1;
+      double x[4], y[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        x[q] = C[ro[q] + a];
+        y[q] = C[co[q] + a];
+      }
+#pragma unroll
+      for (int p = 0; p < 4; ++p)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[p][q] += x[p] * y[q];
     }
-    const bool hr = r0 + 1 < n, hc = c0 + 1 < n;
-    Pm[(size_t)r0 * n + c0] = 2.0 * s00;
-    Pm[(size_t)c0 * n + r0] = 2.0 * s00;
-    if (hc) {
-      Pm[(size_t)r0 * n + c0 + 1] = 2.0 * s01;
-      Pm[(size_t)(c0 + 1) * n + r0] = 2.0 * s01;
-    }
-    if (hr) {
-      Pm[(size_t)(r0 + 1) * n + c0] = 2.0 * s10;
-      Pm[(size_t)c0 * n + r0 + 1] = 2.0 * s10;
-    }
-    if (hr && hc) {
-      Pm[(size_t)(r0 + 1) * n + c0 + 1] = 2.0 * s11;
-      Pm[(size_t)(c0 + 1) * n + r0 + 1] = 2.0 * s11;
-    }
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int r = 4 * bi + p, c = 4 * bj + q;
+        if (r < n && c < n) {
+          Pm[(size_t)r * n + c] = 2.0 * acc[p][q];
+          Pm[(size_t)c * n + r] = 2.0 * acc[p][q];
+        }
+      }
   }
 }
 
