@@ -365,9 +365,16 @@ __global__ __launch_bounds__(GR* GC) void k_eigh_density(pyseqm_layout_t L, cons
     }
     if (tc == 0) red2[tr] = q;
     __syncthreads();
-    double wr[RR];
+    // p at this thread's rows and at the first chunk of its columns are read with the partials;
+    // each later chunk's columns are read while the chunk before it is applied
+    constexpr int kUpdChunk = 4;
+    constexpr int kChunks = (RC + kUpdChunk - 1) / kUpdChunk;
+    double wr[RR], wc[2][kUpdChunk];
 #pragma unroll
     for (int a = 0; a < RR; ++a) wr[a] = pp[tr + GR * a];
+#pragma unroll
+    for (int j = 0; j < kUpdChunk; ++j)
+      if (j < RC) wc[0][j] = pp[tc + GC * j];
     const double r2 = red2[lane32];
     const double pv = sum32(lane32 < GR ? r2 : 0.0);
     const double a2 = -0.5 * tau * pv;
@@ -384,15 +391,15 @@ __global__ __launch_bounds__(GR* GC) void k_eigh_density(pyseqm_layout_t L, cons
     double coln[RR];
 #pragma unroll
     for (int a = 0; a < RR; ++a) coln[a] = 0.0;
-    // columns in chunks of kUpdChunk slots: the chunk's p values are read together, then applied
-    constexpr int kUpdChunk = 4;
 #pragma unroll
-    for (int b0 = 0; b0 < RC; b0 += kUpdChunk) {
+    for (int ch = 0; ch < kChunks; ++ch) {
+      const int b0 = ch * kUpdChunk;
+      if (ch + 1 < kChunks) {
+#pragma unroll
+        for (int j = 0; j < kUpdChunk; ++j)
+          if (b0 + kUpdChunk + j < RC) wc[(ch + 1) & 1][j] = pp[tc + GC * (b0 + kUpdChunk + j)];
+      }
       if (GC * min(b0 + kUpdChunk, RC) - 1 <= i || GC * b0 >= n) continue;  // whole chunk idle (uniform)
-      double wc[kUpdChunk];
-#pragma unroll
-      for (int j = 0; j < kUpdChunk; ++j)
-        if (b0 + j < RC) wc[j] = pp[tc + GC * (b0 + j)];
 #pragma unroll
       for (int j = 0; j < kUpdChunk; ++j) {
         const int b = b0 + j;
@@ -400,7 +407,7 @@ __global__ __launch_bounds__(GR* GC) void k_eigh_density(pyseqm_layout_t L, cons
         if (GC * b + GC - 1 <= i || GC * b >= n) continue;
         const int c = tc + GC * b;
         const double vb = vc[b];
-        const double wb = (c > i && c < n) ? fma(a2, vb, wc[j]) : 0.0;
+        const double wb = (c > i && c < n) ? fma(a2, vb, wc[ch & 1][j]) : 0.0;
 #pragma unroll
         for (int a = 0; a < RR; ++a) Ar[a][b] = fma(-vr[a], wb, fma(-wr[a], vb, Ar[a][b]));
       }
