@@ -33,8 +33,13 @@ constexpr int kRed = 128;           // doubles of reduction scratch (two halves 
 //   [0, nmax^2)          reflectors: row i holds v_i (v_i[i+1] = 1) and tau_i at [i][i]
 //   [nmax^2, 4 nmax^2)   the U factor of each inverse-iteration LU (reciprocal diagonal and two
 //                        superdiagonals), element k of vector j at k * nmax + j
-// LDS: eigenvectors n x nocc, then d, e, eigenvalues, two reflector buffers, p (n each), scratch
-__host__ __device__ inline int lds_doubles(int n, int nocc) { return n * nocc + 6 * n + kRed; }
+// LDS: eigenvectors n x nocc, then d, e, eigenvalues (n each), two reflector buffers and p (S each,
+// S = the variant's 48 / 96 / 192 register-grid rows, zero-padded past n), kRed of scratch.  The
+// variant, and so S, follows the launch's largest block nmax.
+__host__ __device__ inline int grid_slots(int nmax) { return nmax <= 48 ? 48 : (nmax <= 96 ? 96 : 192); }
+__host__ __device__ inline int lds_doubles(int nmax, int nocc) {
+  return nmax * nocc + 3 * nmax + 3 * grid_slots(nmax) + kRed;
+}
 
 template <int NT>
 __device__ inline double eig_block_max(double v, double* red) {
@@ -253,10 +258,12 @@ __global__ __launch_bounds__(GR* GC) void k_eigh_density(pyseqm_layout_t L, cons
   double* d = Z + n * nv;
   double* e = d + n;
   double* lam = e + n;
-  double* xs0 = lam + n;  // reflector columns of even / odd steps
-  double* xs1 = xs0 + n;
-  double* pp = xs1 + n;
-  double* red = pp + n;  // kRed
+  constexpr int S = GR * RR;  // = GC * RC: register-grid rows / columns
+  static_assert(GR * RR == GC * RC, "square register grid");
+  double* xs0 = lam + n;  // reflector columns of even / odd steps, zero at indices <= i + 1 and >= n
+  double* xs1 = xs0 + S;
+  double* pp = xs1 + S;
+  double* red = pp + S;  // kRed
   double* red1 = red;  // GR <= 64 partials each
   double* red2 = red + kRed / 2;
   const double* Fm = F + mo;
@@ -278,8 +285,11 @@ __global__ __launch_bounds__(GR* GC) void k_eigh_density(pyseqm_layout_t L, cons
       const int r = tr + GR * a, c = tc + GC * b;
       Ar[a][b] = (r < n && c < n) ? Fm[(size_t)r * n + c] : 0.0;
     }
-  // column 0 (the owners of column 0, tc == 0, slot b = 0); later columns are written by the
-  // rank-2 update of the step before, which computes them
+  // the reflector buffers and p start at zero (their entries past n stay zero), then column 0
+  // (the owners of column 0, tc == 0, slot b = 0); later columns are written by the rank-2
+  // update of the step before, which computes them
+  for (int t = tid; t < 3 * S; t += NT) xs0[t] = 0.0;
+  __syncthreads();
   if (tc == 0) {
     double part = 0.0;
 #pragma unroll
@@ -301,11 +311,7 @@ __global__ __launch_bounds__(GR* GC) void k_eigh_density(pyseqm_layout_t L, cons
     double* xnext = (i & 1) ? xs0 : xs1;
     __syncthreads();
     // every LDS read of the step's first half is issued here, unconditionally, so that they travel
-    // together instead of one round trip per slot.  Indices run to GC RC - 1 = GR RR - 1, past this
-    // block's n: the reads land in the arrays after x and p (the other reflector buffer, p, kRed
-    // doubles of scratch), and the LDS is sized for the launch's largest block nmax, which picks
-    // the variant (192, 96, 48 slots for nmax > 96, > 48, >= 1), so p + GC RC - 1 <= (the block's
-    // offset of p) + nmax + kRed - 1 stays inside it.  Values at indices >= n are masked out below.
+    // together instead of one round trip per slot (x holds S entries, zero past the live range)
     double xc[RC], xr[RR];
 #pragma unroll
     for (int b = 0; b < RC; ++b) xc[b] = x[tc + GC * b];
@@ -328,13 +334,13 @@ __global__ __launch_bounds__(GR* GC) void k_eigh_density(pyseqm_layout_t L, cons
       e[i] = beta;
       HH[(size_t)i * nmax + i] = tau;
     }
-    // v at this thread's columns and rows (0 at indices <= i and beyond the block)
-    auto vval = [&](int c, double xv) { return (c <= i || c >= n) ? 0.0 : (c == i + 1 ? 1.0 : xv * scal); };
+    // v at this thread's columns and rows: 1 at i + 1, else x scal (x is zero at indices <= i and
+    // >= n, so no range test is needed)
     double vc[RC], vr[RR];
 #pragma unroll
-    for (int b = 0; b < RC; ++b) vc[b] = vval(tc + GC * b, xc[b]);
+    for (int b = 0; b < RC; ++b) vc[b] = (tc + GC * b == i + 1) ? 1.0 : xc[b] * scal;
 #pragma unroll
-    for (int a = 0; a < RR; ++a) vr[a] = vval(tr + GR * a, xr[a]);
+    for (int a = 0; a < RR; ++a) vr[a] = (tr + GR * a == i + 1) ? 1.0 : xr[a] * scal;
     if (tr == 0) {
 #pragma unroll
       for (int b = 0; b < RC; ++b) {
@@ -424,7 +430,7 @@ __global__ __launch_bounds__(GR* GC) void k_eigh_density(pyseqm_layout_t L, cons
       for (int a = 0; a < RR; ++a) {
         const int r = tr + GR * a;
         if (r == i + 1) d[i + 1] = coln[a];
-        if (r > i + 1 && r < n) xnext[r] = coln[a];
+        xnext[r] = (r > i + 1 && r < n) ? coln[a] : 0.0;  // every row: zero off the live range
         if (r > i + 2 && r < n) partn = fma(coln[a], coln[a], partn);
       }
     }

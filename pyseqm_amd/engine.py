@@ -47,13 +47,22 @@ EIGH_WG_LDS_DOUBLES = 20480
 EIGH_WG_CLASSES = (48, 96, 192)
 
 
+def eigh_wg_lds(norb, nvec):
+    """LDS doubles of the workgroup eigensolver for a launch whose largest block has norb orbitals
+    and nvec vectors: the norb x nvec eigenvectors, 3 norb (diagonal, off-diagonal, eigenvalues),
+    three buffers of the variant's 48 / 96 / 192 grid rows, 128 of scratch (eigh.hip lds_doubles)."""
+    n = np.asarray(norb, dtype=np.int64)
+    o = np.asarray(nvec, dtype=np.int64)
+    slots = np.where(n <= 48, 48, np.where(n <= 96, 96, 192))
+    return n * o + 3 * n + 3 * slots + 128
+
+
 def eigh_wg_ok(norb, nocc):
     """Whether the workgroup eigensolver takes blocks of norb orbitals / nocc occupied orbitals:
-    n <= 192 and the n x nocc eigenvectors plus 6 n + 128 doubles fit 160 KiB of LDS
-    (pyseqm_eigh_density_lds_ok)."""
+    n <= 192 and its LDS footprint (eigh_wg_lds) fits 160 KiB (pyseqm_eigh_density_lds_ok)."""
     n = np.asarray(norb, dtype=np.int64)
     o = np.asarray(nocc, dtype=np.int64)
-    return (n >= 1) & (n <= EIGH_WG_MAX_N) & (o <= n) & (n * o + 6 * n + 128 <= EIGH_WG_LDS_DOUBLES)
+    return (n >= 1) & (n <= EIGH_WG_MAX_N) & (o <= n) & (eigh_wg_lds(n, o) <= EIGH_WG_LDS_DOUBLES)
 
 
 def size_groups(norb, ratio=0.7, min_group=32, floor=EIGH_MIN_N):
@@ -490,7 +499,7 @@ class SCFEngine:
         vectors the workgroup solver lost (read by the caller)."""
         b = self.b
         norb = np.asarray(b.norb)
-        wg = (norb <= EIGH_WG_MAX_N) & (norb * norb + 6 * norb + 128 <= EIGH_WG_LDS_DOUBLES)
+        wg = eigh_wg_ok(norb, norb)
         ids = np.nonzero(wg)[0]
         if len(ids):
             idx = torch.as_tensor(ids.astype(np.int32), device=self.device)

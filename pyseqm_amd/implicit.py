@@ -21,6 +21,7 @@ import numpy as np
 import torch
 
 from . import _native as nat
+from .engine import eigh_wg_ok
 
 
 class ResponseWorkspace:
@@ -39,7 +40,7 @@ class ResponseWorkspace:
         self.orb_off = torch.as_tensor(np.concatenate([[0], np.cumsum(norb)[:-1]]).astype(np.int64), device=dev)
         self.evals = torch.zeros(max(int(norb.sum()), 1), **f64)
         self.Q = torch.zeros(total, **f64)
-        wg = norb[norb * norb + 6 * norb + 128 <= 20480]
+        wg = norb[eigh_wg_ok(norb, norb)]
         self.eig_ws = torch.zeros(max(4 * b.nmol * int(wg.max()) ** 2 if len(wg) else 1, 1), **f64)
         self.fail = torch.zeros(1, dtype=torch.int32, device=dev)
         self.Hzero = torch.zeros(total, **f64)

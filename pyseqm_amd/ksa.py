@@ -20,6 +20,7 @@ import numpy as np
 import torch
 
 from . import _native as nat
+from .engine import eigh_wg_ok
 
 KB_EV = 8.61739e-5      # Boltzmann constant, eV/K (xlbomd.py:214)
 CANON_DM_PRT_ITER = 10  # recursion steps of Canon_DM_PRT in EnergyXL (xlbomd.py:55)
@@ -52,7 +53,7 @@ class KSAWorkspace:
         self.S = torch.zeros(b.nmol, **f64)
         self.Q = torch.zeros(total, **f64)
         self.work = torch.zeros(int(nat.lib().pyseqm_fermi_workspace(b.nmol)), dtype=torch.uint8, device=dev)
-        wg = norb[norb * norb + 6 * norb + 128 <= 20480]
+        wg = norb[eigh_wg_ok(norb, norb)]
         self.eig_ws = torch.zeros(max(4 * b.nmol * int(wg.max()) ** 2 if len(wg) else 1, 1), **f64)
         self.fail = torch.zeros(1, dtype=torch.int32, device=dev)
         self.Hzero = torch.zeros(total, **f64)
