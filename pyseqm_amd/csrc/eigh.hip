@@ -360,14 +360,18 @@ __global__ __launch_bounds__(GR* GC) void k_eigh_density(pyseqm_layout_t L, cons
 #pragma unroll
       for (int a = 0; a < RR; ++a) s[a] = fma(Ar[a][b], vc[b], s[a]);
     }
+    // p is written at every row (zero off the live range, finished slots included), so that w
+    // below needs no range test
     double q = 0.0;
 #pragma unroll
     for (int a = 0; a < RR; ++a) {
-      if (GR * a + GR - 1 <= i) continue;  // every row of the slot finished (uniform)
-      const double pa = tau * row16_sum(s[a]);
+      double pa = 0.0;
+      if (GR * a + GR - 1 > i) {  // some row of the slot still live (uniform)
+        pa = tau * row16_sum(s[a]);
+        q = fma(pa, vr[a], q);  // vr = 0 off the live rows
+      }
       const int r = tr + GR * a;
-      q = fma(pa, vr[a], q);  // vr = 0 off the live rows
-      if (tc == 0 && r > i && r < n) pp[r] = pa;
+      if (tc == 0) pp[r] = (r > i && r < n) ? pa : 0.0;
     }
     if (tc == 0) red2[tr] = q;
     __syncthreads();
@@ -385,10 +389,7 @@ __global__ __launch_bounds__(GR* GC) void k_eigh_density(pyseqm_layout_t L, cons
     const double pv = sum32(lane32 < GR ? r2 : 0.0);
     const double a2 = -0.5 * tau * pv;
 #pragma unroll
-    for (int a = 0; a < RR; ++a) {
-      const int r = tr + GR * a;
-      wr[a] = (r > i && r < n) ? fma(a2, vr[a], wr[a]) : 0.0;
-    }
+    for (int a = 0; a < RR; ++a) wr[a] = fma(a2, vr[a], wr[a]);  // p = v = 0 off the live rows
     // A -= v w^T + w v^T on the live column slots, every row (v = w = 0 at finished rows and rows
     // beyond the block leave them exactly unchanged).  Column i + 1 (slot bn, a uniform index) is
     // then copied out and handed to the next step by its owners (tc == tn): the reflector input x,
@@ -411,9 +412,8 @@ __global__ __launch_bounds__(GR* GC) void k_eigh_density(pyseqm_layout_t L, cons
         const int b = b0 + j;
         if (b >= RC) break;
         if (GC * b + GC - 1 <= i || GC * b >= n) continue;
-        const int c = tc + GC * b;
         const double vb = vc[b];
-        const double wb = (c > i && c < n) ? fma(a2, vb, wc[ch & 1][j]) : 0.0;
+        const double wb = fma(a2, vb, wc[ch & 1][j]);  // p = v = 0 off the live columns
 #pragma unroll
         for (int a = 0; a < RR; ++a) Ar[a][b] = fma(-vr[a], wb, fma(-wr[a], vb, Ar[a][b]));
       }
