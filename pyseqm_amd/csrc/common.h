@@ -55,6 +55,52 @@ __device__ inline int tile_wstride(const Tile& t) {
   return (nv + 15) & ~15;
 }
 
+// One DPP lane permutation of a double (two 32-bit moves).  Every control used with it (quad
+// permutations, half-row mirror, 16-lane rotation by 8) names a source lane inside the row for
+// every lane, so no lane keeps an old value.
+template <int CTRL>
+__device__ inline double dpp_d(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+// v + the value of the lane 16 (32) lanes away, with v_permlane16_swap (v_permlane32_swap): the
+// swap leaves the even-row (lower-half) value in one register and the odd-row (upper-half) value
+// in the other in both partner lanes, so both add the same two numbers in the same order.
+__device__ inline double swap16_sum(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const auto lo = __builtin_amdgcn_permlane16_swap((unsigned)b, (unsigned)b, false, false);
+  const auto hi = __builtin_amdgcn_permlane16_swap((unsigned)(b >> 32), (unsigned)(b >> 32), false, false);
+  const double x = __longlong_as_double((long long)(((unsigned long long)hi[0] << 32) | lo[0]));
+  const double y = __longlong_as_double((long long)(((unsigned long long)hi[1] << 32) | lo[1]));
+  return x + y;
+}
+__device__ inline double swap32_sum(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)b, (unsigned)b, false, false);
+  const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)(b >> 32), (unsigned)(b >> 32), false, false);
+  const double x = __longlong_as_double((long long)(((unsigned long long)hi[0] << 32) | lo[0]));
+  const double y = __longlong_as_double((long long)(((unsigned long long)hi[1] << 32) | lo[1]));
+  return x + y;
+}
+// Sum over the 8 lanes of a tile row (lanes 8r..8r+7): xor 1, xor 2 (quad permutations), then the
+// other quad (half-row mirror).  Sum over the 8 lanes of a tile column (c, c+8, ..., c+56): xor 8
+// (rotate a 16-lane row by 8), xor 16, xor 32.  Every lane of the row (column) ends with the same
+// bits.  All 64 lanes must be active.
+__device__ inline double row8(double v) {
+  v += dpp_d<0xB1>(v);
+  v += dpp_d<0x4E>(v);
+  v += dpp_d<0x141>(v);
+  return v;
+}
+__device__ inline double col8(double v) {
+  v += dpp_d<0x128>(v);
+  v = swap16_sum(v);
+  return swap32_sum(v);
+}
+__device__ inline double wave64(double v) { return col8(row8(v)); }
+
 // Sum over the 8 lanes sharing a row r (lanes 8r..8r+7) / a column c (lanes c, c+8, ...).
 __device__ inline double row_sum(double v) {
   v += __shfl_xor(v, 1);
@@ -133,15 +179,20 @@ __device__ inline void write_partials_c(double* part, const pyseqm_layout_t& L, 
   const size_t b = cpart_tile(L, tile, nr, nc);  // (nr, nc) == (NR, NC) for the tile's kind
   double* prow = part + b + (size_t)r * NR;
   double* pcol = part + b + (size_t)8 * NR + (size_t)c * NC;
+  // DPP / permlane sums (no LDS round trips; the values' chains interleave), the same pairings
+  // and so the same bits as row_sum / col_sum
+  double rs[NR], cs[NC];
 #pragma unroll
-  for (int k = 0; k < NR; ++k) {
-    const double rs = row_sum(rowv[k]);
-    if (c == 0) prow[k] = rs;
+  for (int k = 0; k < NR; ++k) rs[k] = row8(rowv[k]);
+#pragma unroll
+  for (int k = 0; k < NC; ++k) cs[k] = col8(colv[k]);
+  if (c == 0) {
+#pragma unroll
+    for (int k = 0; k < NR; ++k) prow[k] = rs[k];
   }
+  if (r == 0) {
 #pragma unroll
-  for (int k = 0; k < NC; ++k) {
-    const double cs = col_sum(colv[k]);
-    if (r == 0) pcol[k] = cs;
+    for (int k = 0; k < NC; ++k) pcol[k] = cs[k];
   }
 }
 

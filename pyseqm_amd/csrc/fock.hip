@@ -364,49 +364,7 @@ __global__ __launch_bounds__(256) void k_fock_diag(pyseqm_layout_t L, const doub
 // (16 benzenes 10 us vs 23 us; (H2O)_8 13 vs 26 us) and slower from ~30 atoms on (alkanes of 2-30
 // carbons 171-181 us vs 157 us; (H2O)_40 108-115 vs 72 us), hence the PYSEQM_FOCK_MOL_ATOMS rule.
 
-// 64-bit DPP move (two 32-bit halves).
-template <int CTRL>
-__device__ inline double dpp_d(double v) {
-  const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-// v + the value of the lane 16 (32) lanes away, with v_permlane16_swap (v_permlane32_swap): the
-// swap leaves the even-row (lower-half) value in one register and the odd-row (upper-half) value
-// in the other in both partner lanes, so both add the same two numbers in the same order.
-__device__ inline double swap16_sum(double v) {
-  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
-  const auto lo = __builtin_amdgcn_permlane16_swap((unsigned)b, (unsigned)b, false, false);
-  const auto hi = __builtin_amdgcn_permlane16_swap((unsigned)(b >> 32), (unsigned)(b >> 32), false, false);
-  const double x = __longlong_as_double((long long)(((unsigned long long)hi[0] << 32) | lo[0]));
-  const double y = __longlong_as_double((long long)(((unsigned long long)hi[1] << 32) | lo[1]));
-  return x + y;
-}
-__device__ inline double swap32_sum(double v) {
-  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
-  const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)b, (unsigned)b, false, false);
-  const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)(b >> 32), (unsigned)(b >> 32), false, false);
-  const double x = __longlong_as_double((long long)(((unsigned long long)hi[0] << 32) | lo[0]));
-  const double y = __longlong_as_double((long long)(((unsigned long long)hi[1] << 32) | lo[1]));
-  return x + y;
-}
-// Sum over the 8 lanes of a tile row (lanes 8r..8r+7): xor 1, xor 2 (quad permutations), then the
-// other quad (half-row mirror).  Sum over the 8 lanes of a tile column (c, c+8, ..., c+56): xor 8
-// (rotate a 16-lane row by 8), xor 16, xor 32.  Every lane of the row (column) ends with the same
-// bits.  All 64 lanes must be active.
-__device__ inline double row8(double v) {
-  v += dpp_d<0xB1>(v);
-  v += dpp_d<0x4E>(v);
-  v += dpp_d<0x141>(v);
-  return v;
-}
-__device__ inline double col8(double v) {
-  v += dpp_d<0x128>(v);
-  v = swap16_sum(v);
-  return swap32_sum(v);
-}
-__device__ inline double wave64(double v) { return col8(row8(v)); }
+// (the DPP and permlane-swap sums row8 / col8 / wave64 are in common.h)
 
 // Element k (0..9) of a per-lane 10-vector, k varying over lanes, without dynamic register indexing.
 __device__ inline double pick10(const double (&v)[10], int k) {
