@@ -197,11 +197,22 @@ __device__ inline void write_partials_c(double* part, const pyseqm_layout_t& L, 
 }
 
 // Sum over the G lanes of an aligned lane group (G = 16 or 64), every lane gets the total.
+// G = 16 and 64 run on DPP / permlane swaps (the whole group active, as every caller's exits are
+// uniform over its group): the pairings of the xor tree, so the same bits, with no LDS round trip.
 template <int G>
 __device__ inline double group_sum(double v) {
+  if constexpr (G == 16) {
+    v += dpp_d<0xB1>(v);   // xor 1
+    v += dpp_d<0x4E>(v);   // xor 2
+    v += dpp_d<0x141>(v);  // the other quad of the half-row
+    return v + dpp_d<0x140>(v);  // the other half-row
+  } else if constexpr (G == 64) {
+    return col8(row8(v));
+  } else {
 #pragma unroll
-  for (int o = 1; o < G; o <<= 1) v += __shfl_xor(v, o);
-  return v;
+    for (int o = 1; o < G; o <<= 1) v += __shfl_xor(v, o);
+    return v;
+  }
 }
 
 // Sum of an atom's tile partial slots csr_idx[e0..e1) by a group of G lanes: group lane l adds
