@@ -130,7 +130,7 @@ __global__ __launch_bounds__(64) void k_fock_tiles(pyseqm_layout_t L, int tile_b
     }
   }
   write_partials_c(part, L, tile, lane, rowv, colv);
-  const double es = wave_sum(e);
+  const double es = wave64(e);  // every lane active here
   if (lane == 0) etile[tile] = es;
 }
 
@@ -262,7 +262,7 @@ __global__ __launch_bounds__(128) void k_fock_xx2(pyseqm_layout_t L, const doubl
     }
   }
   write_partials_c(part, L, tile, lane, row10, colv);
-  const double es = wave_sum(e);
+  const double es = wave64(e);  // every lane active here
   if (lane == 0) etile[tile] = es;
 }
 

@@ -589,7 +589,7 @@ __global__ __launch_bounds__(64) void k_pair_tiles(pyseqm_layout_t L, int method
     if (enuc_pair) enuc_pair[pair_ref[(size_t)tile * 64 + lane]] = en;
   }
   write_partials_c(part, L, tile, lane, rowv, colv);
-  const double es = wave_sum(en);
+  const double es = wave64(en);  // every lane active here
   if (lane == 0) enuc_tile[tile] = es;
 }
 
