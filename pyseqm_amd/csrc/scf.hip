@@ -17,7 +17,7 @@ namespace {
 constexpr int kEl = 256;  // threads of the element-wise / reduction kernels
 
 __device__ inline double block_sum(double v, double* sh) {
-  v = wave_sum(v);
+  v = wave64(v);  // DPP / permlane swaps, the pairings of the xor tree (every thread active here)
   const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
   if (lane == 0) sh[wid] = v;
   __syncthreads();
@@ -442,7 +442,7 @@ __global__ __launch_bounds__(64) void k_antisym_final(const int* __restrict__ id
 __global__ __launch_bounds__(kEl) void k_diis_dots(pyseqm_layout_t L, const int* __restrict__ idx, int nb,
                                                    const double* __restrict__ ring, size_t ring_stride, int slot,
                                                    int count, double* __restrict__ red) {
-  __shared__ double sh[kEl / 64];
+  __shared__ double sh[16][kEl / 64];
   const int k = blockIdx.y;
   const int m = idx[k];
   const int n = L.mol_norb[m];
@@ -461,9 +461,20 @@ __global__ __launch_bounds__(kEl) void k_diis_dots(pyseqm_layout_t L, const int*
         if (j < count) acc[j] += av * ring[(size_t)j * ring_stride + off + row + c];
     }
   }
-  for (int j = 0; j < count; ++j) {
-    const double r = block_sum(acc[j], sh);
-    if (threadIdx.x == 0) red[((size_t)k * nb + blockIdx.x) * 16 + j] = r;
+  // all `count` products reduced together: wave sums on DPP / permlane swaps (chains interleaved),
+  // one barrier, then thread j adds the wave totals in wave order
+#pragma unroll
+  for (int j = 0; j < 16; ++j)
+    if (j < count) {  // uniform
+      const double w = wave64(acc[j]);
+      if (lane == 0) sh[j][wave] = w;
+    }
+  __syncthreads();
+  if ((int)threadIdx.x < count) {
+    double r = 0.0;
+#pragma unroll
+    for (int w = 0; w < kEl / 64; ++w) r += sh[threadIdx.x][w];
+    red[((size_t)k * nb + blockIdx.x) * 16 + threadIdx.x] = r;
   }
 }
 
