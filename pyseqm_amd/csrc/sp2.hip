@@ -31,10 +31,9 @@ namespace {
 // next step (1: X^2, 0: 2X - X^2), stop flag, iteration count
 enum { S_TR = 0, S_FRO, S_E0, S_E1, S_E2, S_BRANCH, S_DONE, S_IT, S_NST };
 
-__device__ inline double wsum(double v) {
-  for (int o = 1; o < 64; o <<= 1) v += __shfl_xor(v, o);
-  return v;
-}
+// wave sum on DPP / permlane swaps: the pairings of the xor tree, so the same bits, without LDS
+// round trips (every call site has its whole wave active)
+__device__ inline double wsum(double v) { return wave64(v); }
 __device__ inline double wmax(double v) {
   for (int o = 1; o < 64; o <<= 1) v = fmax(v, __shfl_xor(v, o));
   return v;
